@@ -1,0 +1,288 @@
+"""Benchmark: Two-Choice Filter bulk insert + query at 90 % load on one MI355X.
+
+Workload (one "step"): a 2^28-slot TCF (16-bit fingerprints + 16-bit values,
+16-slot buckets = 64 B, two choices from one fmix64 hash split into 32-bit
+halves) with a 1 % linear backing level is cleared, filled to 90 % of its
+primary slots (241 591 910 distinct uniform uint64 keys, batches of 2^24),
+then queried with 2^26 inserted and 2^26 never-inserted keys.  All inputs are
+generated on the GPU before timing and stay resident in HBM.
+
+Prints one JSON line (the bench contract).  `value` is whole-step throughput
+in Mops/s (inserts + queries per second).  `roofline` is for the dominant
+kernel (insert), with algorithmic bytes counted by the kernels' own
+accounting counters in an untimed pass; `cpu_baseline` times the CPU oracle
+on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--log2-slots", type=int, default=28)
+    ap.add_argument("--tile", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-roofline", action="store_true", help="skip the random-sector microbenchmark")
+    return ap.parse_args()
+
+
+def load_traffic(path):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE), committed as a small JSON summary; None when absent."""
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(log2_slots=26):
+    """CPU oracle (single thread) on a bounded sample: a 2^26-slot TCF with the
+    same shape, filled to 90 % and queried with 2^24 present + 2^24 absent keys."""
+    import oracle
+    oracle.build()
+    nslots = 1 << log2_slots
+    levels = [oracle.level(nslots, oracle.SCHEME_SHORTCUT, 16),
+              oracle.level(nslots // 100, oracle.SCHEME_LINEAR_BUCKETS, 8, prober=oracle.PROBE_LINEAR,
+                           max_probes=4000)]
+    t = oracle.Table(oracle.REP_FP16_V16, levels, seed=42)
+    n = int(0.9 * nslots)
+    keys = oracle.gen_keys(1234, 0, n)
+    nq = 1 << (log2_slots - 2)
+    absent = oracle.gen_keys(1234, 1 << 40, nq)
+    t0 = time.perf_counter()
+    t.bulk_insert(keys)
+    t1 = time.perf_counter()
+    t.bulk_query(keys[:nq])
+    t.bulk_query(absent)
+    t2 = time.perf_counter()
+    ops = n + 2 * nq
+    return {
+        "value": round(ops / (t2 - t0) / 1e6, 3),
+        "unit": "Mops/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"TCF 2^{log2_slots} slots + 1% backing, {n} inserts to 90% + {nq} present + {nq} "
+                  f"absent queries, single-threaded oracle",
+        "insert_mops": round(n / (t1 - t0) / 1e6, 3),
+        "query_mops": round(2 * nq / (t2 - t1) / 1e6, 3),
+        "seconds": round(t2 - t0, 2),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU: run with --gpus 1 (multi-GPU sharding "
+                                   "is a design note only)"}))
+        sys.exit(2)
+
+    import torch
+    import poggers_amd as pg
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+
+    nslots = 1 << args.log2_slots
+    backing = nslots // 100
+    desc = pg.tcf_desc(nslots, backing, bucket_size=16, tile=args.tile, prober=pg.PROBE_SPLIT64,
+                       hasher=pg.HASH_FMIX64)
+    table = pg.Table(desc, seed=42)
+    n_ins = int(0.9 * nslots)
+    batch = 1 << 24 if args.log2_slots >= 26 else max(1, n_ins // 8)
+    nq = 1 << (args.log2_slots - 2)
+    seed = 1234
+    keys = torch.empty(n_ins, dtype=torch.int64, device=dev)
+    absent = torch.empty(nq, dtype=torch.int64, device=dev)
+    pg.gen_keys(seed, 0, n_ins, keys)
+    pg.gen_keys(seed, 1 << 40, nq, absent)
+    present = keys[:nq]
+    found_p = torch.empty(nq, dtype=torch.uint8, device=dev)
+    found_a = torch.empty(nq, dtype=torch.uint8, device=dev)
+    vals_p = torch.empty(nq, dtype=torch.int16, device=dev)
+    vals_a = torch.empty(nq, dtype=torch.int16, device=dev)
+    fails = torch.zeros(1, dtype=torch.int64, device=dev)
+    batches = [(s, min(s + batch, n_ins)) for s in range(0, n_ins, batch)]
+    torch.cuda.synchronize()
+
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    acc = {"insert": 0.0, "query_present": 0.0, "query_absent": 0.0, "clear": 0.0}
+    launches = {"insert": 0, "query_present": 0, "query_absent": 0}
+
+    def step(timed):
+        e = [ev() for _ in range(2 * len(batches) + 6)]
+        k = 0
+        e[k].record(stream); table.clear(stream); e[k + 1].record(stream); k += 2
+        for (s, t) in batches:
+            e[k].record(stream)
+            table.bulk_insert(keys[s:t], None, None, fails, stream)
+            e[k + 1].record(stream)
+            k += 2
+        e[k].record(stream); table.bulk_query(present, vals_p, found_p, stream); e[k + 1].record(stream)
+        e[k + 2].record(stream); table.bulk_query(absent, vals_a, found_a, stream); e[k + 3].record(stream)
+        return e
+
+    def collect(e):
+        torch.cuda.synchronize()
+        acc["clear"] += e[0].elapsed_time(e[1])
+        k = 2
+        for _ in batches:
+            acc["insert"] += e[k].elapsed_time(e[k + 1])
+            launches["insert"] += 1
+            k += 2
+        acc["query_present"] += e[k].elapsed_time(e[k + 1])
+        acc["query_absent"] += e[k + 2].elapsed_time(e[k + 3])
+        launches["query_present"] += 1
+        launches["query_absent"] += 1
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    fails.zero_()
+    for key in acc:
+        acc[key] = 0.0
+    events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        events.append(step(True))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for e in events:
+        collect(e)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # correctness of the last timed step
+    false_neg = int((found_p == 0).sum().item())
+    fp_rate = float(found_a.float().mean().item())
+    insert_fails = int(fails.item())
+
+    # accounting pass (untimed): algorithmic bucket examinations per phase
+    stats = torch.zeros(3, dtype=torch.int64, device=dev)
+    table.set_stats(stats)
+    table.clear(stream)
+    for (s, t) in batches:
+        table.bulk_insert(keys[s:t], None, None, None, stream)
+    torch.cuda.synchronize()
+    st_ins = [int(x) for x in stats.tolist()]
+    stats.zero_()
+    table.bulk_query(present, vals_p, found_p, stream)
+    torch.cuda.synchronize()
+    st_qp = [int(x) for x in stats.tolist()]
+    stats.zero_()
+    table.bulk_query(absent, vals_a, found_a, stream)
+    torch.cuda.synchronize()
+    st_qa = [int(x) for x in stats.tolist()]
+    table.set_stats(None)
+
+    b0, b1, wsec = 64, 32, 64  # bytes per primary bucket, backing bucket, dirtied sector
+    bytes_ins = st_ins[0] * b0 + st_ins[1] * b1 + st_ins[2] * wsec
+    bytes_qp = st_qp[0] * b0 + st_qp[1] * b1
+    bytes_qa = st_qa[0] * b0 + st_qa[1] * b1
+
+    def kinfo(name, nops, nbytes):
+        ms = acc[name] / args.steps
+        gbs = nbytes / (ms / 1e3) / 1e9
+        return {"ms": round(ms, 4), "mops": round(nops / (ms / 1e3) / 1e6, 1),
+                "algorithmic_bytes": nbytes, "bytes_per_op": round(nbytes / nops, 2),
+                "achieved_gbs": round(gbs, 1), "frac_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+
+    kernels = {
+        "insert": kinfo("insert", n_ins, bytes_ins),
+        "query_present": kinfo("query_present", nq, bytes_qp),
+        "query_absent": kinfo("query_absent", nq, bytes_qa),
+    }
+
+    # measured random 64 B-sector roofline over a buffer of the table's footprint
+    rand = None
+    if not args.no_roofline:
+        foot = nslots * 4
+        buf = torch.zeros(foot // 8, dtype=torch.int64, device=dev)
+        sink = torch.zeros(1, dtype=torch.int64, device=dev)
+        rand = {}
+        for op, name in ((0, "read"), (1, "read_cas")):
+            n_reads = 1 << 26
+            pg.random_sector_bench(buf, n_reads, op, 7, sink, stream)
+            e0, e1 = ev(), ev()
+            reps = 3
+            e0.record(stream)
+            for r in range(reps):
+                pg.random_sector_bench(buf, n_reads, op, 7 + r, sink, stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            rand[name] = round(n_reads * 64 / (ms / 1e3) / 1e9, 1)
+        del buf
+        kernels["insert"]["frac_random_roofline"] = round(kernels["insert"]["achieved_gbs"] / rand["read_cas"], 4)
+        for q in ("query_present", "query_absent"):
+            kernels[q]["frac_random_roofline"] = round(kernels[q]["achieved_gbs"] / rand["read"], 4)
+
+    dom = max(("insert", "query_present", "query_absent"), key=lambda k: acc[k])
+    traffic_info = load_traffic(os.path.join(ROOT, "profiles", "tcf_2p28_traffic.json"))
+    traffic = None
+    if traffic_info and traffic_info.get("kernel") == dom:
+        traffic = traffic_info.get("hbm_bytes_per_launch")
+    roofline = {
+        "bound": "hbm",
+        "achieved": kernels[dom]["achieved_gbs"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": kernels[dom]["frac_hbm_peak"],
+        "traffic": traffic,
+        "kernel": dom,
+        "algorithmic_bytes_per_launch": kernels[dom]["algorithmic_bytes"] // max(1, launches[dom] // args.steps),
+        "avg_launch_ms": round(acc[dom] / max(1, launches[dom]), 4),
+        "measured_random_64B_gbs": rand,
+    }
+
+    ops_per_step = n_ins + 2 * nq
+    value = ops_per_step / (ms_per_step / 1e3) / 1e6
+    out = {
+        "metric": "TCF insert/query Mops/s at 90% load",
+        "value": round(value, 1),
+        "unit": "Mops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (splitmix64 uint64 keys generated on the GPU)",
+        "config": {
+            "workload": f"TCF 2^{args.log2_slots} slots (16-bit fp, 16-slot buckets, split-fmix64 two choices, "
+                        f"tile {args.tile}) + 1% linear backing; insert {n_ins} keys (90%) in batches of "
+                        f"{batch}, query {nq} present + {nq} absent",
+            "slots": nslots, "backing_slots": backing, "inserts": n_ins, "queries": 2 * nq,
+        },
+        "insert_mops": kernels["insert"]["mops"],
+        "query_mops": round(2 * nq / ((acc["query_present"] + acc["query_absent"]) / args.steps / 1e3) / 1e6, 1),
+        "false_negatives": false_neg,
+        "fp_rate": fp_rate,
+        "insert_failures": insert_fails,
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,222 @@
+/*
+ * poggers_amd.h -- C ABI of the MI355X (gfx950) bulk hash-table / Two-Choice
+ * Filter library.
+ *
+ * The reference (huntermBerkeley/poggers) is a header-only CUDA template
+ * library with no host bulk API: callers compose
+ *     bucketed_table<Key, Val, Rep, P, B, InsertScheme, MaxProbes, Prober,
+ *                    Hasher, IsRecursive, Backing>
+ * (tables/bucketed_table.cuh:72) and write their own __global__ kernels
+ * that call the per-tile device ops (tests/src/test_mhm_tcf.cu:318-487).
+ * This ABI replaces that composition with a runtime descriptor and bulk,
+ * stream-ordered entry points.  Each entry point cites the reference
+ * interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status (PG_OK == 0); no exception
+ *     crosses the ABI.  pg_error_string() names a status.
+ *   - Key, value and result buffers are caller-owned DEVICE pointers
+ *     (the reference tests pass cudaMalloc'ed arrays the same way,
+ *     tests/src/test_mhm_tcf.cu:616-621).  The handle owns table memory.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *     Calls are asynchronous and stream-ordered, except pg_create,
+ *     pg_destroy and pg_fill, which synchronise.
+ *   - A handle is not safe for concurrent host calls without external
+ *     ordering.  Device-side ops within one call are lock-free per slot.
+ *   - Keys 0 and ~0 are reserved (empty / tombstone of the KV slot,
+ *     representations/key_val_pair.cuh:70,115-117); filters remap their
+ *     fingerprints so any key is accepted there.
+ */
+#ifndef POGGERS_AMD_H
+#define POGGERS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PG_MAX_LEVELS 2
+
+/* status codes */
+enum pg_status {
+    PG_OK = 0,
+    PG_ERR_INVALID = 1,      /* bad descriptor / argument */
+    PG_ERR_UNSUPPORTED = 2,  /* op or shape not offered by this scheme */
+    PG_ERR_NOMEM = 3,        /* device allocation failed */
+    PG_ERR_HIP = 4,          /* a HIP runtime call failed */
+    PG_ERR_NODEVICE = 5      /* no usable GPU */
+};
+
+/* Slot representation.
+ *   KV64      key_val_pair<uint64_t,uint64_t>: 16 B {key, val}
+ *             (representations/key_val_pair.cuh:63-171)
+ *   FP16_V16  dynamic_container<bit_grouped_container<16,16>, uint16_t>:
+ *             one uint32 word = val16 << 16 | fp16
+ *             (representations/dynamic_container.cuh:23-141,
+ *              grouped_storage_sub_bits.cuh:99-260)                      */
+enum pg_rep { PG_REP_KV64 = 0, PG_REP_FP16_V16 = 1 };
+
+/* Insert scheme of a level.
+ *   BUCKET        insert_schemes/bucket_insert.cuh:38-361
+ *   SINGLE_SLOT   insert_schemes/single_slot_insert.cuh:38-233 (no remove)
+ *   POWER_OF_N_SHORTCUT  insert_schemes/power_of_n_shortcut_buckets.cuh:35-783
+ *                 (the TCF primary: two choices + 75 % shortcut)
+ *   LINEAR_BUCKETS insert_schemes/linear_insert_buckets.cuh:35-445
+ *                 (the TCF backing table)                                */
+enum pg_scheme {
+    PG_SCHEME_BUCKET = 0,
+    PG_SCHEME_SINGLE_SLOT = 1,
+    PG_SCHEME_POWER_OF_N_SHORTCUT = 2,
+    PG_SCHEME_LINEAR_BUCKETS = 3
+};
+
+/* Probing scheme (probing_schemes/*.cuh).  Probe i of a key, before the
+ * consumer's "% num_buckets":
+ *   LINEAR   h + i                       (linear_probing.cuh:72-78)
+ *   DOUBLE   h1 + i*h2, h2 seeded seed^2 (double_hashing.cuh:66-89)
+ *   P2       {h1, h2}                    (power_of_two.cuh:68-91)
+ *   XOR_P2   {h1, h1 ^ (tag*0x5bd1e995)} (xor_power_of_two.cuh:64-84)
+ *   SPLIT64  {lo32(h), hi32(h)}: one 64-bit hash split into halves
+ *            (no reference counterpart)                                  */
+enum pg_prober {
+    PG_PROBE_LINEAR = 0,
+    PG_PROBE_DOUBLE = 1,
+    PG_PROBE_P2 = 2,
+    PG_PROBE_XOR_P2 = 3,
+    PG_PROBE_SPLIT64 = 4
+};
+
+/* Hasher: FMIX64 = MurmurHash3 fmix64(key ^ seed);
+ *         MURMUR64A = MurmurHash64A(&key, 8, seed)
+ *         (hash_schemes/murmurhash.cuh:20-100)                          */
+enum pg_hasher { PG_HASH_FMIX64 = 0, PG_HASH_MURMUR64A = 1 };
+
+/* Execution mode (pg_set_mode). */
+enum pg_mode {
+    PG_CONCURRENT = 0,    /* one tile per key, all keys in flight */
+    PG_DETERMINISTIC = 1  /* keys applied in array order by one tile: the
+                             slot layout equals the sequential oracle's */
+};
+
+/* One level of a (possibly recursive) table.  Mirrors one instantiation of
+ * the reference's Insert_Scheme<Key, Val, Partition_Size, Bucket_Size,
+ * Internal_Rep, Max_Probes, Hasher, Probing_Scheme>.                     */
+typedef struct pg_level_desc {
+    uint64_t nslots;       /* size_in_num_slots entry (sizing/default_sizing.cuh:30-95) */
+    uint32_t scheme;       /* enum pg_scheme */
+    uint32_t bucket_size;  /* Bucket_Size (slots per bucket) */
+    uint32_t tile;         /* Partition_Size: lanes per key, 1..64, power of two */
+    uint32_t prober;       /* enum pg_prober */
+    uint32_t hasher;       /* enum pg_hasher */
+    uint32_t max_probes;   /* Max_Probes */
+} pg_level_desc;
+
+/* A table: level 0 plus an optional backing level (Is_Recursive /
+ * Recursive_Type, tables/bucketed_table.cuh:72,166-176).  The backing level
+ * is seeded with seed*seed (bucketed_table.cuh:170).                     */
+typedef struct pg_desc {
+    uint32_t rep;          /* enum pg_rep */
+    uint32_t num_levels;   /* 1 or 2 */
+    pg_level_desc level[PG_MAX_LEVELS];
+} pg_desc;
+
+typedef struct pg_table_s *pg_handle;
+
+/* Replaces bucketed_table::generate_on_device(Sizing*, seed)
+ * (tables/bucketed_table.cuh:132-192) and static_table::generate_on_device
+ * (tables/base_table.cuh:110-145).  Allocates and zeroes the slot arrays. */
+int pg_create(const pg_desc *desc, uint64_t seed, pg_handle *out);
+
+/* Replaces free_on_device (tables/bucketed_table.cuh:194-224). */
+int pg_destroy(pg_handle h);
+
+/* Zero every slot (a fill kernel, safe to record in a hipGraph). */
+int pg_clear(pg_handle h, void *stream);
+
+/* Select PG_CONCURRENT or PG_DETERMINISTIC for later mutating calls.
+ * The reference's only ordered mode is a single-tile launch
+ * (tests/src/test_mhm_tcf.cu:265-314,848). */
+int pg_set_mode(pg_handle h, uint32_t mode);
+
+/* Bulk insert: replaces the per-key device op insert(tile, key, val)
+ * (tables/bucketed_table.cuh:271-291) driven by speed_insert_kernel
+ * (tests/src/test_mhm_tcf.cu:318-345).
+ *   d_vals       uint64 per key (KV64) or uint16 per key (FP16_V16);
+ *                NULL inserts value 0 (bucketed_table.cuh:324-329).
+ *   d_status     optional, one byte per key: 0 failed, 1 + level index.
+ *   d_fail_count optional uint64 device counter, incremented per failure
+ *                (the reference's `misses` counter). */
+int pg_bulk_insert(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n,
+                   uint8_t *d_status, uint64_t *d_fail_count, void *stream);
+
+/* Bulk insert that may reuse tombstones: replaces insert_with_delete
+ * (tables/bucketed_table.cuh:293-321; power_of_n_shortcut_buckets.cuh:
+ * 309-417; linear_insert_buckets.cuh:223-244).  Filters only. */
+int pg_bulk_insert_with_delete(pg_handle h, const uint64_t *d_keys, const void *d_vals,
+                               uint64_t n, uint8_t *d_status, uint64_t *d_fail_count,
+                               void *stream);
+
+/* Bulk query: replaces query(tile, key, Val&) (bucketed_table.cuh:332-349)
+ * driven by speed_query_kernel (tests/src/test_mhm_tcf.cu:456-487).
+ *   d_vals_out optional; value per key (0 when absent).
+ *   d_found    one byte per key. */
+int pg_bulk_query(pg_handle h, const uint64_t *d_keys, uint64_t n, void *d_vals_out,
+                  uint8_t *d_found, void *stream);
+
+/* Bulk remove: replaces remove(tile, key) (bucketed_table.cuh:351-381)
+ * driven by speed_delete_kernel (tests/src/delete_tests.cu:335-374).
+ *   d_found optional, one byte per key: 1 if a slot was tombstoned. */
+int pg_bulk_remove(pg_handle h, const uint64_t *d_keys, uint64_t n, uint8_t *d_found,
+                   void *stream);
+
+/* Filled slots over all levels: replaces get_fill() and count_fill_kernel
+ * (tables/bucketed_table.cuh:36-52,498-548).  Synchronises `stream`;
+ * *out is a HOST pointer. */
+int pg_fill(pg_handle h, uint64_t *out, void *stream);
+
+/* Bytes of slot memory over all levels: replaces host_bytes_in_use
+ * (tables/bucketed_table.cuh:405-425). */
+int pg_bytes(pg_handle h, uint64_t *out);
+
+/* Slots over all levels: replaces get_num_slots (bucketed_table.cuh:462-489). */
+int pg_num_slots(pg_handle h, uint64_t *out);
+
+/* Device pointer and byte size of one level's slot array (raw bucket dump
+ * for layout checks).  No reference counterpart: the reference never
+ * exposes its slot arrays outside the device structs. */
+int pg_level_ptr(pg_handle h, uint32_t level, void **d_ptr, uint64_t *bytes);
+
+/* Human-readable name of a status code. */
+const char *pg_error_string(int status);
+
+/* ---- measurement helpers (no reference counterpart) ---- */
+
+/* Fill d_out[i] = splitmix64 key i+start of stream `seed` (the seeded key
+ * streams every benchmark input is drawn from). */
+int pg_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *d_out, void *stream);
+
+/* Random 64 B-sector read microbenchmark over d_buf (bytes, multiple of
+ * 64): n_reads independent reads of 64 B at hashed sector indices, by
+ * 4-lane tiles of 16 B per lane, each tile reading `items` sectors.
+ * `op` 0 = read, 1 = read + CAS one word of the sector.  Writes one
+ * checksum word to d_sink so nothing is optimised away. */
+int pg_random_sector_bench(void *d_buf, uint64_t bytes, uint64_t n_reads, uint32_t op,
+                           uint64_t seed, uint64_t *d_sink, void *stream);
+
+/* Accounting: while d_stats (3 device uint64 counters) is set, bulk ops add
+ * [0] level-0 buckets examined (KV: slots examined), [1] backing-level
+ * buckets examined, [2] slots written.  These give the algorithmic bytes
+ * of a call (each bucket examination reads B slot words; each write dirties
+ * one 64 B sector).  NULL disables it.  No reference counterpart. */
+int pg_set_stats(pg_handle h, uint64_t *d_stats);
+
+/* Host-side range reduction used by every probe (exported for tests):
+ * returns n % d computed with the library's invariant-divisor method. */
+uint64_t pg_fastmod_host(uint64_t n, uint64_t d);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* POGGERS_AMD_H */

@@ -1,0 +1,876 @@
+/*
+ * pg_oracle.c -- single-threaded CPU restatement of poggers' bulk
+ * insert / query / remove path for the modular hash tables and the
+ * Two-Choice Filter (TCF).
+ *
+ * THIS IS TEST INFRASTRUCTURE.  It is the checker that the HIP path is
+ * compared against (and the CPU baseline that bench.py times).  Nothing in
+ * the product library links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference's include/poggers/ directory unless marked tests/).
+ *
+ * Pinning (what checks each function, see tests/test_oracle_*.py):
+ *   or_murmur64a  -- SMHasher verification code 0x1F0D3804 and the recorded
+ *                    value MurmurHash64A(1, seed 42) = 0x63cc585a9ca03f57.
+ *   or_fmix64     -- SMHasher verification code of MurmurHash3_x64_128
+ *                    (0x6384BA69, built on this finalizer) and inversion by
+ *                    an independently derived inverse.
+ *   probes        -- recorded double-hash probes of key 1 (seed 42) and the
+ *                    closed forms of each scheme.
+ *   slot formats  -- exhaustive pack/unpack round trip (as the reference's
+ *                    tests/src/kmer_speed_tests.cu:664-684) and the
+ *                    known answers of tests/src/test_key_val_pair.cu:50-62.
+ *   TCF insert    -- hand-derived placement scenarios (shortcut threshold,
+ *                    fill-ordered retry, tie order, backing spill), zero
+ *                    false negatives, FP-rate closed form.
+ *   KV insert     -- tests/src/test_components.cu:88-127 and :345-366 known
+ *                    answers, linear-probing closed-form probe lengths,
+ *                    brute force on tiny tables.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_END (~(uint64_t)0)
+
+/* ---- enums (values mirror include/poggers_amd.h so tests can share one
+ *      descriptor layout; the oracle does not include that header) ---- */
+enum { OR_REP_KV64 = 0, OR_REP_FP16_V16 = 1, OR_REP_FP8_V8 = 2 };
+enum { OR_SCHEME_BUCKET = 0, OR_SCHEME_SINGLE_SLOT = 1,
+       OR_SCHEME_POWER_OF_N_SHORTCUT = 2, OR_SCHEME_LINEAR_BUCKETS = 3 };
+enum { OR_PROBE_LINEAR = 0, OR_PROBE_DOUBLE = 1, OR_PROBE_P2 = 2,
+       OR_PROBE_XOR_P2 = 3, OR_PROBE_SPLIT64 = 4 };
+enum { OR_HASH_FMIX64 = 0, OR_HASH_MURMUR64A = 1 };
+
+/* ------------------------------------------------------------------ */
+/* Hashers                                                             */
+/* ------------------------------------------------------------------ */
+
+/* Austin Appleby's MurmurHash64A over len bytes.
+ * Follows hash_schemes/murmurhash.cuh:20-61. */
+uint64_t or_murmur64a(const void *key, int len, uint64_t seed)
+{
+    const uint64_t m = 0xc6a4a7935bd1e995ULL;
+    const int r = 47;
+    uint64_t h = seed ^ ((uint64_t)len * m);
+    const unsigned char *p = (const unsigned char *)key;
+    int nblocks = len / 8;
+    for (int b = 0; b < nblocks; b++) {
+        uint64_t k;
+        memcpy(&k, p + 8 * b, 8);
+        k *= m;
+        k ^= k >> r;
+        k *= m;
+        h ^= k;
+        h *= m;
+    }
+    const unsigned char *tail = p + 8 * nblocks;
+    switch (len & 7) {
+    case 7: h ^= (uint64_t)tail[6] << 48; /* fall through */
+    case 6: h ^= (uint64_t)tail[5] << 40; /* fall through */
+    case 5: h ^= (uint64_t)tail[4] << 32; /* fall through */
+    case 4: h ^= (uint64_t)tail[3] << 24; /* fall through */
+    case 3: h ^= (uint64_t)tail[2] << 16; /* fall through */
+    case 2: h ^= (uint64_t)tail[1] << 8;  /* fall through */
+    case 1: h ^= (uint64_t)tail[0];
+            h *= m;
+    }
+    h ^= h >> r;
+    h *= m;
+    h ^= h >> r;
+    return h;
+}
+
+/* MurmurHash3's 64-bit finalizer (SMHasher, MurmurHash3.cpp fmix64).
+ * The reference has no counterpart (its hasher is MurmurHash64A); this is
+ * the hasher the MI355X build uses by default. */
+uint64_t or_fmix64(uint64_t k)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ULL;
+    k ^= k >> 33;
+    return k;
+}
+
+/* MurmurHash3_x64_128 (SMHasher), only used to pin or_fmix64 through the
+ * published verification code of that function. */
+static uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+
+void or_murmur3_x64_128(const void *key, int len, uint32_t seed, void *out)
+{
+    const unsigned char *data = (const unsigned char *)key;
+    const int nblocks = len / 16;
+    uint64_t h1 = seed, h2 = seed;
+    const uint64_t c1 = 0x87c37b91114253d5ULL;
+    const uint64_t c2 = 0x4cf5ad432745937fULL;
+    for (int i = 0; i < nblocks; i++) {
+        uint64_t k1, k2;
+        memcpy(&k1, data + 16 * i, 8);
+        memcpy(&k2, data + 16 * i + 8, 8);
+        k1 *= c1; k1 = rotl64(k1, 31); k1 *= c2; h1 ^= k1;
+        h1 = rotl64(h1, 27); h1 += h2; h1 = h1 * 5 + 0x52dce729;
+        k2 *= c2; k2 = rotl64(k2, 33); k2 *= c1; h2 ^= k2;
+        h2 = rotl64(h2, 31); h2 += h1; h2 = h2 * 5 + 0x38495ab5;
+    }
+    const unsigned char *tail = data + nblocks * 16;
+    uint64_t k1 = 0, k2 = 0;
+    switch (len & 15) {
+    case 15: k2 ^= (uint64_t)tail[14] << 48; /* fall through */
+    case 14: k2 ^= (uint64_t)tail[13] << 40; /* fall through */
+    case 13: k2 ^= (uint64_t)tail[12] << 32; /* fall through */
+    case 12: k2 ^= (uint64_t)tail[11] << 24; /* fall through */
+    case 11: k2 ^= (uint64_t)tail[10] << 16; /* fall through */
+    case 10: k2 ^= (uint64_t)tail[9] << 8;   /* fall through */
+    case 9:  k2 ^= (uint64_t)tail[8];
+             k2 *= c2; k2 = rotl64(k2, 33); k2 *= c1; h2 ^= k2;
+             /* fall through */
+    case 8:  k1 ^= (uint64_t)tail[7] << 56; /* fall through */
+    case 7:  k1 ^= (uint64_t)tail[6] << 48; /* fall through */
+    case 6:  k1 ^= (uint64_t)tail[5] << 40; /* fall through */
+    case 5:  k1 ^= (uint64_t)tail[4] << 32; /* fall through */
+    case 4:  k1 ^= (uint64_t)tail[3] << 24; /* fall through */
+    case 3:  k1 ^= (uint64_t)tail[2] << 16; /* fall through */
+    case 2:  k1 ^= (uint64_t)tail[1] << 8;  /* fall through */
+    case 1:  k1 ^= (uint64_t)tail[0];
+             k1 *= c1; k1 = rotl64(k1, 31); k1 *= c2; h1 ^= k1;
+    }
+    h1 ^= (uint64_t)len; h2 ^= (uint64_t)len;
+    h1 += h2; h2 += h1;
+    h1 = or_fmix64(h1); h2 = or_fmix64(h2);
+    h1 += h2; h2 += h1;
+    memcpy(out, &h1, 8);
+    memcpy((unsigned char *)out + 8, &h2, 8);
+}
+
+/* Seeded 64-bit key hash used by every probing scheme.
+ *   MURMUR64A: murmurHasher<uint64_t>::hash, hash_schemes/murmurhash.cuh:93-100
+ *              (MurmurHash64A over the 8 key bytes with the stored seed).
+ *   FMIX64:    fmix64(key XOR seed). */
+uint64_t or_hash(uint32_t hasher, uint64_t key, uint64_t seed)
+{
+    if (hasher == OR_HASH_MURMUR64A)
+        return or_murmur64a(&key, 8, seed);
+    return or_fmix64(key ^ seed);
+}
+
+/* ------------------------------------------------------------------ */
+/* Probing schemes: produce the raw 64-bit probe values (before the    */
+/* "% num_buckets" the consumer applies).  The reference iterates     */
+/*   for (p = begin(key); p != end(); p = next(tag))                   */
+/* with end() == ~0, so a probe value equal to ~0 ends the sequence.   */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint32_t prober, max_probes;
+    uint32_t i;
+    uint64_t h1, h2, tag;
+} or_probe_it;
+
+/* begin(): probing_schemes/double_hashing.cuh:72-81 (h1 = H_seed, h2 =
+ * H_seed^2), xor_power_of_two.cuh:69-76, linear_probing.cuh:64-70,
+ * power_of_two.cuh:74-83.  SPLIT64 has no reference counterpart: one
+ * 64-bit hash split into its low and high 32-bit halves. */
+static uint64_t probe_begin(or_probe_it *it, uint32_t prober, uint32_t hasher,
+                            uint32_t max_probes, uint64_t seed, uint64_t key,
+                            uint64_t tag)
+{
+    it->prober = prober;
+    it->max_probes = max_probes;
+    it->i = 0;
+    it->tag = tag;
+    it->h1 = or_hash(hasher, key, seed);
+    it->h2 = 0;
+    if (prober == OR_PROBE_DOUBLE || prober == OR_PROBE_P2)
+        it->h2 = or_hash(hasher, key, seed * seed);
+    if (max_probes == 0)
+        return OR_END;
+    if (prober == OR_PROBE_SPLIT64)
+        return it->h1 & 0xffffffffULL;
+    return it->h1;
+}
+
+/* next(): double_hashing.cuh:83-89 (h1 + i*h2), xor_power_of_two.cuh:78-84
+ * (h1 ^ (tag * 0x5bd1e995)), linear_probing.cuh:72-78 (h + i),
+ * power_of_two.cuh:85-91 (h2). */
+static uint64_t probe_next(or_probe_it *it)
+{
+    it->i += 1;
+    if (it->i >= it->max_probes)
+        return OR_END;
+    switch (it->prober) {
+    case OR_PROBE_LINEAR: return it->h1 + (uint64_t)it->i;
+    case OR_PROBE_DOUBLE: return it->h1 + (uint64_t)it->i * it->h2;
+    case OR_PROBE_P2:     return it->h2;
+    case OR_PROBE_XOR_P2: return it->h1 ^ (it->tag * 0x5bd1e995ULL);
+    case OR_PROBE_SPLIT64: return it->h1 >> 32;
+    }
+    return OR_END;
+}
+
+/* Test hook: write up to cap raw probe values, return their count. */
+int or_probes(uint32_t prober, uint32_t hasher, uint32_t max_probes,
+              uint64_t seed, uint64_t key, uint64_t tag, uint64_t *out, int cap)
+{
+    or_probe_it it;
+    int n = 0;
+    for (uint64_t p = probe_begin(&it, prober, hasher, max_probes, seed, key, tag);
+         p != OR_END && n < cap; p = probe_next(&it))
+        out[n++] = p;
+    return n;
+}
+
+/* ------------------------------------------------------------------ */
+/* Slot encodings                                                      */
+/* ------------------------------------------------------------------ */
+
+/* Fingerprint: the low bits of the RAW key, 0 -> 1, all-ones -> all-ones-1.
+ * representations/dynamic_container.cuh:35-50. */
+static uint32_t fp_of(uint32_t rep, uint64_t key)
+{
+    if (rep == OR_REP_FP8_V8) {
+        uint32_t f = (uint8_t)key;
+        if (f == 0) f += 1;
+        if (f == 0xFF) f -= 1;
+        return f;
+    }
+    uint32_t f = (uint16_t)key;
+    if (f == 0) f += 1;
+    if (f == 0xFFFF) f -= 1;
+    return f;
+}
+
+/* grouped_bits_pair<K,V,kb,vb>: word = (val & vmask) << kb | (fp & kmask).
+ * representations/grouped_storage_sub_bits.cuh:77-95 (join_in_storage),
+ * :35-74 (retrieve_key/val_from_storage). */
+uint32_t or_pack(uint32_t kbits, uint32_t vbits, uint32_t fp, uint32_t val)
+{
+    uint32_t vm = (vbits >= 32) ? 0xffffffffu : ((1u << vbits) - 1u);
+    uint32_t km = (kbits >= 32) ? 0xffffffffu : ((1u << kbits) - 1u);
+    return ((val & vm) << kbits) | (fp & km);
+}
+uint32_t or_unpack_fp(uint32_t kbits, uint32_t w) { return w & ((1u << kbits) - 1u); }
+uint32_t or_unpack_val(uint32_t kbits, uint32_t vbits, uint32_t w)
+{
+    return (w >> kbits) & ((1u << vbits) - 1u);
+}
+
+/* ------------------------------------------------------------------ */
+/* Table state                                                         */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint64_t nslots;
+    uint32_t scheme, bucket_size, tile, prober, hasher, max_probes;
+} or_level_desc;
+
+typedef struct {
+    or_level_desc d;
+    uint64_t seed;
+    uint64_t nb;        /* buckets (for SINGLE_SLOT: slots) */
+    uint32_t B;         /* slots per bucket */
+    void *slots;        /* uint32_t[] (FP16_V16), uint16_t[] (FP8_V8), uint64_t[2*] (KV64) */
+    uint64_t nslot_alloc;
+} or_level;
+
+typedef struct {
+    uint32_t rep;
+    uint32_t nlevels;
+    or_level lv[2];
+    /* algorithmic accounting, reset by or_stats_reset */
+    uint64_t sectors_read;     /* distinct 64 B sectors examined */
+    uint64_t sectors_written;  /* 64 B sectors dirtied */
+    uint64_t buckets_read[2];  /* distinct bucket examinations per level */
+} or_table;
+
+/* Word access for the packed (filter) representations. */
+static inline uint32_t wget(const or_table *t, const or_level *L, uint64_t i)
+{
+    if (t->rep == OR_REP_FP8_V8) return ((const uint16_t *)L->slots)[i];
+    return ((const uint32_t *)L->slots)[i];
+}
+static inline void wset(const or_table *t, or_level *L, uint64_t i, uint32_t w)
+{
+    if (t->rep == OR_REP_FP8_V8) ((uint16_t *)L->slots)[i] = (uint16_t)w;
+    else ((uint32_t *)L->slots)[i] = w;
+}
+static inline uint32_t kbits_of(const or_table *t) { return t->rep == OR_REP_FP8_V8 ? 8 : 16; }
+static inline uint32_t kmask_of(const or_table *t) { return t->rep == OR_REP_FP8_V8 ? 0xFFu : 0xFFFFu; }
+static inline uint32_t slot_bytes(const or_table *t)
+{
+    return t->rep == OR_REP_KV64 ? 16u : (t->rep == OR_REP_FP8_V8 ? 2u : 4u);
+}
+
+/* Packed-slot predicates: grouped_storage_sub_bits.cuh:160-195.
+ * empty == fp bits 0; tombstone == fp bits all-ones (get_empty()-1). */
+static inline int p_empty(const or_table *t, uint32_t w) { return (w & kmask_of(t)) == 0; }
+static inline int p_tomb(const or_table *t, uint32_t w) { return (w & kmask_of(t)) == kmask_of(t); }
+static inline uint32_t tomb_word(const or_table *t) { return kmask_of(t); } /* join(tombstone, 0) */
+
+/* KV predicates: representations/key_val_pair.cuh:70,111-125 (empty key 0,
+ * tombstone key ~0). */
+#define KV_EMPTY 0ULL
+#define KV_TOMB (~0ULL)
+
+static void account_bucket(or_table *t, int level, uint64_t b)
+{
+    (void)b;
+    t->buckets_read[level] += 1;
+    uint64_t bytes = (uint64_t)t->lv[level].B * slot_bytes(t);
+    t->sectors_read += (bytes + 63) / 64;
+}
+
+/* ------------------------------------------------------------------ */
+/* Bucket routines for packed slots.                                   */
+/* representations/packed_bucket.cuh: the tile scans slot stripes in    */
+/* ascending order and the leader is the lowest lane, so every routine */
+/* acts on the LOWEST-index qualifying slot, whatever the tile width.  */
+/* ------------------------------------------------------------------ */
+
+/* get_fill: slots neither empty nor tombstone. packed_bucket.cuh:40-62 */
+static int pb_fill(const or_table *t, const or_level *L, uint64_t b)
+{
+    int f = 0;
+    for (uint32_t i = 0; i < L->B; i++) {
+        uint32_t w = wget(t, L, b * L->B + i);
+        if (!(p_empty(t, w) || p_tomb(t, w))) f++;
+    }
+    return f;
+}
+
+/* insert: claim the lowest EMPTY slot (CAS from the all-zero word).
+ * packed_bucket.cuh:64-114, grouped_storage_sub_bits.cuh:127-141. */
+static int pb_insert(or_table *t, or_level *L, uint64_t b, uint32_t word)
+{
+    for (uint32_t i = 0; i < L->B; i++) {
+        uint64_t s = b * L->B + i;
+        uint32_t w = wget(t, L, s);
+        if (p_empty(t, w)) {
+            if (w == 0) { /* CAS(0 -> word) */
+                wset(t, L, s, word);
+                t->sectors_written += 1;
+                return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+/* insert_delete: lowest empty-or-tombstone slot; CAS from the empty word or
+ * from the tombstone word.  packed_bucket.cuh:116-180,
+ * grouped_storage_sub_bits.cuh:143-157. */
+static int pb_insert_delete(or_table *t, or_level *L, uint64_t b, uint32_t word)
+{
+    for (uint32_t i = 0; i < L->B; i++) {
+        uint64_t s = b * L->B + i;
+        uint32_t w = wget(t, L, s);
+        if (p_empty(t, w) || p_tomb(t, w)) {
+            uint32_t expect = p_empty(t, w) ? 0u : tomb_word(t);
+            if (w == expect) {
+                wset(t, L, s, word);
+                t->sectors_written += 1;
+                return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+/* query: value of the lowest slot whose fp matches.  packed_bucket.cuh:182-219 */
+static int pb_query(const or_table *t, const or_level *L, uint64_t b, uint32_t fp,
+                    uint32_t *val)
+{
+    for (uint32_t i = 0; i < L->B; i++) {
+        uint32_t w = wget(t, L, b * L->B + i);
+        if ((w & kmask_of(t)) == fp) {
+            *val = w >> kbits_of(t);
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* remove: CAS the lowest matching slot's word to the tombstone word.
+ * packed_bucket.cuh:227-271, grouped_storage_sub_bits.cuh:217-234. */
+static int pb_remove(or_table *t, or_level *L, uint64_t b, uint32_t fp)
+{
+    for (uint32_t i = 0; i < L->B; i++) {
+        uint64_t s = b * L->B + i;
+        uint32_t w = wget(t, L, s);
+        if ((w & kmask_of(t)) == fp) {
+            wset(t, L, s, tomb_word(t));
+            t->sectors_written += 1;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Insert schemes for packed slots                                     */
+/* ------------------------------------------------------------------ */
+#define MAXP 8
+
+/* power_of_n_insert_shortcut_bucket_scheme::insert / insert_with_delete,
+ * insert_schemes/power_of_n_shortcut_buckets.cuh:198-306 and :309-417.
+ *   pass 1: for each probe j (in probe order) b_j = p_j % nb, f_j = fill;
+ *           if f_j < 0.75*B try to insert into b_j (return on success; on
+ *           failure f_j := B).  "fill < B*.75" is evaluated exactly as
+ *           4*fill < 3*B.
+ *   pass 2: for level = min f, ascending through the distinct fills < B,
+ *           try every probe j with f_j == level, in probe order. */
+static int shortcut_insert(or_table *t, or_level *L, uint64_t key, uint32_t word,
+                           int with_delete)
+{
+    uint64_t buckets[MAXP];
+    int fill[MAXP];
+    int np = 0;
+    int B = (int)L->B;
+    int min_fill = B;
+    uint32_t fp = fp_of(t->rep, key);
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, fp);
+         p != OR_END && np < MAXP; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        buckets[np] = b;
+        account_bucket(t, 0, b);
+        int f = pb_fill(t, L, b);
+        if (4 * f < 3 * B) {
+            int ok = with_delete ? pb_insert_delete(t, L, b, word) : pb_insert(t, L, b, word);
+            if (ok) return 1;
+            f = B;
+        }
+        fill[np] = f;
+        if (f < min_fill) min_fill = f;
+        np++;
+    }
+    int lvl = min_fill;
+    while (lvl < B) {
+        int next = B;
+        for (int j = 0; j < np; j++) {
+            if (fill[j] == lvl) {
+                int ok = with_delete ? pb_insert_delete(t, L, buckets[j], word)
+                                     : pb_insert(t, L, buckets[j], word);
+                if (ok) return 1;
+            }
+            if (fill[j] > lvl && fill[j] < next) next = fill[j];
+        }
+        lvl = next;
+    }
+    return 0;
+}
+
+/* power_of_n_shortcut_buckets.cuh:676-707: check every probe bucket in
+ * order, the first match wins (no early exit on empty slots). */
+static int shortcut_query(or_table *t, or_level *L, uint64_t key, uint32_t *val)
+{
+    uint32_t fp = fp_of(t->rep, key);
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, fp);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        account_bucket(t, 0, b);
+        if (pb_query(t, L, b, fp, val)) return 1;
+    }
+    return 0;
+}
+
+/* power_of_n_shortcut_buckets.cuh:709-740 */
+static int shortcut_remove(or_table *t, or_level *L, uint64_t key)
+{
+    uint32_t fp = fp_of(t->rep, key);
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, fp);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        account_bucket(t, 0, b);
+        if (pb_remove(t, L, b, fp)) return 1;
+    }
+    return 0;
+}
+
+/* linear_insert_bucket_scheme, insert_schemes/linear_insert_buckets.cuh:
+ * insert :200-221 (first probe bucket with an empty slot), insert_with_delete
+ * :223-244, query :362-398 and remove :323-360 (stop at the first probe
+ * bucket whose fill != B). */
+static int linear_insert(or_table *t, or_level *L, int lvl, uint64_t key, uint32_t word,
+                         int with_delete)
+{
+    uint32_t fp = fp_of(t->rep, key);
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, fp);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        account_bucket(t, lvl, b);
+        int ok = with_delete ? pb_insert_delete(t, L, b, word) : pb_insert(t, L, b, word);
+        if (ok) return 1;
+    }
+    return 0;
+}
+
+static int linear_query(or_table *t, or_level *L, int lvl, uint64_t key, uint32_t *val)
+{
+    uint32_t fp = fp_of(t->rep, key);
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, fp);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        account_bucket(t, lvl, b);
+        if (pb_query(t, L, b, fp, val)) return 1;
+        if (pb_fill(t, L, b) != (int)L->B) return 0;
+    }
+    return 0;
+}
+
+static int linear_remove(or_table *t, or_level *L, int lvl, uint64_t key)
+{
+    uint32_t fp = fp_of(t->rep, key);
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, fp);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        account_bucket(t, lvl, b);
+        if (pb_remove(t, L, b, fp)) return 1;
+        if (pb_fill(t, L, b) != (int)L->B) return 0;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* KV (key_val_pair<uint64,uint64>) schemes                            */
+/* ------------------------------------------------------------------ */
+static inline uint64_t *kv_slot(or_level *L, uint64_t s) { return (uint64_t *)L->slots + 2 * s; }
+
+/* Sector accounting for KV probes: count distinct 64 B sectors in the
+ * examined slot range [s0, s1) of the slot array, skipping the sector the
+ * previous probe already covered (linear probing walks adjacent buckets). */
+static void account_kv_range(or_table *t, int lvl, uint64_t s0, uint64_t s1, uint64_t *last_sector)
+{
+    t->buckets_read[lvl] += 1;
+    uint64_t a = (s0 * 16) / 64, z = (s1 * 16 - 1) / 64;
+    for (uint64_t s = a; s <= z; s++) {
+        if (s != *last_sector) t->sectors_read += 1;
+        *last_sector = s;
+    }
+}
+
+/* bucket_insert::insert, insert_schemes/bucket_insert.cuh:127-203: the
+ * lowest empty slot of the first probe bucket that has one.
+ * key_val_pair::atomic_swap (key_val_pair.cuh:84-95): CAS key 0 -> key,
+ * then store the value. */
+static int kv_insert(or_table *t, or_level *L, int lvl, uint64_t key, uint64_t val)
+{
+    or_probe_it it;
+    uint64_t last = OR_END;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, key);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        uint64_t base = b * L->B;
+        for (uint32_t i = 0; i < L->B; i++) {
+            uint64_t *s = kv_slot(L, base + i);
+            if (s[0] == KV_EMPTY) {
+                account_kv_range(t, lvl, base, base + i + 1, &last);
+                s[0] = key;
+                s[1] = val;
+                t->sectors_written += 1;
+                return 1;
+            }
+        }
+        account_kv_range(t, lvl, base, base + L->B, &last);
+    }
+    return 0;
+}
+
+/* bucket_insert::query, bucket_insert.cuh:295-357.  The tile scans stripes
+ * of P slots (lane i owns slot stripe*P + i); a stripe with a match returns
+ * the lowest matching slot; otherwise a stripe holding an empty slot ends
+ * the search.  single_slot_insert::query (single_slot_insert.cuh:168-229)
+ * is the P=B=1 case. */
+static int kv_query(or_table *t, or_level *L, int lvl, uint64_t key, uint64_t *val)
+{
+    uint32_t P = L->d.tile ? L->d.tile : 1;
+    if (L->d.scheme == OR_SCHEME_SINGLE_SLOT) P = 1;
+    or_probe_it it;
+    uint64_t last = OR_END;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, key);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        uint64_t base = b * L->B;
+        for (uint32_t s0 = 0; s0 < L->B; s0 += P) {
+            uint32_t s1 = s0 + P < L->B ? s0 + P : L->B;
+            account_kv_range(t, lvl, base + s0, base + s1, &last);
+            for (uint32_t i = s0; i < s1; i++) {
+                uint64_t *s = kv_slot(L, base + i);
+                if (s[0] == key) { *val = s[1]; return 1; }
+            }
+            for (uint32_t i = s0; i < s1; i++)
+                if (kv_slot(L, base + i)[0] == KV_EMPTY) return 0;
+        }
+    }
+    return 0;
+}
+
+/* bucket_insert::remove, bucket_insert.cuh:205-293: CAS the lowest match in
+ * the stripe to the tombstone key (key_val_pair.cuh:143-151); a stripe with
+ * no match but an empty slot ends the search. */
+static int kv_remove(or_table *t, or_level *L, int lvl, uint64_t key)
+{
+    uint32_t P = L->d.tile ? L->d.tile : 1;
+    or_probe_it it;
+    uint64_t last = OR_END;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, key);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        uint64_t base = b * L->B;
+        for (uint32_t s0 = 0; s0 < L->B; s0 += P) {
+            uint32_t s1 = s0 + P < L->B ? s0 + P : L->B;
+            account_kv_range(t, lvl, base + s0, base + s1, &last);
+            for (uint32_t i = s0; i < s1; i++) {
+                uint64_t *s = kv_slot(L, base + i);
+                if (s[0] == key) {
+                    s[0] = KV_TOMB;
+                    t->sectors_written += 1;
+                    return 1;
+                }
+            }
+            for (uint32_t i = s0; i < s1; i++)
+                if (kv_slot(L, base + i)[0] == KV_EMPTY) return 0;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Table lifecycle                                                     */
+/* ------------------------------------------------------------------ */
+
+/* bucketed_table::generate_on_device / static_table::generate_on_device
+ * (tables/bucketed_table.cuh:132-192, tables/base_table.cuh:110-145): level
+ * i takes the next size of the sizing (sizing/default_sizing.cuh:54-73) and
+ * seed, the backing level seed*seed.  Buckets = ceil(nslots / B)
+ * (power_of_n_shortcut_buckets.cuh:94, bucket_insert.cuh:90-92); the single
+ * slot scheme allocates nslots slots (single_slot_insert.cuh:89-96). */
+or_table *or_create(uint32_t rep, uint32_t nlevels, const or_level_desc *lv, uint64_t seed)
+{
+    if (nlevels < 1 || nlevels > 2) return NULL;
+    or_table *t = (or_table *)calloc(1, sizeof(or_table));
+    if (!t) return NULL;
+    t->rep = rep;
+    t->nlevels = nlevels;
+    uint64_t s = seed;
+    for (uint32_t l = 0; l < nlevels; l++) {
+        or_level *L = &t->lv[l];
+        L->d = lv[l];
+        L->seed = s;
+        s = s * s;
+        if (lv[l].nslots == 0) { free(t); return NULL; }
+        if (lv[l].scheme == OR_SCHEME_SINGLE_SLOT) {
+            L->B = 1;
+            L->nb = lv[l].nslots;
+        } else {
+            L->B = lv[l].bucket_size;
+            L->nb = (lv[l].nslots - 1) / L->B + 1;
+        }
+        L->nslot_alloc = L->nb * L->B;
+        L->slots = calloc(L->nslot_alloc, slot_bytes(t));
+        if (!L->slots) { for (uint32_t q = 0; q < l; q++) free(t->lv[q].slots); free(t); return NULL; }
+    }
+    return t;
+}
+
+void or_destroy(or_table *t)
+{
+    if (!t) return;
+    for (uint32_t l = 0; l < t->nlevels; l++) free(t->lv[l].slots);
+    free(t);
+}
+
+void or_clear(or_table *t)
+{
+    for (uint32_t l = 0; l < t->nlevels; l++)
+        memset(t->lv[l].slots, 0, t->lv[l].nslot_alloc * slot_bytes(t));
+}
+
+/* Per-level dispatch for the packed representations.  Every table op
+ * tries level 0 and, on false, the backing level (bucketed_table.cuh:271-381). */
+static int lvl_insert(or_table *t, int l, uint64_t key, uint32_t word, int with_delete)
+{
+    or_level *L = &t->lv[l];
+    if (L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT) return shortcut_insert(t, L, key, word, with_delete);
+    return linear_insert(t, L, l, key, word, with_delete);
+}
+static int lvl_query(or_table *t, int l, uint64_t key, uint32_t *val)
+{
+    or_level *L = &t->lv[l];
+    if (L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT) return shortcut_query(t, L, key, val);
+    return linear_query(t, L, l, key, val);
+}
+static int lvl_remove(or_table *t, int l, uint64_t key)
+{
+    or_level *L = &t->lv[l];
+    if (L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT) return shortcut_remove(t, L, key);
+    return linear_remove(t, L, l, key);
+}
+
+/* Insert one key; returns 0 on failure, else 1 + level index. */
+int or_insert(or_table *t, uint64_t key, uint64_t val, int with_delete)
+{
+    if (t->rep == OR_REP_KV64) {
+        for (uint32_t l = 0; l < t->nlevels; l++)
+            if (kv_insert(t, &t->lv[l], (int)l, key, val)) return 1 + (int)l;
+        return 0;
+    }
+    uint32_t vbits = kbits_of(t);
+    uint32_t word = or_pack(vbits, vbits, fp_of(t->rep, key), (uint32_t)val);
+    for (uint32_t l = 0; l < t->nlevels; l++)
+        if (lvl_insert(t, (int)l, key, word, with_delete)) return 1 + (int)l;
+    return 0;
+}
+
+int or_query(or_table *t, uint64_t key, uint64_t *val)
+{
+    if (t->rep == OR_REP_KV64) {
+        for (uint32_t l = 0; l < t->nlevels; l++)
+            if (kv_query(t, &t->lv[l], (int)l, key, val)) return 1;
+        return 0;
+    }
+    for (uint32_t l = 0; l < t->nlevels; l++) {
+        uint32_t v;
+        if (lvl_query(t, (int)l, key, &v)) { *val = v; return 1; }
+    }
+    return 0;
+}
+
+int or_remove(or_table *t, uint64_t key)
+{
+    if (t->rep == OR_REP_KV64) {
+        for (uint32_t l = 0; l < t->nlevels; l++) {
+            if (t->lv[l].d.scheme == OR_SCHEME_SINGLE_SLOT) return -1; /* no remove in the reference */
+            if (kv_remove(t, &t->lv[l], (int)l, key)) return 1;
+        }
+        return 0;
+    }
+    for (uint32_t l = 0; l < t->nlevels; l++)
+        if (lvl_remove(t, (int)l, key)) return 1;
+    return 0;
+}
+
+/* ---- bulk wrappers: the harness kernels of tests/src/test_mhm_tcf.cu:
+ * 318-487 apply one op per key; sequential order here. ---- */
+void or_bulk_insert(or_table *t, const uint64_t *keys, const void *vals, uint32_t val_bytes,
+                    uint64_t n, uint8_t *status, int with_delete)
+{
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t v = 0;
+        if (vals) {
+            if (val_bytes == 8) v = ((const uint64_t *)vals)[i];
+            else if (val_bytes == 2) v = ((const uint16_t *)vals)[i];
+            else if (val_bytes == 1) v = ((const uint8_t *)vals)[i];
+        }
+        int r = or_insert(t, keys[i], v, with_delete);
+        if (status) status[i] = (uint8_t)r;
+    }
+}
+
+void or_bulk_query(or_table *t, const uint64_t *keys, uint64_t n, void *vals_out,
+                   uint32_t val_bytes, uint8_t *found)
+{
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t v = 0;
+        int r = or_query(t, keys[i], &v);
+        if (found) found[i] = (uint8_t)r;
+        if (vals_out) {
+            if (!r) v = 0;
+            if (val_bytes == 8) ((uint64_t *)vals_out)[i] = v;
+            else if (val_bytes == 2) ((uint16_t *)vals_out)[i] = (uint16_t)v;
+            else if (val_bytes == 1) ((uint8_t *)vals_out)[i] = (uint8_t)v;
+        }
+    }
+}
+
+void or_bulk_remove(or_table *t, const uint64_t *keys, uint64_t n, uint8_t *found)
+{
+    for (uint64_t i = 0; i < n; i++) {
+        int r = or_remove(t, keys[i]);
+        if (found) found[i] = (uint8_t)(r > 0);
+    }
+}
+
+/* bucketed_table::get_fill (tables/bucketed_table.cuh:498-548): filled
+ * slots (neither empty nor tombstone) summed over levels. */
+uint64_t or_fill(const or_table *t)
+{
+    uint64_t f = 0;
+    for (uint32_t l = 0; l < t->nlevels; l++) {
+        const or_level *L = &t->lv[l];
+        for (uint64_t s = 0; s < L->nslot_alloc; s++) {
+            if (t->rep == OR_REP_KV64) {
+                uint64_t k = ((const uint64_t *)L->slots)[2 * s];
+                if (k != KV_EMPTY && k != KV_TOMB) f++;
+            } else {
+                uint32_t w = wget(t, L, s);
+                if (!(p_empty(t, w) || p_tomb(t, w))) f++;
+            }
+        }
+    }
+    return f;
+}
+
+uint64_t or_level_bytes(const or_table *t, uint32_t l)
+{
+    if (l >= t->nlevels) return 0;
+    return t->lv[l].nslot_alloc * slot_bytes(t);
+}
+const void *or_level_data(const or_table *t, uint32_t l)
+{
+    if (l >= t->nlevels) return NULL;
+    return t->lv[l].slots;
+}
+uint64_t or_level_buckets(const or_table *t, uint32_t l)
+{
+    if (l >= t->nlevels) return 0;
+    return t->lv[l].nb;
+}
+
+void or_stats(const or_table *t, uint64_t *out4)
+{
+    out4[0] = t->sectors_read;
+    out4[1] = t->sectors_written;
+    out4[2] = t->buckets_read[0];
+    out4[3] = t->buckets_read[1];
+}
+void or_stats_reset(or_table *t)
+{
+    t->sectors_read = t->sectors_written = 0;
+    t->buckets_read[0] = t->buckets_read[1] = 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Seeded key streams: splitmix64 (Vigna) output of counter i of a      */
+/* stream; a bijection of the counter, so keys of one seed are distinct.*/
+/* 0 and ~0 are reserved slot words; they map to fixed other values.   */
+/* ------------------------------------------------------------------ */
+uint64_t or_key_at(uint64_t seed, uint64_t i)
+{
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    if (z == 0) z = 0x8000000000000000ULL;
+    if (z == ~0ULL) z = 0x7FFFFFFFFFFFFFFFULL;
+    return z;
+}
+
+void or_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *out)
+{
+    for (uint64_t i = 0; i < n; i++) out[i] = or_key_at(seed, start + i);
+}

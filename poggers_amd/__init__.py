@@ -1,0 +1,310 @@
+"""poggers_amd -- MI355X-native bulk hash tables and Two-Choice Filter.
+
+A thin ctypes binding over the C ABI in ``include/poggers_amd.h``
+(``libpoggers_amd.so``, hand-written HIP for gfx950).  PyTorch is only used
+for device buffers and streams.  There is no CPU fallback: if the shared
+library is missing every entry point raises.
+
+The naming follows the reference's table API (tables/bucketed_table.cuh:
+``generate_on_device``, ``insert``, ``query``, ``remove``,
+``insert_with_delete``, ``get_fill``, ``get_num_slots``,
+``host_bytes_in_use``) with the bulk forms the C ABI provides.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpoggers_amd.so")
+
+# enum values of include/poggers_amd.h
+PG_OK = 0
+REP_KV64, REP_FP16_V16 = 0, 1
+SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS = 0, 1, 2, 3
+PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
+HASH_FMIX64, HASH_MURMUR64A = 0, 1
+CONCURRENT, DETERMINISTIC = 0, 1
+MAX_LEVELS = 2
+
+# Every symbol the header declares (checked by the CPU test suite).
+EXPORTS = [
+    "pg_create", "pg_destroy", "pg_clear", "pg_set_mode", "pg_bulk_insert",
+    "pg_bulk_insert_with_delete", "pg_bulk_query", "pg_bulk_remove", "pg_fill",
+    "pg_bytes", "pg_num_slots", "pg_level_ptr", "pg_error_string", "pg_gen_keys",
+    "pg_random_sector_bench", "pg_fastmod_host", "pg_set_stats",
+]
+
+
+class PgError(RuntimeError):
+    pass
+
+
+class LevelDesc(ctypes.Structure):
+    _fields_ = [
+        ("nslots", ctypes.c_uint64),
+        ("scheme", ctypes.c_uint32),
+        ("bucket_size", ctypes.c_uint32),
+        ("tile", ctypes.c_uint32),
+        ("prober", ctypes.c_uint32),
+        ("hasher", ctypes.c_uint32),
+        ("max_probes", ctypes.c_uint32),
+    ]
+
+
+class Desc(ctypes.Structure):
+    _fields_ = [
+        ("rep", ctypes.c_uint32),
+        ("num_levels", ctypes.c_uint32),
+        ("level", LevelDesc * MAX_LEVELS),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libpoggers_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PgError(f"{LIB_PATH} is missing: run `python -m poggers_amd.build` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    u64, u32, vp, i32 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int
+    L.pg_create.argtypes = [ctypes.POINTER(Desc), u64, ctypes.POINTER(vp)]
+    L.pg_destroy.argtypes = [vp]
+    L.pg_clear.argtypes = [vp, vp]
+    L.pg_set_mode.argtypes = [vp, u32]
+    L.pg_bulk_insert.argtypes = [vp, vp, vp, u64, vp, vp, vp]
+    L.pg_bulk_insert_with_delete.argtypes = [vp, vp, vp, u64, vp, vp, vp]
+    L.pg_bulk_query.argtypes = [vp, vp, u64, vp, vp, vp]
+    L.pg_bulk_remove.argtypes = [vp, vp, u64, vp, vp]
+    L.pg_fill.argtypes = [vp, ctypes.POINTER(u64), vp]
+    L.pg_bytes.argtypes = [vp, ctypes.POINTER(u64)]
+    L.pg_num_slots.argtypes = [vp, ctypes.POINTER(u64)]
+    L.pg_level_ptr.argtypes = [vp, u32, ctypes.POINTER(vp), ctypes.POINTER(u64)]
+    L.pg_error_string.argtypes = [i32]
+    L.pg_error_string.restype = ctypes.c_char_p
+    L.pg_gen_keys.argtypes = [u64, u64, u64, vp, vp]
+    L.pg_random_sector_bench.argtypes = [vp, u64, u64, u32, u64, vp, vp]
+    L.pg_set_stats.argtypes = [vp, vp]
+    L.pg_fastmod_host.argtypes = [u64, u64]
+    L.pg_fastmod_host.restype = u64
+    for name in EXPORTS:
+        if name not in ("pg_error_string", "pg_fastmod_host"):
+            getattr(L, name).restype = i32
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != PG_OK:
+        msg = lib().pg_error_string(rc).decode()
+        raise PgError(f"{what}: {msg} (status {rc})")
+
+
+def _ptr(t):
+    """Device pointer of a tensor (or None)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+@dataclass
+class Level:
+    nslots: int
+    scheme: int
+    bucket_size: int
+    tile: int = 4
+    prober: int = PROBE_SPLIT64
+    hasher: int = HASH_FMIX64
+    max_probes: int = 2
+
+    def as_dict(self):
+        return dict(nslots=int(self.nslots), scheme=self.scheme, bucket_size=self.bucket_size,
+                    tile=self.tile, prober=self.prober, hasher=self.hasher, max_probes=self.max_probes)
+
+
+@dataclass
+class TableDesc:
+    rep: int
+    levels: list = field(default_factory=list)
+
+    def to_c(self) -> Desc:
+        d = Desc()
+        d.rep = self.rep
+        d.num_levels = len(self.levels)
+        for i, lv in enumerate(self.levels):
+            for k, v in lv.as_dict().items():
+                setattr(d.level[i], k, v)
+        return d
+
+
+def tcf_desc(nslots: int, backing_slots: int | None = None, bucket_size: int = 16, tile: int = 4,
+             prober: int = PROBE_SPLIT64, hasher: int = HASH_FMIX64, backing_bucket: int = 8,
+             backing_prober: int = PROBE_LINEAR, backing_probes: int = 4000) -> TableDesc:
+    """Two-Choice Filter: a POWER_OF_N_SHORTCUT level with 16-bit fingerprints
+    and 16-bit values plus a LINEAR_BUCKETS backing level (the reference's
+    del_TCF composition, tests/src/delete_tests.cu:138-152).  By default the
+    backing level holds 1 % of the primary's slots."""
+    if backing_slots is None:
+        backing_slots = nslots // 100
+    levels = [Level(nslots, SCHEME_SHORTCUT, bucket_size, tile, prober, hasher, 2)]
+    if backing_slots:
+        levels.append(Level(backing_slots, SCHEME_LINEAR_BUCKETS, backing_bucket, tile, backing_prober,
+                            hasher, backing_probes))
+    return TableDesc(REP_FP16_V16, levels)
+
+
+def kv_desc(nslots: int, bucket_size: int = 1, tile: int = 4, prober: int = PROBE_LINEAR,
+            hasher: int = HASH_FMIX64, max_probes: int | None = None, scheme: int = SCHEME_BUCKET,
+            backing: Level | None = None) -> TableDesc:
+    """Key-value table of key_val_pair<uint64,uint64> slots (bucket_insert)."""
+    if max_probes is None:
+        nb = (nslots + bucket_size - 1) // bucket_size
+        max_probes = nb if prober == PROBE_LINEAR else 20
+    levels = [Level(nslots, scheme, bucket_size, tile, prober, hasher, int(min(max_probes, 2**32 - 1)))]
+    if backing is not None:
+        levels.append(backing)
+    return TableDesc(REP_KV64, levels)
+
+
+class Table:
+    """A device-resident table (``pg_create``).  Key/value/result arguments
+    are torch CUDA tensors: keys int64 (uint64 bit patterns), values int64
+    (KV) or int16 (filters), flags uint8."""
+
+    def __init__(self, desc: TableDesc, seed: int = 42):
+        self.desc = desc
+        self.seed = seed
+        self._cdesc = desc.to_c()
+        h = ctypes.c_void_p()
+        check(lib().pg_create(ctypes.byref(self._cdesc), seed, ctypes.byref(h)), "pg_create")
+        self._h = h
+
+    # reference-style constructor name (tables/bucketed_table.cuh:132)
+    @classmethod
+    def generate_on_device(cls, desc: TableDesc, seed: int = 42) -> "Table":
+        return cls(desc, seed)
+
+    def free_on_device(self):
+        if getattr(self, "_h", None):
+            check(lib().pg_destroy(self._h), "pg_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free_on_device()
+        except Exception:
+            pass
+
+    @property
+    def is_kv(self) -> bool:
+        return self.desc.rep == REP_KV64
+
+    def set_mode(self, mode: int):
+        check(lib().pg_set_mode(self._h, mode), "pg_set_mode")
+
+    def clear(self, stream=None):
+        check(lib().pg_clear(self._h, _stream(stream)), "pg_clear")
+
+    def bulk_insert(self, keys, vals=None, status=None, fail_count=None, stream=None, with_delete=False):
+        fn = lib().pg_bulk_insert_with_delete if with_delete else lib().pg_bulk_insert
+        check(fn(self._h, _ptr(keys), _ptr(vals), keys.numel(), _ptr(status), _ptr(fail_count),
+                 _stream(stream)), "pg_bulk_insert")
+
+    def bulk_insert_with_delete(self, keys, vals=None, status=None, fail_count=None, stream=None):
+        self.bulk_insert(keys, vals, status, fail_count, stream, with_delete=True)
+
+    def bulk_query(self, keys, vals_out=None, found=None, stream=None):
+        check(lib().pg_bulk_query(self._h, _ptr(keys), keys.numel(), _ptr(vals_out), _ptr(found),
+                                  _stream(stream)), "pg_bulk_query")
+
+    def bulk_remove(self, keys, found=None, stream=None):
+        check(lib().pg_bulk_remove(self._h, _ptr(keys), keys.numel(), _ptr(found), _stream(stream)),
+              "pg_bulk_remove")
+
+    def set_stats(self, stats=None):
+        """Accumulate algorithmic counters into a 3-element int64 CUDA tensor
+        (None disables)."""
+        check(lib().pg_set_stats(self._h, _ptr(stats)), "pg_set_stats")
+
+    def get_fill(self, stream=None) -> int:
+        out = ctypes.c_uint64()
+        check(lib().pg_fill(self._h, ctypes.byref(out), _stream(stream)), "pg_fill")
+        return out.value
+
+    def host_bytes_in_use(self) -> int:
+        out = ctypes.c_uint64()
+        check(lib().pg_bytes(self._h, ctypes.byref(out)), "pg_bytes")
+        return out.value
+
+    def get_num_slots(self) -> int:
+        out = ctypes.c_uint64()
+        check(lib().pg_num_slots(self._h, ctypes.byref(out)), "pg_num_slots")
+        return out.value
+
+    def level_ptr(self, level: int):
+        p = ctypes.c_void_p()
+        n = ctypes.c_uint64()
+        check(lib().pg_level_ptr(self._h, level, ctypes.byref(p), ctypes.byref(n)), "pg_level_ptr")
+        return p.value, n.value
+
+    def level_bytes(self, level: int) -> bytes:
+        """Raw copy of one level's slot array (device -> host)."""
+        import torch
+        ptr, n = self.level_ptr(level)
+        torch.cuda.synchronize()
+        buf = ctypes.create_string_buffer(n)
+        _hip_memcpy(buf, ptr, n, 2)  # hipMemcpyDeviceToHost
+        return buf.raw
+
+    def level_upload(self, level: int, data: bytes):
+        """Overwrite one level's slot array with a raw image (host -> device)."""
+        import torch
+        ptr, n = self.level_ptr(level)
+        if len(data) != n:
+            raise PgError(f"image is {len(data)} bytes, level {level} holds {n}")
+        torch.cuda.synchronize()
+        buf = ctypes.create_string_buffer(data, n)
+        _hip_memcpy(ptr, ctypes.addressof(buf), n, 1)  # hipMemcpyHostToDevice
+
+
+_hip = None
+
+
+def _hip_memcpy(dst, src, n: int, kind: int):
+    global _hip
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipMemcpy.restype = ctypes.c_int
+    d = dst if isinstance(dst, ctypes.Array) else ctypes.c_void_p(dst)
+    rc = _hip.hipMemcpy(d, ctypes.c_void_p(src), n, kind)
+    if rc != 0:
+        raise PgError(f"hipMemcpy failed ({rc})")
+
+
+def gen_keys(seed: int, start: int, n: int, out, stream=None):
+    """Fill ``out`` (int64 CUDA tensor) with splitmix64 keys start..start+n-1."""
+    check(lib().pg_gen_keys(seed, start, n, _ptr(out), _stream(stream)), "pg_gen_keys")
+
+
+def random_sector_bench(buf, n_reads: int, op: int, seed: int, sink, stream=None):
+    check(lib().pg_random_sector_bench(_ptr(buf), buf.numel() * buf.element_size(), n_reads, op, seed,
+                                       _ptr(sink), _stream(stream)), "pg_random_sector_bench")
+
+
+def fastmod_host(n: int, d: int) -> int:
+    return lib().pg_fastmod_host(n, d)

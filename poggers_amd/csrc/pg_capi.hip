@@ -1,0 +1,269 @@
+// pg_capi.hip -- host side of the C ABI (include/poggers_amd.h).
+#include <new>
+#include <stdlib.h>
+#include <string.h>
+
+#include "pg_internal.hpp"
+
+struct pg_table_s {
+    pg_desc desc;
+    pg::TableDev dev;
+    uint64_t seed;
+    uint32_t mode;
+    int device;
+    uint64_t bytes[PG_MAX_LEVELS];
+    uint64_t *d_counter;  // 8 B device scratch for pg_fill
+    uint64_t *d_stats;    // caller's accounting counters, or NULL
+};
+
+namespace {
+
+bool pow2_tile(uint32_t p) { return p >= 1 && p <= 64 && (p & (p - 1)) == 0; }
+
+// Validate one level; returns PG_OK or an error.
+int check_level(uint32_t rep, const pg_level_desc &l, uint32_t index) {
+    if (l.nslots == 0) return PG_ERR_INVALID;
+    if (!pow2_tile(l.tile)) return PG_ERR_INVALID;
+    if (l.prober > PG_PROBE_SPLIT64 || l.hasher > PG_HASH_MURMUR64A) return PG_ERR_INVALID;
+    if (l.max_probes == 0) return PG_ERR_INVALID;
+    // power_of_two.cuh:32, xor_power_of_two.cuh:32: exactly two probes.
+    if ((l.prober == PG_PROBE_P2 || l.prober == PG_PROBE_XOR_P2 || l.prober == PG_PROBE_SPLIT64) &&
+        l.max_probes != 2)
+        return PG_ERR_INVALID;
+    if (rep == PG_REP_KV64) {
+        if (l.scheme != PG_SCHEME_BUCKET && l.scheme != PG_SCHEME_SINGLE_SLOT) return PG_ERR_UNSUPPORTED;
+        if (l.scheme == PG_SCHEME_BUCKET && (l.bucket_size == 0 || l.bucket_size > 64)) return PG_ERR_INVALID;
+    } else if (rep == PG_REP_FP16_V16) {
+        if (l.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT && l.scheme != PG_SCHEME_LINEAR_BUCKETS)
+            return PG_ERR_UNSUPPORTED;
+        if (l.bucket_size == 0 || l.bucket_size > 64) return PG_ERR_INVALID;
+        if (l.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT && l.max_probes > pg::kMaxShortcutProbes)
+            return PG_ERR_UNSUPPORTED;
+    } else {
+        return PG_ERR_INVALID;
+    }
+    (void)index;
+    return PG_OK;
+}
+
+uint32_t slot_bytes(uint32_t rep) { return rep == PG_REP_KV64 ? 16u : 4u; }
+
+}  // namespace
+
+extern "C" {
+
+const char *pg_error_string(int status) {
+    switch (status) {
+        case PG_OK: return "ok";
+        case PG_ERR_INVALID: return "invalid argument";
+        case PG_ERR_UNSUPPORTED: return "unsupported by this scheme";
+        case PG_ERR_NOMEM: return "device allocation failed";
+        case PG_ERR_HIP: return "HIP runtime error";
+        case PG_ERR_NODEVICE: return "no usable GPU";
+        default: return "unknown status";
+    }
+}
+
+uint64_t pg_fastmod_host(uint64_t n, uint64_t d) {
+    if (d == 0) return 0;
+    return pg::fastmod(n, pg::make_fastmod(d));
+}
+
+int pg_create(const pg_desc *desc, uint64_t seed, pg_handle *out) {
+    if (!desc || !out) return PG_ERR_INVALID;
+    *out = nullptr;
+    if (desc->num_levels < 1 || desc->num_levels > PG_MAX_LEVELS) return PG_ERR_INVALID;
+    // The tile is shared by all levels: the reference passes the level-0 tile
+    // to the backing table's ops (bucketed_table.cuh:280-282).
+    for (uint32_t i = 0; i < desc->num_levels; i++) {
+        const int rc = check_level(desc->rep, desc->level[i], i);
+        if (rc != PG_OK) return rc;
+        if (desc->level[i].tile != desc->level[0].tile) return PG_ERR_INVALID;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PG_ERR_NODEVICE;
+
+    pg_table_s *t = new (std::nothrow) pg_table_s();
+    if (!t) return PG_ERR_NOMEM;
+    t->desc = *desc;
+    t->seed = seed;
+    t->mode = PG_CONCURRENT;
+    if (hipGetDevice(&t->device) != hipSuccess) {
+        delete t;
+        return PG_ERR_HIP;
+    }
+    t->dev.nlevels = desc->num_levels;
+    t->dev.rep = desc->rep;
+    uint64_t s = seed;
+    int rc = PG_OK;
+    for (uint32_t i = 0; i < desc->num_levels && rc == PG_OK; i++) {
+        const pg_level_desc &l = desc->level[i];
+        pg::LevelDev &L = t->dev.lv[i];
+        // Buckets = ceil(nslots / B) (power_of_n_shortcut_buckets.cuh:94,
+        // bucket_insert.cuh:90-92); single slot allocates nslots slots
+        // (single_slot_insert.cuh:89-96).
+        const uint32_t B = (l.scheme == PG_SCHEME_SINGLE_SLOT) ? 1u : l.bucket_size;
+        const uint64_t nb = (l.nslots - 1) / B + 1;
+        L.nb = pg::make_fastmod(nb);
+        L.B = B;
+        L.seed = s;
+        L.seed2 = s * s;
+        L.scheme = l.scheme;
+        L.prober = l.prober;
+        L.hasher = l.hasher;
+        L.max_probes = l.max_probes;
+        s = s * s;  // recursive level seeded seed*seed (bucketed_table.cuh:170)
+        const uint64_t bytes = ((nb * B * slot_bytes(desc->rep)) + 255) & ~uint64_t(255);
+        t->bytes[i] = nb * B * slot_bytes(desc->rep);
+        if (hipMalloc(&L.slots, bytes) != hipSuccess) {
+            L.slots = nullptr;
+            rc = PG_ERR_NOMEM;
+            break;
+        }
+        rc = pg::launch_zero(L.slots, bytes, nullptr);
+    }
+    if (rc == PG_OK && hipMalloc((void **)&t->d_counter, 256) != hipSuccess) rc = PG_ERR_NOMEM;
+    if (rc == PG_OK && hipDeviceSynchronize() != hipSuccess) rc = PG_ERR_HIP;
+    if (rc != PG_OK) {
+        pg_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return PG_OK;
+}
+
+int pg_destroy(pg_handle h) {
+    if (!h) return PG_ERR_INVALID;
+    hipDeviceSynchronize();
+    for (uint32_t i = 0; i < h->dev.nlevels; i++)
+        if (h->dev.lv[i].slots) hipFree(h->dev.lv[i].slots);
+    if (h->d_counter) hipFree(h->d_counter);
+    delete h;
+    return PG_OK;
+}
+
+int pg_clear(pg_handle h, void *stream) {
+    if (!h) return PG_ERR_INVALID;
+    for (uint32_t i = 0; i < h->dev.nlevels; i++) {
+        const int rc = pg::launch_zero(h->dev.lv[i].slots, (h->bytes[i] + 15) & ~uint64_t(15),
+                                       (hipStream_t)stream);
+        if (rc != PG_OK) return rc;
+    }
+    return PG_OK;
+}
+
+int pg_set_mode(pg_handle h, uint32_t mode) {
+    if (!h || mode > PG_DETERMINISTIC) return PG_ERR_INVALID;
+    h->mode = mode;
+    return PG_OK;
+}
+
+static int run_op(pg_handle h, pg::Op op, const pg::OpArgs &a, hipStream_t s) {
+    if (h->dev.rep == PG_REP_KV64) return pg::launch_kv(h->dev, h->desc.level[0].tile, op, a, s);
+    return pg::launch_tcf(h->dev, h->desc.level[0].tile, op, a, s);
+}
+
+static int do_insert(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n, uint8_t *d_status,
+                     uint64_t *d_fail_count, void *stream, uint32_t with_delete) {
+    if (!h || (n && !d_keys)) return PG_ERR_INVALID;
+    pg::OpArgs a{};
+    a.keys = d_keys;
+    a.vals = d_vals;
+    a.n = n;
+    a.flags = d_status;
+    a.fail_count = d_fail_count;
+    a.with_delete = with_delete;
+    a.deterministic = h->mode == PG_DETERMINISTIC;
+    a.stats = h->d_stats;
+    return run_op(h, pg::Op::Insert, a, (hipStream_t)stream);
+}
+
+int pg_bulk_insert(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n, uint8_t *d_status,
+                   uint64_t *d_fail_count, void *stream) {
+    return do_insert(h, d_keys, d_vals, n, d_status, d_fail_count, stream, 0);
+}
+
+int pg_bulk_insert_with_delete(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n,
+                               uint8_t *d_status, uint64_t *d_fail_count, void *stream) {
+    if (!h) return PG_ERR_INVALID;
+    // bucket_insert / single_slot_insert have no insert_with_delete.
+    if (h->dev.rep == PG_REP_KV64) return PG_ERR_UNSUPPORTED;
+    return do_insert(h, d_keys, d_vals, n, d_status, d_fail_count, stream, 1);
+}
+
+int pg_bulk_query(pg_handle h, const uint64_t *d_keys, uint64_t n, void *d_vals_out, uint8_t *d_found,
+                  void *stream) {
+    if (!h || (n && !d_keys)) return PG_ERR_INVALID;
+    pg::OpArgs a{};
+    a.keys = d_keys;
+    a.n = n;
+    a.vals_out = d_vals_out;
+    a.flags = d_found;
+    a.stats = h->d_stats;
+    return run_op(h, pg::Op::Query, a, (hipStream_t)stream);
+}
+
+int pg_bulk_remove(pg_handle h, const uint64_t *d_keys, uint64_t n, uint8_t *d_found, void *stream) {
+    if (!h || (n && !d_keys)) return PG_ERR_INVALID;
+    for (uint32_t i = 0; i < h->dev.nlevels; i++)
+        if (h->dev.lv[i].scheme == PG_SCHEME_SINGLE_SLOT) return PG_ERR_UNSUPPORTED;
+    pg::OpArgs a{};
+    a.keys = d_keys;
+    a.n = n;
+    a.flags = d_found;
+    a.deterministic = h->mode == PG_DETERMINISTIC;
+    a.stats = h->d_stats;
+    return run_op(h, pg::Op::Remove, a, (hipStream_t)stream);
+}
+
+int pg_set_stats(pg_handle h, uint64_t *d_stats) {
+    if (!h) return PG_ERR_INVALID;
+    h->d_stats = d_stats;
+    return PG_OK;
+}
+
+int pg_fill(pg_handle h, uint64_t *out, void *stream) {
+    if (!h || !out) return PG_ERR_INVALID;
+    hipStream_t s = (hipStream_t)stream;
+    int rc = pg::launch_fill(h->dev, h->d_counter, s);
+    if (rc != PG_OK) return rc;
+    if (hipMemcpyAsync(out, h->d_counter, sizeof(uint64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return PG_ERR_HIP;
+    return hipStreamSynchronize(s) == hipSuccess ? PG_OK : PG_ERR_HIP;
+}
+
+int pg_bytes(pg_handle h, uint64_t *out) {
+    if (!h || !out) return PG_ERR_INVALID;
+    uint64_t b = 0;
+    for (uint32_t i = 0; i < h->dev.nlevels; i++) b += h->bytes[i];
+    *out = b;
+    return PG_OK;
+}
+
+int pg_num_slots(pg_handle h, uint64_t *out) {
+    if (!h || !out) return PG_ERR_INVALID;
+    uint64_t n = 0;
+    for (uint32_t i = 0; i < h->dev.nlevels; i++) n += h->dev.lv[i].nb.d * h->dev.lv[i].B;
+    *out = n;
+    return PG_OK;
+}
+
+int pg_level_ptr(pg_handle h, uint32_t level, void **d_ptr, uint64_t *bytes) {
+    if (!h || level >= h->dev.nlevels) return PG_ERR_INVALID;
+    if (d_ptr) *d_ptr = h->dev.lv[level].slots;
+    if (bytes) *bytes = h->bytes[level];
+    return PG_OK;
+}
+
+int pg_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *d_out, void *stream) {
+    if (n && !d_out) return PG_ERR_INVALID;
+    return pg::launch_gen_keys(seed, start, n, d_out, (hipStream_t)stream);
+}
+
+int pg_random_sector_bench(void *d_buf, uint64_t bytes, uint64_t n_reads, uint32_t op, uint64_t seed,
+                           uint64_t *d_sink, void *stream) {
+    if (!d_buf || !d_sink || bytes < 64 || (bytes % 64) || op > 1) return PG_ERR_INVALID;
+    return pg::launch_random_sector(d_buf, bytes, n_reads, op, seed, d_sink, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,184 @@
+// pg_internal.hpp -- structures and helpers shared by the HIP kernels and the
+// host side of the C ABI.  Host + device.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/poggers_amd.h"
+
+namespace pg {
+
+constexpr uint64_t kEnd = ~uint64_t(0);  // probing end() sentinel (probing_schemes/*.cuh end())
+constexpr uint32_t kMaxShortcutProbes = 4;
+
+// Range reduction n % d with an invariant divisor (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1):
+//   l = ceil(log2 d), m = floor(2^64 (2^l - d) / d) + 1,
+//   t = mulhi(m, n), q = (t + ((n - t) >> 1)) >> (l - 1),  r = n - q d.
+// Powers of two use a mask.  Exact for every 64-bit n and d >= 1; the
+// reference computes "probe % num_buckets" (power_of_n_shortcut_buckets.cuh:216).
+struct FastMod {
+    uint64_t d;
+    uint64_t magic;
+    uint64_t mask;   // d - 1 when d is a power of two
+    uint32_t sh2;
+    uint32_t pow2;
+};
+
+__host__ __device__ inline uint64_t mulhi64(uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+inline FastMod make_fastmod(uint64_t d) {
+    FastMod f{};
+    f.d = d;
+    if ((d & (d - 1)) == 0) {
+        f.pow2 = 1;
+        f.mask = d - 1;
+        return f;
+    }
+    uint32_t l = 64 - (uint32_t)__builtin_clzll(d - 1);  // ceil(log2 d), d >= 3
+    unsigned __int128 two_l = (unsigned __int128)1 << l;
+    unsigned __int128 num = (two_l - d) << 64;
+    f.magic = (uint64_t)(num / d) + 1;
+    f.sh2 = l - 1;
+    f.pow2 = 0;
+    return f;
+}
+
+__host__ __device__ inline uint64_t fastmod(uint64_t n, const FastMod &f) {
+    if (f.pow2) return n & f.mask;
+    uint64_t t = mulhi64(f.magic, n);
+    uint64_t q = (t + ((n - t) >> 1)) >> f.sh2;
+    return n - q * f.d;
+}
+
+// ---- hashers (hash_schemes/murmurhash.cuh:20-61 and MurmurHash3 fmix64) ----
+__host__ __device__ inline uint64_t fmix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdULL;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ULL;
+    k ^= k >> 33;
+    return k;
+}
+
+// MurmurHash64A specialised to one 8-byte block (len = 8, no tail).
+__host__ __device__ inline uint64_t murmur64a_u64(uint64_t key, uint64_t seed) {
+    const uint64_t m = 0xc6a4a7935bd1e995ULL;
+    uint64_t h = seed ^ (8ULL * m);
+    uint64_t k = key;
+    k *= m;
+    k ^= k >> 47;
+    k *= m;
+    h ^= k;
+    h *= m;
+    h ^= h >> 47;
+    h *= m;
+    h ^= h >> 47;
+    return h;
+}
+
+__host__ __device__ inline uint64_t hash64(uint32_t hasher, uint64_t key, uint64_t seed) {
+    return hasher == PG_HASH_MURMUR64A ? murmur64a_u64(key, seed) : fmix64(key ^ seed);
+}
+
+// splitmix64 key stream (the seeded generator of every benchmark input).
+__host__ __device__ inline uint64_t key_at(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    if (z == 0) z = 0x8000000000000000ULL;
+    if (z == ~0ULL) z = 0x7FFFFFFFFFFFFFFFULL;
+    return z;
+}
+
+// One level as the kernels see it (passed by value in the kernarg segment:
+// no device-side struct pointer chase, unlike the reference's
+// table -> scheme -> slots chain, tables/bucketed_table.cuh:93-95).
+struct LevelDev {
+    void *slots;
+    FastMod nb;          // number of buckets and its divider
+    uint64_t seed;       // H seed
+    uint64_t seed2;      // seed^2, second hash of DOUBLE / P2
+    uint32_t B;          // slots per bucket
+    uint32_t scheme;
+    uint32_t prober;
+    uint32_t hasher;
+    uint32_t max_probes;
+    uint32_t pad;
+};
+
+struct TableDev {
+    LevelDev lv[PG_MAX_LEVELS];
+    uint32_t nlevels;
+    uint32_t rep;
+};
+
+// Raw probe sequence of a key (probing_schemes/*.cuh begin()/next()).
+// Probe i is returned by at(i); a value of kEnd ends the sequence, exactly
+// like the reference's `p != end()` loop condition.
+struct Probe {
+    uint64_t h1, h2, tag;
+    uint32_t prober, max_probes;
+
+    __host__ __device__ inline void begin(const LevelDev &L, uint64_t key, uint64_t tagv) {
+        prober = L.prober;
+        max_probes = L.max_probes;
+        tag = tagv;
+        h1 = hash64(L.hasher, key, L.seed);
+        h2 = (prober == PG_PROBE_DOUBLE || prober == PG_PROBE_P2) ? hash64(L.hasher, key, L.seed2) : 0;
+    }
+    __host__ __device__ inline uint64_t at(uint32_t i) const {
+        if (i >= max_probes) return kEnd;
+        if (i == 0) return prober == PG_PROBE_SPLIT64 ? (h1 & 0xffffffffULL) : h1;
+        switch (prober) {
+            case PG_PROBE_LINEAR: return h1 + (uint64_t)i;
+            case PG_PROBE_DOUBLE: return h1 + (uint64_t)i * h2;
+            case PG_PROBE_P2: return h2;
+            case PG_PROBE_XOR_P2: return h1 ^ (tag * 0x5bd1e995ULL);
+            default: return h1 >> 32;  // SPLIT64
+        }
+    }
+};
+
+// Fingerprint of the FP16_V16 slot: low 16 bits of the raw key, 0 -> 1,
+// 0xFFFF -> 0xFFFE (representations/dynamic_container.cuh:35-50).
+__host__ __device__ inline uint32_t fp16_of(uint64_t key) {
+    uint32_t f = (uint32_t)(key & 0xFFFFu);
+    f += (f == 0u);
+    f -= (f == 0xFFFFu);
+    return f;
+}
+constexpr uint32_t kTomb16 = 0x0000FFFFu;  // join(tombstone fp 0xFFFF, val 0)
+
+// ---- launchers (defined in the .hip files) ----
+struct OpArgs {
+    const uint64_t *keys;
+    const void *vals;
+    uint64_t n;
+    void *vals_out;
+    uint8_t *flags;        // status (insert) or found (query/remove)
+    uint64_t *fail_count;
+    uint64_t *stats;       // optional accounting counters (pg_set_stats)
+    uint32_t with_delete;
+    uint32_t deterministic;
+};
+
+enum class Op { Insert, Query, Remove };
+
+int launch_tcf(const TableDev &T, uint32_t tile, Op op, const OpArgs &a, hipStream_t s);
+int launch_kv(const TableDev &T, uint32_t tile, Op op, const OpArgs &a, hipStream_t s);
+int launch_fill(const TableDev &T, uint64_t *d_counter, hipStream_t s);
+int launch_zero(void *p, uint64_t bytes, hipStream_t s);
+int launch_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *out, hipStream_t s);
+int launch_random_sector(void *buf, uint64_t bytes, uint64_t n, uint32_t op, uint64_t seed,
+                         uint64_t *sink, hipStream_t s);
+
+}  // namespace pg

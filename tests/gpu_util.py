@@ -1,0 +1,67 @@
+"""Helpers shared by the GPU parity tests: build matching oracle / HIP tables
+from one descriptor and move uint64 arrays between numpy and torch."""
+import numpy as np
+import torch
+
+import oracle
+import poggers_amd as pg
+
+
+def dev_u64(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint64).view(np.int64)).cuda()
+
+
+def dev_u16(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint16).view(np.int16)).cuda()
+
+
+def host(t: torch.Tensor, dtype) -> np.ndarray:
+    return t.cpu().numpy().view(dtype)
+
+
+def pair(desc: pg.TableDesc, seed: int = 42):
+    """(oracle table, GPU table) built from the same descriptor."""
+    o = oracle.Table(desc.rep, [lv.as_dict() for lv in desc.levels], seed=seed)
+    g = pg.Table(desc, seed=seed)
+    return o, g
+
+
+class Ops:
+    """Bulk ops on a GPU table with host arrays in and out."""
+
+    def __init__(self, table: pg.Table):
+        self.t = table
+        self.kv = table.is_kv
+
+    def insert(self, keys, vals=None, with_delete=False):
+        n = len(keys)
+        k = dev_u64(keys)
+        v = None
+        if vals is not None:
+            v = dev_u64(vals) if self.kv else dev_u16(vals)
+        st = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        fails = torch.zeros(1, dtype=torch.int64, device="cuda")
+        self.t.bulk_insert(k, v, st, fails, with_delete=with_delete)
+        torch.cuda.synchronize()
+        return st.cpu().numpy(), int(fails.item())
+
+    def query(self, keys):
+        n = len(keys)
+        k = dev_u64(keys)
+        found = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        vals = torch.zeros(n, dtype=torch.int64 if self.kv else torch.int16, device="cuda")
+        self.t.bulk_query(k, vals, found)
+        torch.cuda.synchronize()
+        return found.cpu().numpy(), host(vals, np.uint64 if self.kv else np.uint16)
+
+    def remove(self, keys):
+        n = len(keys)
+        k = dev_u64(keys)
+        found = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        self.t.bulk_remove(k, found)
+        torch.cuda.synchronize()
+        return found.cpu().numpy()
+
+    def level(self, lvl):
+        dt = np.uint64 if self.kv else np.uint32
+        return np.frombuffer(self.t.level_bytes(lvl), dtype=dt)

@@ -1,0 +1,108 @@
+"""GPU parity of the key-value tables (key_val_pair<u64,u64>) against the CPU
+oracle: bit-exact layouts in deterministic mode, bit-exact query results,
+deletes and re-queries."""
+import numpy as np
+import pytest
+
+import oracle
+import poggers_amd as pg
+from tests.gpu_util import Ops, pair
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+@pytest.mark.parametrize("tile", [1, 4, 8])
+def test_linear_2p20_config(tile):
+    """2^20 slots, linear probing, fmix64, 0.85 load, value = key ^ golden;
+    all inserted keys plus 2^20 absent keys queried: bit-exact with the oracle."""
+    m = 1 << 20
+    desc = pg.kv_desc(m, bucket_size=1, tile=tile, prober=pg.PROBE_LINEAR)
+    o, g = pair(desc)
+    ops = Ops(g)
+    n = int(0.85 * m)
+    keys = oracle.gen_keys(42, 0, n)
+    vals = keys ^ GOLD
+    st, fails = ops.insert(keys, vals)
+    assert fails == 0 and (st == 1).all()
+    o.bulk_insert(keys, vals)
+    q = np.concatenate([keys, oracle.gen_keys(42, 1 << 40, 1 << 20)])
+    f_o, v_o = o.bulk_query(q)
+    f_g, v_g = ops.query(q)
+    assert (f_g == f_o).all() and (v_g == v_o).all()
+    assert f_g[:n].all() and not f_g[n:].any()
+    assert g.get_fill() == n
+
+
+@pytest.mark.parametrize("tile,B,prober,hasher", [
+    (1, 1, pg.PROBE_LINEAR, pg.HASH_FMIX64),
+    (4, 1, pg.PROBE_LINEAR, pg.HASH_FMIX64),
+    (4, 16, pg.PROBE_DOUBLE, pg.HASH_MURMUR64A),
+    (16, 16, pg.PROBE_DOUBLE, pg.HASH_FMIX64),
+    (8, 8, pg.PROBE_DOUBLE, pg.HASH_MURMUR64A),
+    (2, 4, pg.PROBE_XOR_P2, pg.HASH_FMIX64),
+    (64, 1, pg.PROBE_DOUBLE, pg.HASH_FMIX64),
+])
+def test_deterministic_layout_bit_exact(tile, B, prober, hasher):
+    m = 1 << 12
+    mp = (m // B) if prober == pg.PROBE_LINEAR else (2 if prober == pg.PROBE_XOR_P2 else 20)
+    desc = pg.kv_desc(m, bucket_size=B, tile=tile, prober=prober, hasher=hasher, max_probes=mp)
+    o, g = pair(desc)
+    g.set_mode(pg.DETERMINISTIC)
+    ops = Ops(g)
+    keys = oracle.gen_keys(6, 0, int(0.97 * m))
+    vals = keys * np.uint64(3)
+    st_o = o.bulk_insert(keys, vals)
+    st_g, fails = ops.insert(keys, vals)
+    assert (st_g == st_o).all() and fails == int((st_o == 0).sum())
+    assert (ops.level(0) == o.level_array(0)).all()
+    q = np.concatenate([keys, oracle.gen_keys(6, 1 << 40, 2048)])
+    f_o, v_o = o.bulk_query(q)
+    f_g, v_g = ops.query(q)
+    assert (f_g == f_o).all() and (v_g == v_o).all()
+    rm = keys[1::4]
+    assert (ops.remove(rm) == o.bulk_remove(rm)).all()
+    assert (ops.level(0) == o.level_array(0)).all()
+    f_o, v_o = o.bulk_query(q)
+    f_g, v_g = ops.query(q)
+    assert (f_g == f_o).all() and (v_g == v_o).all()
+
+
+def test_double_hashing_delete_requery_concurrent():
+    """hash_table_64-style table (B=16, P=4, double hashing, 20 probes):
+    fill to 0.9, delete 10 %, re-query; results bit-exact with the oracle."""
+    m = 1 << 20
+    desc = pg.kv_desc(m, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, hasher=pg.HASH_FMIX64, max_probes=20)
+    o, g = pair(desc)
+    ops = Ops(g)
+    n = int(0.9 * m)
+    keys = oracle.gen_keys(77, 0, n)
+    vals = keys ^ GOLD
+    st, fails = ops.insert(keys, vals)
+    o.bulk_insert(keys, vals)
+    assert fails == 0
+    rm = keys[: n // 10]
+    assert ops.remove(rm).all()
+    o.bulk_remove(rm)
+    q = np.concatenate([keys, oracle.gen_keys(77, 1 << 40, 1 << 18)])
+    f_o, v_o = o.bulk_query(q)
+    f_g, v_g = ops.query(q)
+    assert (f_g == f_o).all() and (v_g == v_o).all()
+    assert not f_g[: n // 10].any() and f_g[n // 10:n].all()
+    assert g.get_fill() == n - n // 10
+
+
+def test_single_slot_scheme_and_unsupported_ops():
+    desc = pg.kv_desc(1 << 10, bucket_size=1, tile=4, prober=pg.PROBE_DOUBLE, max_probes=5,
+                      scheme=pg.SCHEME_SINGLE_SLOT)
+    o, g = pair(desc)
+    ops = Ops(g)
+    keys = oracle.gen_keys(2, 0, 600)
+    g.set_mode(pg.DETERMINISTIC)
+    assert (ops.insert(keys, keys)[0] == o.bulk_insert(keys, keys)).all()
+    assert (ops.level(0) == o.level_array(0)).all()
+    with pytest.raises(pg.PgError):
+        ops.remove(keys[:10])
+    with pytest.raises(pg.PgError):
+        ops.insert(keys[:10], keys[:10], with_delete=True)

@@ -1,0 +1,244 @@
+"""Pin the oracle's Two-Choice Filter against hand-derived placements.
+
+Every expected slot below was worked out by hand from the reference's insert
+scheme (insert_schemes/power_of_n_shortcut_buckets.cuh:198-306), bucket
+routines (representations/packed_bucket.cuh) and backing scheme
+(insert_schemes/linear_insert_buckets.cuh), not by calling the oracle: a
+wrong threshold, tie order, slot order or stopping rule fails one of them.
+The probe functions used to pick keys are pinned in test_oracle_hash.py.
+"""
+import random
+
+import numpy as np
+import pytest
+
+NB0, B0 = 4, 16     # primary: 64 slots -> 4 buckets of 16
+NB1, B1 = 2, 8      # backing: 16 slots -> 2 buckets of 8
+SEED = 42
+TOMB = 0x0000FFFF
+
+
+def fp16(key):
+    f = key & 0xFFFF
+    if f == 0:
+        f = 1
+    if f == 0xFFFF:
+        f = 0xFFFE
+    return f
+
+
+def word(key, val):
+    return ((val & 0xFFFF) << 16) | fp16(key)
+
+
+class Picker:
+    """Draws random keys with prescribed (b0, b1) and distinct fingerprints."""
+
+    def __init__(self, o, seed=5):
+        self.o = o
+        self.rng = random.Random(seed)
+        self.used_fp = set()
+
+    def buckets(self, key):
+        p = self.o.probes(self.o.PROBE_SPLIT64, self.o.HASH_FMIX64, 2, SEED, key, fp16(key))
+        return p[0] % NB0, p[1] % NB0
+
+    def backing_bucket(self, key, i=0):
+        h = self.o.hash64(self.o.HASH_FMIX64, key, SEED * SEED)
+        return ((h + i) & ((1 << 64) - 1)) % NB1
+
+    def key(self, b0, b1, bb=None, fp=None):
+        while True:
+            k = self.rng.getrandbits(64)
+            if fp is not None:
+                k = (k & ~0xFFFF) | fp
+            f = fp16(k)
+            if (fp is None and f in self.used_fp) or f in (1, 0xFFFE):
+                continue
+            if self.buckets(k) != (b0, b1):
+                continue
+            if bb is not None and self.backing_bucket(k) != bb:
+                continue
+            if fp is None:
+                self.used_fp.add(f)
+            return k
+
+
+def make(o):
+    return o.Table(o.REP_FP16_V16, [
+        o.level(NB0 * B0, o.SCHEME_SHORTCUT, B0, prober=o.PROBE_SPLIT64, hasher=o.HASH_FMIX64, max_probes=2),
+        o.level(NB1 * B1, o.SCHEME_LINEAR_BUCKETS, B1, prober=o.PROBE_LINEAR, hasher=o.HASH_FMIX64,
+                max_probes=4000),
+    ], seed=SEED)
+
+
+def bucket(t, lvl, b, B):
+    return t.level_array(lvl)[b * B:(b + 1) * B]
+
+
+def test_fingerprint_remap():
+    # dynamic_container.cuh:35-50: low 16 bits of the raw key, 0 -> 1, 0xFFFF -> 0xFFFE
+    assert [fp16(k) for k in (0, 0xFFFF, 0x10000, 0x1FFFF, 0x1234)] == [1, 0xFFFE, 1, 0xFFFE, 0x1234]
+
+
+def test_pack_roundtrip_exhaustive_8_8(oracle_lib):
+    # tests/src/kmer_speed_tests.cu:664-684: 256 x 256 join / retrieve.
+    o = oracle_lib.lib()
+    for k in range(256):
+        for v in range(256):
+            w = o.or_pack(8, 8, k, v)
+            assert o.or_unpack_fp(8, w) == k and o.or_unpack_val(8, 8, w) == v
+
+
+def test_pack_16_16(oracle_lib):
+    o = oracle_lib.lib()
+    rng = random.Random(3)
+    for _ in range(5000):
+        k, v = rng.getrandbits(16), rng.getrandbits(16)
+        w = o.or_pack(16, 16, k, v)
+        assert w == (v << 16) | k
+        assert o.or_unpack_fp(16, w) == k and o.or_unpack_val(16, 16, w) == v
+
+
+def test_shortcut_then_second_choice_then_fill_order(oracle_lib):
+    o = oracle_lib
+    t = make(o)
+    pk = Picker(o)
+    # 12 keys whose first choice is bucket 0: fill < 12 each time, so each
+    # lands in bucket 0 at the lowest empty slot, in insert order.
+    ks = [pk.key(0, 1) for _ in range(12)]
+    for i, k in enumerate(ks):
+        assert t.insert(k, 100 + i) == 1
+    exp0 = [word(k, 100 + i) for i, k in enumerate(ks)] + [0] * 4
+    assert list(bucket(t, 0, 0, B0)) == exp0
+    # fill(0) == 12 is not < 0.75*16: the second choice (bucket 1, fill 0) takes it.
+    k13 = pk.key(0, 1)
+    assert t.insert(k13, 7) == 1
+    assert bucket(t, 0, 1, B0)[0] == word(k13, 7)
+    assert bucket(t, 0, 0, B0)[12] == 0
+    # One more in bucket 0 via pass 2?  First fill bucket 1 to 12 through its
+    # own first choice.
+    fill1 = [pk.key(1, 2) for _ in range(11)]
+    for k in fill1:
+        assert t.insert(k, 1) == 1
+    assert (bucket(t, 0, 1, B0) != 0).sum() == 12
+    # Now key (b0=0, b1=1): fills 12 and 12 -> pass 2, lowest level 12, probe
+    # order: bucket 0 first, slot 12.
+    ka = pk.key(0, 1)
+    assert t.insert(ka, 2) == 1
+    assert bucket(t, 0, 0, B0)[12] == word(ka, 2)
+    # Fills now 13 (bucket 0) and 12 (bucket 1): key (0, 1) -> pass 2 picks
+    # the lower fill, bucket 1, slot 12.
+    kb = pk.key(0, 1)
+    assert t.insert(kb, 3) == 1
+    assert bucket(t, 0, 1, B0)[12] == word(kb, 3)
+    # Both at 13 now; key (1, 0): tie -> probe order -> bucket 1 slot 13.
+    kc = pk.key(1, 0)
+    assert t.insert(kc, 4) == 1
+    assert bucket(t, 0, 1, B0)[13] == word(kc, 4)
+    assert t.fill() == 12 + 1 + 11 + 3
+
+
+def test_full_buckets_spill_to_backing_and_backing_order(oracle_lib):
+    o = oracle_lib
+    t = make(o)
+    pk = Picker(o, seed=9)
+    for _ in range(16):
+        assert t.insert(pk.key(2, 3), 1) == 1
+    for _ in range(16):
+        assert t.insert(pk.key(3, 2), 1) == 1
+    assert (bucket(t, 0, 2, B0) != 0).all() and (bucket(t, 0, 3, B0) != 0).all()
+    # Both choices full -> backing level (status 2), first backing probe
+    # bucket, lowest slot.
+    spill = [pk.key(2, 3, bb=0) for _ in range(8)]
+    for i, k in enumerate(spill):
+        assert t.insert(k, 50 + i) == 2
+    assert list(bucket(t, 1, 0, B1)) == [word(k, 50 + i) for i, k in enumerate(spill)]
+    # Backing bucket 0 is full: the next spill whose first backing bucket is 0
+    # probes linearly into bucket 1.
+    late = pk.key(3, 2, bb=0)
+    assert t.insert(late, 60) == 2
+    assert bucket(t, 1, 1, B1)[0] == word(late, 60)
+    found, v = t.query(late)
+    assert found and v == 60
+    # linear_insert_buckets.cuh:362-398: a query stops at the first probe
+    # bucket whose fill != B.  Tombstone one slot of backing bucket 0 and the
+    # key in bucket 1 becomes unreachable (the reference's behaviour).
+    assert t.remove(spill[3]) == 1
+    assert bucket(t, 1, 0, B1)[3] == TOMB
+    assert t.query(late) == (False, 0)
+
+
+def test_remove_tombstone_and_insert_with_delete(oracle_lib):
+    o = oracle_lib
+    t = make(o)
+    pk = Picker(o, seed=13)
+    ks = [pk.key(0, 1) for _ in range(5)]
+    for i, k in enumerate(ks):
+        t.insert(k, i)
+    assert t.remove(ks[1]) == 1
+    assert list(bucket(t, 0, 0, B0)[:6]) == [word(ks[0], 0), TOMB, word(ks[2], 2), word(ks[3], 3),
+                                             word(ks[4], 4), 0]
+    assert t.query(ks[1]) == (False, 0)
+    assert t.fill() == 4
+    # plain insert claims the lowest EMPTY slot (packed_bucket.cuh:64-114) ...
+    k5 = pk.key(0, 1)
+    t.insert(k5, 5)
+    assert bucket(t, 0, 0, B0)[5] == word(k5, 5) and bucket(t, 0, 0, B0)[1] == TOMB
+    # ... insert_with_delete the lowest empty-or-tombstone (packed_bucket.cuh:116-180)
+    k6 = pk.key(0, 1)
+    t.insert(k6, 6, with_delete=True)
+    assert bucket(t, 0, 0, B0)[1] == word(k6, 6)
+    # remove of an absent key changes nothing
+    before = t.level_array(0).copy()
+    assert t.remove(pk.key(0, 1)) == 0
+    assert (t.level_array(0) == before).all()
+
+
+def test_query_returns_lowest_fp_match_and_false_positive(oracle_lib):
+    o = oracle_lib
+    t = make(o)
+    pk = Picker(o, seed=21)
+    a = pk.key(0, 1, fp=0x4242)
+    b = pk.key(0, 1, fp=0x4242)
+    t.insert(a, 11)
+    t.insert(b, 22)
+    # b shares a's fingerprint: the lowest matching slot (a's) answers for b
+    assert t.query(b) == (True, 11)
+    # a never-inserted key with that fingerprint in bucket 1 only misses
+    # bucket 0 ... it is looked up in both choices (no early exit)
+    c = pk.key(1, 0, fp=0x4242)
+    assert t.query(c) == (True, 11)
+    # remove(b) tombstones the lowest match, i.e. a's slot
+    assert t.remove(b) == 1
+    assert t.query(a) == (True, 22)
+
+
+def test_fp_rate_matches_closed_form(oracle_lib):
+    """FPR of a query that examines slots holding f fingerprints is ~ f * q,
+    with q = sum_v Pr[fp = v]^2 (fp 1 and 0xFFFE are twice as likely).  For a
+    never-inserted key the two primary choices are uniform buckets, so
+    E[examined fill] = 2 * items_in_primary / nb, plus the backing term."""
+    o = oracle_lib
+    nslots = 1 << 18
+    t = o.Table(o.REP_FP16_V16, [
+        o.level(nslots, o.SCHEME_SHORTCUT, 16),
+        o.level(nslots // 100, o.SCHEME_LINEAR_BUCKETS, 8, prober=o.PROBE_LINEAR, max_probes=4000),
+    ])
+    n = int(0.9 * (nslots + nslots // 100))
+    keys = o.gen_keys(1, 0, n)
+    st = t.bulk_insert(keys)
+    assert (st > 0).all()
+    in_backing = int((st == 2).sum())
+    absent = o.gen_keys(1, 1 << 40, 1 << 21)
+    found, _ = t.bulk_query(absent)
+    q = (65532 + 8) / 2.0 ** 32
+    nb = nslots // 16
+    backing_fill = in_backing / (nslots // 100 // 8)  # mean fill of a backing bucket
+    expect = (2 * (n - in_backing) / nb + backing_fill) * q
+    got = found.mean()
+    sigma = (expect / len(absent)) ** 0.5
+    assert abs(got - expect) < 4 * sigma, (got, expect)
+    # zero false negatives
+    f2, _ = t.bulk_query(keys)
+    assert f2.all()
