@@ -196,10 +196,12 @@ const char *pg_error_string(int status);
  * streams every benchmark input is drawn from). */
 int pg_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *d_out, void *stream);
 
-/* Random 64 B-sector read microbenchmark over d_buf (bytes, multiple of
- * 64): n_reads independent reads of 64 B at hashed sector indices, by
- * 4-lane tiles of 16 B per lane, each tile reading `items` sectors.
- * `op` 0 = read, 1 = read + CAS one word of the sector.  Writes one
+/* Random 64 B-sector microbenchmark (the measured random-access roofline):
+ * n_reads independent 64 B reads at hashed sector indices of d_buf (bytes, a
+ * multiple of 64).  `op` bit 0 adds a CAS on one word of each sector (the
+ * insert pattern: read, then dirty the line); bits 4-7 = log2 sectors in
+ * flight per tile (0 -> 4); bit 8 = nontemporal loads; bits 9-10 = tile
+ * shape (0: 4 lanes x 16 B, 1: 1 lane x 64 B, 2: 16 lanes x 4 B).  Writes a
  * checksum word to d_sink so nothing is optimised away. */
 int pg_random_sector_bench(void *d_buf, uint64_t bytes, uint64_t n_reads, uint32_t op,
                            uint64_t seed, uint64_t *d_sink, void *stream);
@@ -210,6 +212,14 @@ int pg_random_sector_bench(void *d_buf, uint64_t bytes, uint64_t n_reads, uint32
  * of a call (each bucket examination reads B slot words; each write dirties
  * one 64 B sector).  NULL disables it.  No reference counterpart. */
 int pg_set_stats(pg_handle h, uint64_t *d_stats);
+
+/* Launch shape: keys handled per tile by concurrent bulk inserts and queries
+ * (1, 2 or 4; 0 = library default).  A tile issues the first-choice bucket
+ * loads of all its keys before examining any, trading registers for memory
+ * requests in flight.  Results do not depend on it.  The reference launches
+ * exactly one key per tile (tables/bucketed_table.cuh:395-403); this is the
+ * knob its manual tile/bucket sweeps tuned by recompiling. */
+int pg_set_items(pg_handle h, uint32_t insert_items, uint32_t query_items);
 
 /* Host-side range reduction used by every probe (exported for tests):
  * returns n % d computed with the library's invariant-divisor method. */

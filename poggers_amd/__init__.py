@@ -33,7 +33,7 @@ EXPORTS = [
     "pg_create", "pg_destroy", "pg_clear", "pg_set_mode", "pg_bulk_insert",
     "pg_bulk_insert_with_delete", "pg_bulk_query", "pg_bulk_remove", "pg_fill",
     "pg_bytes", "pg_num_slots", "pg_level_ptr", "pg_error_string", "pg_gen_keys",
-    "pg_random_sector_bench", "pg_fastmod_host", "pg_set_stats",
+    "pg_random_sector_bench", "pg_fastmod_host", "pg_set_stats", "pg_set_items",
 ]
 
 
@@ -90,6 +90,7 @@ def lib():
     L.pg_gen_keys.argtypes = [u64, u64, u64, vp, vp]
     L.pg_random_sector_bench.argtypes = [vp, u64, u64, u32, u64, vp, vp]
     L.pg_set_stats.argtypes = [vp, vp]
+    L.pg_set_items.argtypes = [vp, u32, u32]
     L.pg_fastmod_host.argtypes = [u64, u64]
     L.pg_fastmod_host.restype = u64
     for name in EXPORTS:
@@ -239,6 +240,10 @@ class Table:
         """Accumulate algorithmic counters into a 3-element int64 CUDA tensor
         (None disables)."""
         check(lib().pg_set_stats(self._h, _ptr(stats)), "pg_set_stats")
+
+    def set_items(self, insert_items: int = 0, query_items: int = 0):
+        """Keys per tile for concurrent inserts / queries (1, 2, 4; 0 = default)."""
+        check(lib().pg_set_items(self._h, insert_items, query_items), "pg_set_items")
 
     def get_fill(self, stream=None) -> int:
         out = ctypes.c_uint64()

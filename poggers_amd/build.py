@@ -18,7 +18,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpoggers_amd.so")
 ARCH = os.environ.get("PG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["pg_capi.hip", "pg_tcf.hip", "pg_kv.hip", "pg_util.hip"]
+SOURCES = ["pg_capi.hip", "pg_tcf.hip", "pg_kv.hip", "pg_util.hip"] + [
+    f"pg_tcf_p{p}.hip" for p in (1, 2, 4, 8, 16, 32, 64)]
 
 # -packed-fp32-ops: no v_pk_*_f32 generation (none of this code is float, the
 # flag keeps it that way).  The host pass warns that it does not know the
@@ -45,7 +46,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), 8)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(len(SOURCES), int(os.environ.get('MAX_JOBS', '8')))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs]

@@ -14,6 +14,7 @@ struct pg_table_s {
     uint64_t bytes[PG_MAX_LEVELS];
     uint64_t *d_counter;  // 8 B device scratch for pg_fill
     uint64_t *d_stats;    // caller's accounting counters, or NULL
+    uint32_t items_insert, items_query;  // keys per tile (pg_set_items)
 };
 
 namespace {
@@ -175,6 +176,7 @@ static int do_insert(pg_handle h, const uint64_t *d_keys, const void *d_vals, ui
     a.with_delete = with_delete;
     a.deterministic = h->mode == PG_DETERMINISTIC;
     a.stats = h->d_stats;
+    a.items = h->items_insert;
     return run_op(h, pg::Op::Insert, a, (hipStream_t)stream);
 }
 
@@ -200,6 +202,7 @@ int pg_bulk_query(pg_handle h, const uint64_t *d_keys, uint64_t n, void *d_vals_
     a.vals_out = d_vals_out;
     a.flags = d_found;
     a.stats = h->d_stats;
+    a.items = h->items_query;
     return run_op(h, pg::Op::Query, a, (hipStream_t)stream);
 }
 
@@ -219,6 +222,14 @@ int pg_bulk_remove(pg_handle h, const uint64_t *d_keys, uint64_t n, uint8_t *d_f
 int pg_set_stats(pg_handle h, uint64_t *d_stats) {
     if (!h) return PG_ERR_INVALID;
     h->d_stats = d_stats;
+    return PG_OK;
+}
+
+int pg_set_items(pg_handle h, uint32_t insert_items, uint32_t query_items) {
+    auto ok = [](uint32_t v) { return v == 0 || v == 1 || v == 2 || v == 4; };
+    if (!h || !ok(insert_items) || !ok(query_items)) return PG_ERR_INVALID;
+    h->items_insert = insert_items;
+    h->items_query = query_items;
     return PG_OK;
 }
 
@@ -262,7 +273,7 @@ int pg_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *d_out, void
 
 int pg_random_sector_bench(void *d_buf, uint64_t bytes, uint64_t n_reads, uint32_t op, uint64_t seed,
                            uint64_t *d_sink, void *stream) {
-    if (!d_buf || !d_sink || bytes < 64 || (bytes % 64) || op > 1) return PG_ERR_INVALID;
+    if (!d_buf || !d_sink || bytes < 64 || (bytes % 64) || op >= (1u << 11)) return PG_ERR_INVALID;
     return pg::launch_random_sector(d_buf, bytes, n_reads, op, seed, d_sink, (hipStream_t)stream);
 }
 

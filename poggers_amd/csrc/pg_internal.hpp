@@ -169,6 +169,7 @@ struct OpArgs {
     uint64_t *stats;       // optional accounting counters (pg_set_stats)
     uint32_t with_delete;
     uint32_t deterministic;
+    uint32_t items;        // keys per tile (pg_set_items), 0 = default
 };
 
 enum class Op { Insert, Query, Remove };

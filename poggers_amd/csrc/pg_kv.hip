@@ -239,7 +239,7 @@ __device__ bool kv_remove_key(const TableDev &T, uint64_t key, const Tile<P> &t)
 
 constexpr int kBlock = 256;
 
-template <int P>
+template <int P, bool ACC>
 __global__ __launch_bounds__(kBlock) void k_kv_insert(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
@@ -251,10 +251,10 @@ __global__ __launch_bounds__(kBlock) void k_kv_insert(TableDev T, OpArgs a) {
         if (a.flags && t.leader()) a.flags[tid] = (uint8_t)st;
     }
     wave_count(a.fail_count, act && t.leader() && st == 0);
-    flush_acc(a.stats, t, act);
+    if constexpr (ACC) flush_acc(a.stats, t, act);
 }
 
-template <int P>
+template <int P, bool ACC>
 __global__ __launch_bounds__(kBlock) void k_kv_query(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
@@ -268,10 +268,10 @@ __global__ __launch_bounds__(kBlock) void k_kv_query(TableDev T, OpArgs a) {
             if (a.vals_out) ((uint64_t *)a.vals_out)[tid] = found ? val : 0ull;
         }
     }
-    flush_acc(a.stats, t, tid < a.n);
+    if constexpr (ACC) flush_acc(a.stats, t, tid < a.n);
 }
 
-template <int P>
+template <int P, bool ACC>
 __global__ __launch_bounds__(kBlock) void k_kv_remove(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_kv_remove(TableDev T, OpArgs a) {
         const bool ok = kv_remove_key<P>(T, a.keys[tid], t);
         if (a.flags && t.leader()) a.flags[tid] = ok ? 1 : 0;
     }
-    flush_acc(a.stats, t, tid < a.n);
+    if constexpr (ACC) flush_acc(a.stats, t, tid < a.n);
 }
 
 template <int P>
@@ -308,18 +308,24 @@ __global__ __launch_bounds__(64) void k_kv_remove_seq(TableDev T, OpArgs a) {
     flush_acc(a.stats, t, true);
 }
 
+template <int P, bool ACC>
+static void launch_kv_conc(Op op, unsigned grid, const TableDev &T, const OpArgs &a, hipStream_t s) {
+    if (op == Op::Query) hipLaunchKernelGGL((k_kv_query<P, ACC>), dim3(grid), dim3(kBlock), 0, s, T, a);
+    else if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert<P, ACC>), dim3(grid), dim3(kBlock), 0, s, T, a);
+    else hipLaunchKernelGGL((k_kv_remove<P, ACC>), dim3(grid), dim3(kBlock), 0, s, T, a);
+}
+
 template <int P>
 static int launch_kv_p(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
     if (a.n == 0) return PG_OK;
     const unsigned grid = (unsigned)((a.n * (uint64_t)P + kBlock - 1) / kBlock);
-    if (op == Op::Query) {
-        hipLaunchKernelGGL((k_kv_query<P>), dim3(grid), dim3(kBlock), 0, s, T, a);
-    } else if (a.deterministic) {
+    if (op != Op::Query && a.deterministic) {
         if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert_seq<P>), dim3(1), dim3(P), 0, s, T, a);
         else hipLaunchKernelGGL((k_kv_remove_seq<P>), dim3(1), dim3(P), 0, s, T, a);
+    } else if (a.stats) {
+        launch_kv_conc<P, true>(op, grid, T, a, s);
     } else {
-        if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert<P>), dim3(grid), dim3(kBlock), 0, s, T, a);
-        else hipLaunchKernelGGL((k_kv_remove<P>), dim3(grid), dim3(kBlock), 0, s, T, a);
+        launch_kv_conc<P, false>(op, grid, T, a, s);
     }
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
 }

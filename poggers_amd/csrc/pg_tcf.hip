@@ -1,645 +1,10 @@
-// pg_tcf.hip -- bulk insert / query / remove kernels for packed 16/16 filter
-// slots (the Two-Choice Filter and its linear backing table), gfx950 wave64.
-//
-// Semantics follow the reference's device ops:
-//   level 0 POWER_OF_N_SHORTCUT  insert_schemes/power_of_n_shortcut_buckets.cuh
-//   level 1 LINEAR_BUCKETS       insert_schemes/linear_insert_buckets.cuh
-//   buckets                      representations/packed_bucket.cuh
-//   slot words                   representations/grouped_storage_sub_bits.cuh
-// Every bucket routine acts on the LOWEST-index qualifying slot, which is what
-// the reference's ascending stripes + lowest-lane leader give for any tile
-// width, so results do not depend on P.
-//
-// Layout in HBM: one uint32 word per slot (val16 << 16 | fp16), buckets of B
-// contiguous words (64 B at B = 16), bucket b at word b*B.
+// pg_tcf.hip -- tile-width dispatch of the filter kernels (pg_tcf.hpp).
 #include "pg_internal.hpp"
-#include "pg_tile.hpp"
 
 namespace pg {
 
-__device__ inline bool w_empty(uint32_t w) { return (w & 0xFFFFu) == 0u; }
-__device__ inline bool w_tomb(uint32_t w) { return (w & 0xFFFFu) == 0xFFFFu; }
-__device__ inline bool w_filled(uint32_t w) { return !(w_empty(w) || w_tomb(w)); }
-
-template <int V>
-__device__ inline uint32_t select_word(const uint32_t (&w)[V], int v) {
-    uint32_t x = w[0];
-#pragma unroll
-    for (int i = 1; i < V; i++) x = (v == i) ? w[i] : x;
-    return x;
-}
-
-// ---------------------------------------------------------------------------
-// A bucket held in registers: compile-time B and P.
-// ---------------------------------------------------------------------------
-template <int P, int B>
-struct Bucket {
-    using Sh = Shape<P, B>;
-    static constexpr int S = Sh::S, V = Sh::V, ROW = Sh::ROW;
-    uint32_t w[S][V];
-
-    __device__ static inline bool ok(int s, const Tile<P> &t) {
-        return (uint32_t)(s * ROW) + t.rank * V < (uint32_t)B;
-    }
-
-    __device__ inline void load(const uint32_t *slots, uint64_t b, const Tile<P> &t) {
-        const uint32_t *base = slots + b * (uint64_t)B;
-        t.acc[0]++;
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            if (ok(s, t)) {
-                load_words<V>(base + s * ROW + t.rank * V, w[s]);
-            } else {
-#pragma unroll
-                for (int v = 0; v < V; v++) w[s][v] = 0;
-            }
-        }
-    }
-
-    // get_fill: packed_bucket.cuh:40-62
-    __device__ inline int fill(const Tile<P> &t) const {
-        int f = 0;
-#pragma unroll
-        for (int s = 0; s < S; s++)
-#pragma unroll
-            for (int v = 0; v < V; v++) f += t.popc(ok(s, t) && w_filled(w[s][v]));
-        return f;
-    }
-
-    // insert / insert_delete: claim the lowest empty (or tombstone) slot.
-    // packed_bucket.cuh:64-180; CAS words grouped_storage_sub_bits.cuh:127-157.
-    __device__ inline bool claim(uint32_t *slots, uint64_t b, uint32_t word, bool wd,
-                                 const Tile<P> &t) const {
-        uint32_t *base = slots + b * (uint64_t)B;
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            uint32_t cand = 0;
-            if (ok(s, t)) {
-#pragma unroll
-                for (int v = 0; v < V; v++)
-                    if (w_empty(w[s][v]) || (wd && w_tomb(w[s][v]))) cand |= 1u << v;
-            }
-            uint64_t lanes = t.bits(cand != 0);
-            while (lanes) {
-                const uint32_t l = (uint32_t)ctz64(lanes);
-                bool success = false;
-                if (t.rank == l) {
-                    const int v = __builtin_ctz(cand);
-                    const uint32_t cur = select_word<V>(w[s], v);
-                    const uint32_t expect = (wd && w_tomb(cur)) ? kTomb16 : 0u;
-                    success = atomicCAS(base + s * ROW + t.rank * V + v, expect, word) == expect;
-                    cand &= cand - 1u;
-                }
-                if (t.any(success)) {
-                    t.acc[2]++;
-                    return true;
-                }
-                lanes = t.bits(cand != 0);
-            }
-        }
-        return false;
-    }
-
-    // query: lowest matching slot, packed_bucket.cuh:182-219.  The owner lane
-    // (the one holding the match) receives the value.
-    __device__ inline bool find(uint32_t fp, const Tile<P> &t, uint32_t &val, bool &owner) const {
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            uint32_t m = 0;
-            if (ok(s, t)) {
-#pragma unroll
-                for (int v = 0; v < V; v++)
-                    if ((w[s][v] & 0xFFFFu) == fp) m |= 1u << v;
-            }
-            const uint64_t lanes = t.bits(m != 0);
-            if (lanes) {
-                const uint32_t l = (uint32_t)ctz64(lanes);
-                owner = (t.rank == l);
-                if (owner) val = select_word<V>(w[s], __builtin_ctz(m)) >> 16;
-                return true;
-            }
-        }
-        return false;
-    }
-
-    // remove: CAS the lowest match to the tombstone word, packed_bucket.cuh:
-    // 227-271, grouped_storage_sub_bits.cuh:217-234.
-    __device__ inline bool remove(uint32_t *slots, uint64_t b, uint32_t fp, const Tile<P> &t) const {
-        uint32_t *base = slots + b * (uint64_t)B;
-#pragma unroll
-        for (int s = 0; s < S; s++) {
-            uint32_t m = 0;
-            if (ok(s, t)) {
-#pragma unroll
-                for (int v = 0; v < V; v++)
-                    if ((w[s][v] & 0xFFFFu) == fp) m |= 1u << v;
-            }
-            uint64_t lanes = t.bits(m != 0);
-            while (lanes) {
-                const uint32_t l = (uint32_t)ctz64(lanes);
-                bool success = false;
-                if (t.rank == l) {
-                    const int v = __builtin_ctz(m);
-                    const uint32_t cur = select_word<V>(w[s], v);
-                    success = atomicCAS(base + s * ROW + t.rank * V + v, cur, kTomb16) == cur;
-                    m &= m - 1u;
-                }
-                if (t.any(success)) {
-                    t.acc[2]++;
-                    return true;
-                }
-                lanes = t.bits(m != 0);
-            }
-        }
-        return false;
-    }
-};
-
-// ---------------------------------------------------------------------------
-// Generic bucket routines: runtime B, one word per lane per row, re-read per
-// pass (the reference reads the bucket once for its fill and again for the
-// claim).  Used by the backing level and by shapes without a fast path.
-// ---------------------------------------------------------------------------
 template <int P>
-__device__ int gen_fill(const uint32_t *base, uint32_t B, const Tile<P> &t) {
-    int f = 0;
-    for (uint32_t s0 = 0; s0 < B; s0 += P) {
-        const uint32_t i = s0 + t.rank;
-        const bool ok = i < B;
-        const uint32_t w = ok ? base[i] : 0u;
-        f += t.popc(ok && w_filled(w));
-    }
-    return f;
-}
-
-template <int P>
-__device__ bool gen_claim(uint32_t *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t) {
-    for (uint32_t s0 = 0; s0 < B; s0 += P) {
-        const uint32_t i = s0 + t.rank;
-        const bool ok = i < B;
-        const uint32_t w = ok ? base[i] : 0u;
-        bool cand = ok && (w_empty(w) || (wd && w_tomb(w)));
-        uint64_t lanes = t.bits(cand);
-        while (lanes) {
-            const uint32_t l = (uint32_t)ctz64(lanes);
-            bool success = false;
-            if (t.rank == l) {
-                const uint32_t expect = (wd && w_tomb(w)) ? kTomb16 : 0u;
-                success = atomicCAS(base + i, expect, word) == expect;
-                cand = false;
-            }
-            if (t.any(success)) {
-                t.acc[2]++;
-                return true;
-            }
-            lanes = t.bits(cand);
-        }
-    }
-    return false;
-}
-
-// Lowest match; when none, *fill receives the bucket's fill (same pass).
-template <int P>
-__device__ bool gen_find(const uint32_t *base, uint32_t B, uint32_t fp, const Tile<P> &t,
-                         uint32_t &val, bool &owner, int &fill) {
-    fill = 0;
-    for (uint32_t s0 = 0; s0 < B; s0 += P) {
-        const uint32_t i = s0 + t.rank;
-        const bool ok = i < B;
-        const uint32_t w = ok ? base[i] : 0u;
-        const uint64_t lanes = t.bits(ok && (w & 0xFFFFu) == fp);
-        if (lanes) {
-            const uint32_t l = (uint32_t)ctz64(lanes);
-            owner = (t.rank == l);
-            if (owner) val = w >> 16;
-            return true;
-        }
-        fill += t.popc(ok && w_filled(w));
-    }
-    return false;
-}
-
-template <int P>
-__device__ bool gen_remove(uint32_t *base, uint32_t B, uint32_t fp, const Tile<P> &t, int &fill) {
-    fill = 0;
-    for (uint32_t s0 = 0; s0 < B; s0 += P) {
-        const uint32_t i = s0 + t.rank;
-        const bool ok = i < B;
-        const uint32_t w = ok ? base[i] : 0u;
-        bool m = ok && (w & 0xFFFFu) == fp;
-        uint64_t lanes = t.bits(m);
-        while (lanes) {
-            const uint32_t l = (uint32_t)ctz64(lanes);
-            bool success = false;
-            if (t.rank == l) {
-                success = atomicCAS(base + i, w, kTomb16) == w;
-                m = false;
-            }
-            if (t.any(success)) {
-                t.acc[2]++;
-                return true;
-            }
-            lanes = t.bits(m);
-        }
-        fill += t.popc(ok && w_filled(w));
-    }
-    return false;
-}
-
-// ---------------------------------------------------------------------------
-// Level operations, generic shapes.
-// ---------------------------------------------------------------------------
-
-// power_of_n_shortcut_buckets.cuh:198-306 (insert) and :309-417
-// (insert_with_delete), runtime B.
-template <int P>
-__device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word,
-                                    bool wd, const Tile<P> &t, int ai) {
-    Probe pr;
-    pr.begin(L, key, fp);
-    uint32_t *slots = (uint32_t *)L.slots;
-    const int B = (int)L.B;
-    uint64_t bidx[kMaxShortcutProbes];
-    int fill[kMaxShortcutProbes];
-    int np = 0, min_fill = B;
-#pragma unroll
-    for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
-        const uint64_t p = pr.at(j);
-        if (p == kEnd) break;
-        const uint64_t b = fastmod(p, L.nb);
-        bidx[j] = b;
-        t.acc[ai]++;
-        int f = gen_fill<P>(slots + b * L.B, L.B, t);
-        if (4 * f < 3 * B) {
-            if (gen_claim<P>(slots + b * L.B, L.B, word, wd, t)) return true;
-            f = B;
-        }
-        fill[j] = f;
-        if (f < min_fill) min_fill = f;
-        np = j + 1;
-    }
-    int lvl = min_fill;
-    while (lvl < B) {
-        int next = B;
-#pragma unroll
-        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
-            if (j < np) {
-                if (fill[j] == lvl && gen_claim<P>(slots + bidx[j] * L.B, L.B, word, wd, t)) return true;
-                if (fill[j] > lvl && fill[j] < next) next = fill[j];
-            }
-        }
-        lvl = next;
-    }
-    return false;
-}
-
-// linear_insert_buckets.cuh:200-244: first probe bucket with a free slot.
-template <int P>
-__device__ bool gen_linear_insert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word,
-                                  bool wd, const Tile<P> &t, int ai) {
-    Probe pr;
-    pr.begin(L, key, fp);
-    uint32_t *slots = (uint32_t *)L.slots;
-    for (uint32_t j = 0;; j++) {
-        const uint64_t p = pr.at(j);
-        if (p == kEnd) return false;
-        const uint64_t b = fastmod(p, L.nb);
-        t.acc[ai]++;
-        if (gen_claim<P>(slots + b * L.B, L.B, word, wd, t)) return true;
-    }
-}
-
-template <int P>
-__device__ bool gen_level_insert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word, bool wd,
-                                 const Tile<P> &t, int ai) {
-    if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT) return gen_shortcut_insert<P>(L, key, fp, word, wd, t, ai);
-    return gen_linear_insert<P>(L, key, fp, word, wd, t, ai);
-}
-
-// Shortcut query (power_of_n_shortcut_buckets.cuh:676-707): every probe
-// bucket, first match wins.  Linear query (linear_insert_buckets.cuh:362-398):
-// stop at the first probe bucket whose fill != B.
-template <int P>
-__device__ bool gen_level_query(const LevelDev &L, uint64_t key, uint32_t fp, const Tile<P> &t,
-                                uint32_t &val, bool &owner, int ai) {
-    Probe pr;
-    pr.begin(L, key, fp);
-    const uint32_t *slots = (const uint32_t *)L.slots;
-    const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
-    for (uint32_t j = 0;; j++) {
-        const uint64_t p = pr.at(j);
-        if (p == kEnd) return false;
-        const uint64_t b = fastmod(p, L.nb);
-        int f;
-        t.acc[ai]++;
-        if (gen_find<P>(slots + b * L.B, L.B, fp, t, val, owner, f)) return true;
-        if (linear && f != (int)L.B) return false;
-    }
-}
-
-// Shortcut remove (power_of_n_shortcut_buckets.cuh:709-740); linear remove
-// (linear_insert_buckets.cuh:323-360) stops at the first bucket with fill != B.
-template <int P>
-__device__ bool gen_level_remove(const LevelDev &L, uint64_t key, uint32_t fp, const Tile<P> &t, int ai) {
-    Probe pr;
-    pr.begin(L, key, fp);
-    uint32_t *slots = (uint32_t *)L.slots;
-    const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
-    for (uint32_t j = 0;; j++) {
-        const uint64_t p = pr.at(j);
-        if (p == kEnd) return false;
-        const uint64_t b = fastmod(p, L.nb);
-        int f;
-        t.acc[ai]++;
-        if (gen_remove<P>(slots + b * L.B, L.B, fp, t, f)) return true;
-        if (linear && f != (int)L.B) return false;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Level-0 fast path: POWER_OF_N_SHORTCUT with compile-time B; the first two
-// probe buckets stay in registers for both passes.
-// ---------------------------------------------------------------------------
-template <int P, int B>
-__device__ bool fast_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word,
-                                     bool wd, const Tile<P> &t) {
-    Probe pr;
-    pr.begin(L, key, fp);
-    uint32_t *slots = (uint32_t *)L.slots;
-    Bucket<P, B> bk0, bk1;
-    uint64_t bidx[kMaxShortcutProbes];
-    int fill[kMaxShortcutProbes];
-    int np = 0, min_fill = B;
-#pragma unroll
-    for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
-        const uint64_t p = pr.at(j);
-        if (p == kEnd) break;
-        const uint64_t b = fastmod(p, L.nb);
-        bidx[j] = b;
-        int f;
-        if (j == 0) {
-            bk0.load(slots, b, t);
-            f = bk0.fill(t);
-            if (4 * f < 3 * B) {
-                if (bk0.claim(slots, b, word, wd, t)) return true;
-                f = B;
-            }
-        } else if (j == 1) {
-            bk1.load(slots, b, t);
-            f = bk1.fill(t);
-            if (4 * f < 3 * B) {
-                if (bk1.claim(slots, b, word, wd, t)) return true;
-                f = B;
-            }
-        } else {
-            t.acc[0]++;
-            f = gen_fill<P>(slots + b * B, B, t);
-            if (4 * f < 3 * B) {
-                if (gen_claim<P>(slots + b * B, B, word, wd, t)) return true;
-                f = B;
-            }
-        }
-        fill[j] = f;
-        if (f < min_fill) min_fill = f;
-        np = j + 1;
-    }
-    int lvl = min_fill;
-    while (lvl < B) {
-        int next = B;
-#pragma unroll
-        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
-            if (j < np) {
-                if (fill[j] == lvl) {
-                    bool ok;
-                    if (j == 0) ok = bk0.claim(slots, bidx[0], word, wd, t);
-                    else if (j == 1) ok = bk1.claim(slots, bidx[1], word, wd, t);
-                    else ok = gen_claim<P>(slots + bidx[j] * B, B, word, wd, t);
-                    if (ok) return true;
-                }
-                if (fill[j] > lvl && fill[j] < next) next = fill[j];
-            }
-        }
-        lvl = next;
-    }
-    return false;
-}
-
-template <int P, int B>
-__device__ bool fast_shortcut_remove(const LevelDev &L, uint64_t key, uint32_t fp, const Tile<P> &t) {
-    Probe pr;
-    pr.begin(L, key, fp);
-    uint32_t *slots = (uint32_t *)L.slots;
-    for (uint32_t j = 0;; j++) {
-        const uint64_t p = pr.at(j);
-        if (p == kEnd) return false;
-        const uint64_t b = fastmod(p, L.nb);
-        Bucket<P, B> bk;
-        bk.load(slots, b, t);
-        if (bk.remove(slots, b, fp, t)) return true;
-    }
-}
-
-// Whole-table ops (bucketed_table.cuh:271-381): level 0, then the backing
-// level on false.  B == 0 selects the generic level-0 path.
-template <int P, int B>
-__device__ int tcf_insert_key(const TableDev &T, uint64_t key, uint32_t val, bool wd, const Tile<P> &t) {
-    const uint32_t fp = fp16_of(key);
-    const uint32_t word = ((val & 0xFFFFu) << 16) | fp;
-    bool ok;
-    if constexpr (B != 0) ok = fast_shortcut_insert<P, B>(T.lv[0], key, fp, word, wd, t);
-    else ok = gen_level_insert<P>(T.lv[0], key, fp, word, wd, t, 0);
-    if (ok) return 1;
-    for (uint32_t l = 1; l < T.nlevels; l++)
-        if (gen_level_insert<P>(T.lv[l], key, fp, word, wd, t, 1)) return 1 + (int)l;
-    return 0;
-}
-
-template <int P, int B>
-__device__ bool tcf_remove_key(const TableDev &T, uint64_t key, const Tile<P> &t) {
-    const uint32_t fp = fp16_of(key);
-    bool ok;
-    if constexpr (B != 0) ok = fast_shortcut_remove<P, B>(T.lv[0], key, fp, t);
-    else ok = gen_level_remove<P>(T.lv[0], key, fp, t, 0);
-    if (ok) return true;
-    for (uint32_t l = 1; l < T.nlevels; l++)
-        if (gen_level_remove<P>(T.lv[l], key, fp, t, 1)) return true;
-    return false;
-}
-
-// ---------------------------------------------------------------------------
-// Kernels
-// ---------------------------------------------------------------------------
-constexpr int kBlock = 256;
-
-template <int P, int B>
-__global__ __launch_bounds__(kBlock) void k_tcf_insert(TableDev T, OpArgs a) {
-    Tile<P> t;
-    const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
-    const bool act = tid < a.n;
-    int st = 1;
-    if (act) {
-        const uint64_t key = a.keys[tid];
-        const uint32_t val = a.vals ? ((const uint16_t *)a.vals)[tid] : 0u;
-        st = tcf_insert_key<P, B>(T, key, val, a.with_delete != 0, t);
-        if (a.flags && t.leader()) a.flags[tid] = (uint8_t)st;
-    }
-    wave_count(a.fail_count, act && t.leader() && st == 0);
-    flush_acc(a.stats, t, act);
-}
-
-template <int P, int B>
-__global__ __launch_bounds__(kBlock) void k_tcf_remove(TableDev T, OpArgs a) {
-    Tile<P> t;
-    const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
-    if (tid < a.n) {
-        const bool ok = tcf_remove_key<P, B>(T, a.keys[tid], t);
-        if (a.flags && t.leader()) a.flags[tid] = ok ? 1 : 0;
-    }
-    flush_acc(a.stats, t, tid < a.n);
-}
-
-// Deterministic mode: one tile applies the keys in array order.  The fence
-// after every key makes the next key's loads observe this key's CAS.
-template <int P, int B>
-__global__ __launch_bounds__(64) void k_tcf_insert_seq(TableDev T, OpArgs a) {
-    Tile<P> t;
-    uint64_t fails = 0;
-    for (uint64_t i = 0; i < a.n; i++) {
-        const uint64_t key = a.keys[i];
-        const uint32_t val = a.vals ? ((const uint16_t *)a.vals)[i] : 0u;
-        const int st = tcf_insert_key<P, B>(T, key, val, a.with_delete != 0, t);
-        if (a.flags && t.leader()) a.flags[i] = (uint8_t)st;
-        fails += (st == 0);
-        fence_agent();
-    }
-    if (a.fail_count && t.leader() && fails) atomicAdd((unsigned long long *)a.fail_count, fails);
-    flush_acc(a.stats, t, true);
-}
-
-template <int P, int B>
-__global__ __launch_bounds__(64) void k_tcf_remove_seq(TableDev T, OpArgs a) {
-    Tile<P> t;
-    for (uint64_t i = 0; i < a.n; i++) {
-        const bool ok = tcf_remove_key<P, B>(T, a.keys[i], t);
-        if (a.flags && t.leader()) a.flags[i] = ok ? 1 : 0;
-        fence_agent();
-    }
-    flush_acc(a.stats, t, true);
-}
-
-// Query: each tile handles I keys (tile_id + i * tiles_in_grid); the level-0
-// bucket loads of all I keys are issued before any is examined, so a wave
-// keeps 64/P * I random 64 B reads in flight.
-template <int P, int B, int I>
-__global__ __launch_bounds__(kBlock) void k_tcf_query(TableDev T, OpArgs a) {
-    Tile<P> t;
-    const uint64_t tile_id = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
-    const uint64_t tiles = (uint64_t(gridDim.x) * kBlock) / P;
-    const LevelDev &L = T.lv[0];
-    const uint32_t *slots = (const uint32_t *)L.slots;
-
-    uint64_t idx[I], key[I];
-    uint32_t fp[I], val[I];
-    bool live[I], pend[I], found[I], own[I];
-    Probe pr[I];
-#pragma unroll
-    for (int i = 0; i < I; i++) {
-        idx[i] = tile_id + (uint64_t)i * tiles;
-        live[i] = idx[i] < a.n;
-        key[i] = live[i] ? a.keys[idx[i]] : 0;
-        fp[i] = fp16_of(key[i]);
-        pr[i].begin(L, key[i], fp[i]);
-        pend[i] = live[i];
-        found[i] = false;
-        own[i] = false;
-        val[i] = 0;
-    }
-    if constexpr (B != 0) {
-        for (uint32_t j = 0; j < L.max_probes; j++) {
-            Bucket<P, B> bk[I];
-#pragma unroll
-            for (int i = 0; i < I; i++) {
-                if (pend[i]) {
-                    const uint64_t p = pr[i].at(j);
-                    if (p == kEnd) pend[i] = false;
-                    else bk[i].load(slots, fastmod(p, L.nb), t);
-                }
-            }
-            bool more = false;
-#pragma unroll
-            for (int i = 0; i < I; i++) {
-                if (pend[i] && bk[i].find(fp[i], t, val[i], own[i])) {
-                    found[i] = true;
-                    pend[i] = false;
-                }
-                more |= pend[i];
-            }
-            if (!t.any(more) && !__ballot(more)) break;
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < I; i++)
-            if (live[i]) found[i] = gen_level_query<P>(L, key[i], fp[i], t, val[i], own[i], 0);
-    }
-#pragma unroll
-    for (int i = 0; i < I; i++) {
-        if (live[i] && !found[i]) {
-            for (uint32_t l = 1; l < T.nlevels && !found[i]; l++)
-                found[i] = gen_level_query<P>(T.lv[l], key[i], fp[i], t, val[i], own[i], 1);
-        }
-        if (live[i]) {
-            const bool writer = found[i] ? own[i] : t.leader();
-            if (writer) {
-                if (a.flags) a.flags[idx[i]] = found[i] ? 1 : 0;
-                if (a.vals_out) ((uint16_t *)a.vals_out)[idx[i]] = found[i] ? (uint16_t)val[i] : 0;
-            }
-        }
-    }
-    flush_acc(a.stats, t, live[0]);
-}
-
-// ---------------------------------------------------------------------------
-// Dispatch
-// ---------------------------------------------------------------------------
-static inline unsigned grid_for(uint64_t tiles, int P) {
-    const uint64_t threads = tiles * (uint64_t)P;
-    return (unsigned)((threads + kBlock - 1) / kBlock);
-}
-
-template <int P, int B>
-static int launch_tcf_pb(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
-    if (a.n == 0) return PG_OK;
-    if (op == Op::Query) {
-        constexpr int I = (B != 0) ? 2 : 1;
-        const uint64_t tiles = (a.n + I - 1) / I;
-        hipLaunchKernelGGL((k_tcf_query<P, B, I>), dim3(grid_for(tiles, P)), dim3(kBlock), 0, s, T, a);
-    } else if (a.deterministic) {
-        if (op == Op::Insert)
-            hipLaunchKernelGGL((k_tcf_insert_seq<P, B>), dim3(1), dim3(P), 0, s, T, a);
-        else
-            hipLaunchKernelGGL((k_tcf_remove_seq<P, B>), dim3(1), dim3(P), 0, s, T, a);
-    } else {
-        if (op == Op::Insert)
-            hipLaunchKernelGGL((k_tcf_insert<P, B>), dim3(grid_for(a.n, P)), dim3(kBlock), 0, s, T, a);
-        else
-            hipLaunchKernelGGL((k_tcf_remove<P, B>), dim3(grid_for(a.n, P)), dim3(kBlock), 0, s, T, a);
-    }
-    return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
-}
-
-template <int P>
-static int launch_tcf_p(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
-    const LevelDev &L = T.lv[0];
-    if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT) {
-        switch (L.B) {
-            case 8: return launch_tcf_pb<P, 8>(T, op, a, s);
-            case 16: return launch_tcf_pb<P, 16>(T, op, a, s);
-            case 32: return launch_tcf_pb<P, 32>(T, op, a, s);
-            default: break;
-        }
-    }
-    return launch_tcf_pb<P, 0>(T, op, a, s);
-}
+int launch_tcf_p(const TableDev &T, Op op, const OpArgs &a, hipStream_t s);
 
 int launch_tcf(const TableDev &T, uint32_t tile, Op op, const OpArgs &a, hipStream_t s) {
     switch (tile) {
