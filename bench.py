@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log2-slots", type=int, default=28)
-    ap.add_argument("--tile", type=int, default=4)
+    ap.add_argument("--tile", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-roofline", action="store_true", help="skip the random-sector microbenchmark")
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -71,7 +71,7 @@ enum pg_scheme {
     PG_SCHEME_LINEAR_BUCKETS = 3
 };
 
-/* Probing scheme (probing_schemes/*.cuh).  Probe i of a key, before the
+/* Probing scheme (the probing_schemes directory).  Probe i of a key, before the
  * consumer's "% num_buckets":
  *   LINEAR   h + i                       (linear_probing.cuh:72-78)
  *   DOUBLE   h1 + i*h2, h2 seeded seed^2 (double_hashing.cuh:66-89)
@@ -156,6 +156,25 @@ int pg_bulk_insert(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint
 int pg_bulk_insert_with_delete(pg_handle h, const uint64_t *d_keys, const void *d_vals,
                                uint64_t n, uint8_t *d_status, uint64_t *d_fail_count,
                                void *stream);
+
+/* Bulk upsert: replaces insert_if_not_exists(tile, key, val, Val& ext,
+ * bool& found) (tables/bucketed_table.cuh:226-246;
+ * power_of_n_shortcut_buckets.cuh:421-544; linear_insert_buckets.cuh:249-282).
+ * A key already present is left in place: d_found = 1 and d_vals_out gets
+ * its stored value; otherwise it is inserted.  d_status as for insert (0 =
+ * the op returned false).  Filters only. */
+int pg_bulk_insert_if_not_exists(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n,
+                                 void *d_vals_out, uint8_t *d_found, uint8_t *d_status,
+                                 uint64_t *d_fail_count, void *stream);
+
+/* Toggle upsert: replaces insert_if_not_exists_delete
+ * (bucketed_table.cuh:248-268; power_of_n_shortcut_buckets.cuh:546-674;
+ * linear_insert_buckets.cuh:285-321).  A key already present is removed
+ * (d_found = 1, d_vals_out = its value); otherwise it is inserted, reusing
+ * tombstones in the shortcut level.  Filters only. */
+int pg_bulk_insert_if_not_exists_delete(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n,
+                                        void *d_vals_out, uint8_t *d_found, uint8_t *d_status,
+                                        uint64_t *d_fail_count, void *stream);
 
 /* Bulk query: replaces query(tile, key, Val&) (bucketed_table.cuh:332-349)
  * driven by speed_query_kernel (tests/src/test_mhm_tcf.cu:456-487).

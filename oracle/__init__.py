@@ -87,6 +87,11 @@ def lib():
         L.or_bulk_query.argtypes = [vp, vp, u64, vp, u32, vp]
         L.or_bulk_remove.restype = None
         L.or_bulk_remove.argtypes = [vp, vp, u64, vp]
+        L.or_insert_if_not_exists.restype = i32
+        L.or_insert_if_not_exists.argtypes = [vp, u64, u64, i32, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_uint64)]
+        L.or_bulk_insert_if_not_exists.restype = None
+        L.or_bulk_insert_if_not_exists.argtypes = [vp, vp, vp, u64, i32, vp, vp, vp]
         L.or_fill.restype = u64
         L.or_fill.argtypes = [vp]
         L.or_level_bytes.restype = u64
@@ -204,6 +209,26 @@ class Table:
         vals = np.zeros(len(keys), dtype=self.val_dtype)
         lib().or_bulk_query(self._h, _ptr(keys), len(keys), _ptr(vals), self.val_bytes, _ptr(found))
         return found, vals
+
+    def insert_if_not_exists(self, key, val=0, delete=False):
+        f = ctypes.c_int(0)
+        v = ctypes.c_uint64(0)
+        r = lib().or_insert_if_not_exists(self._h, int(key), int(val), int(delete), ctypes.byref(f),
+                                          ctypes.byref(v))
+        return r, bool(f.value), v.value
+
+    def bulk_insert_if_not_exists(self, keys: np.ndarray, vals: np.ndarray | None = None, delete=False):
+        """Returns (status, found, vals_out) per key."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        n = len(keys)
+        st = np.zeros(n, dtype=np.uint8)
+        found = np.zeros(n, dtype=np.uint8)
+        out = np.zeros(n, dtype=np.uint16)
+        if vals is not None:
+            vals = np.ascontiguousarray(vals, dtype=np.uint16)
+        lib().or_bulk_insert_if_not_exists(self._h, _ptr(keys), _ptr(vals), n, int(delete), _ptr(out),
+                                           _ptr(found), _ptr(st))
+        return st, found, out
 
     def bulk_remove(self, keys: np.ndarray) -> np.ndarray:
         keys = np.ascontiguousarray(keys, dtype=np.uint64)

@@ -764,6 +764,91 @@ int or_remove(or_table *t, uint64_t key)
     return 0;
 }
 
+/* ---- upserts ------------------------------------------------------ */
+
+/* power_of_n_insert_shortcut_bucket_scheme::insert_if_not_exists
+ * (insert_schemes/power_of_n_shortcut_buckets.cuh:421-544): query every probe
+ * bucket first (a match returns its value, found = true); otherwise the plain
+ * shortcut insert.  The _delete form (:546-674) removes the match instead of
+ * returning it, and inserts with tombstone reuse. */
+static int shortcut_upsert(or_table *t, or_level *L, uint64_t key, uint32_t word, int del,
+                           int *found, uint32_t *val)
+{
+    uint32_t fp = fp_of(t->rep, key);
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, fp);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        if (pb_query(t, L, b, fp, val)) {
+            *found = 1;
+            if (del) pb_remove(t, L, b, fp);
+            return 1;
+        }
+    }
+    return shortcut_insert(t, L, key, word, del);
+}
+
+/* linear_insert_bucket_scheme::insert_if_not_exists (linear_insert_buckets.cuh:
+ * 249-282): per probe bucket, a match returns it, else a free slot takes the
+ * key.  The _delete form (:285-321) removes the match; its insert does not
+ * reuse tombstones. */
+static int linear_upsert(or_table *t, or_level *L, uint64_t key, uint32_t word, int del,
+                         int *found, uint32_t *val)
+{
+    uint32_t fp = fp_of(t->rep, key);
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
+                                  L->seed, key, fp);
+         p != OR_END; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        if (pb_query(t, L, b, fp, val)) {
+            *found = 1;
+            if (del) pb_remove(t, L, b, fp);
+            return 1;
+        }
+        if (pb_insert(t, L, b, word)) return 1;
+    }
+    return 0;
+}
+
+/* Table-level upsert (bucketed_table.cuh:226-268): level 0, then the backing
+ * level when level 0 returns false.  Returns 0 on failure, else 1 + level;
+ * *found tells whether the key was already present (value in *val). */
+int or_insert_if_not_exists(or_table *t, uint64_t key, uint64_t val, int del, int *found, uint64_t *val_out)
+{
+    *found = 0;
+    *val_out = 0;
+    if (t->rep == OR_REP_KV64) return -1; /* bucket_insert has no upsert */
+    uint32_t vbits = kbits_of(t);
+    uint32_t word = or_pack(vbits, vbits, fp_of(t->rep, key), (uint32_t)val);
+    for (uint32_t l = 0; l < t->nlevels; l++) {
+        or_level *L = &t->lv[l];
+        uint32_t v = 0;
+        int r = (L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT)
+                    ? shortcut_upsert(t, L, key, word, del, found, &v)
+                    : linear_upsert(t, L, key, word, del, found, &v);
+        if (r) {
+            *val_out = *found ? v : 0;
+            return 1 + (int)l;
+        }
+    }
+    return 0;
+}
+
+void or_bulk_insert_if_not_exists(or_table *t, const uint64_t *keys, const uint16_t *vals, uint64_t n,
+                                  int del, uint16_t *vals_out, uint8_t *found, uint8_t *status)
+{
+    for (uint64_t i = 0; i < n; i++) {
+        int f = 0;
+        uint64_t v = 0;
+        int r = or_insert_if_not_exists(t, keys[i], vals ? vals[i] : 0, del, &f, &v);
+        if (status) status[i] = (uint8_t)(r < 0 ? 0 : r);
+        if (found) found[i] = (uint8_t)f;
+        if (vals_out) vals_out[i] = (uint16_t)v;
+    }
+}
+
 /* ---- bulk wrappers: the harness kernels of tests/src/test_mhm_tcf.cu:
  * 318-487 apply one op per key; sequential order here. ---- */
 void or_bulk_insert(or_table *t, const uint64_t *keys, const void *vals, uint32_t val_bytes,

@@ -34,6 +34,7 @@ EXPORTS = [
     "pg_bulk_insert_with_delete", "pg_bulk_query", "pg_bulk_remove", "pg_fill",
     "pg_bytes", "pg_num_slots", "pg_level_ptr", "pg_error_string", "pg_gen_keys",
     "pg_random_sector_bench", "pg_fastmod_host", "pg_set_stats", "pg_set_items",
+    "pg_bulk_insert_if_not_exists", "pg_bulk_insert_if_not_exists_delete",
 ]
 
 
@@ -91,6 +92,8 @@ def lib():
     L.pg_random_sector_bench.argtypes = [vp, u64, u64, u32, u64, vp, vp]
     L.pg_set_stats.argtypes = [vp, vp]
     L.pg_set_items.argtypes = [vp, u32, u32]
+    L.pg_bulk_insert_if_not_exists.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
+    L.pg_bulk_insert_if_not_exists_delete.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pg_fastmod_host.argtypes = [u64, u64]
     L.pg_fastmod_host.restype = u64
     for name in EXPORTS:
@@ -227,6 +230,12 @@ class Table:
 
     def bulk_insert_with_delete(self, keys, vals=None, status=None, fail_count=None, stream=None):
         self.bulk_insert(keys, vals, status, fail_count, stream, with_delete=True)
+
+    def bulk_insert_if_not_exists(self, keys, vals=None, vals_out=None, found=None, status=None,
+                                  fail_count=None, stream=None, delete=False):
+        fn = lib().pg_bulk_insert_if_not_exists_delete if delete else lib().pg_bulk_insert_if_not_exists
+        check(fn(self._h, _ptr(keys), _ptr(vals), keys.numel(), _ptr(vals_out), _ptr(found), _ptr(status),
+                 _ptr(fail_count), _stream(stream)), "pg_bulk_insert_if_not_exists")
 
     def bulk_query(self, keys, vals_out=None, found=None, stream=None):
         check(lib().pg_bulk_query(self._h, _ptr(keys), keys.numel(), _ptr(vals_out), _ptr(found),

@@ -59,6 +59,22 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_cpp_tests(verbose: bool = False) -> str:
+    """Compile the C++ front-end test program (tests/cpp) against the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "test_frontend.cpp")
+    out = os.path.join(root, "tests", "cpp", "test_frontend")
+    cmd = [hipcc(), "-O2", "-std=c++17", "-Wno-unused-command-line-argument", "-I", os.path.join(root, "include"),
+           src, "-o", out, "-L", HERE, "-lpoggers_amd", "-Wl,-rpath,$ORIGIN/../../poggers_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ front-end test build failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_tests(verbose=True)
     sys.exit(0)

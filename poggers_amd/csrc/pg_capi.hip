@@ -193,6 +193,37 @@ int pg_bulk_insert_with_delete(pg_handle h, const uint64_t *d_keys, const void *
     return do_insert(h, d_keys, d_vals, n, d_status, d_fail_count, stream, 1);
 }
 
+static int do_upsert(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n, void *d_vals_out,
+                     uint8_t *d_found, uint8_t *d_status, uint64_t *d_fail_count, void *stream, uint32_t del) {
+    if (!h || (n && !d_keys)) return PG_ERR_INVALID;
+    // bucket_insert / single_slot_insert have no insert_if_not_exists.
+    if (h->dev.rep == PG_REP_KV64) return PG_ERR_UNSUPPORTED;
+    pg::OpArgs a{};
+    a.keys = d_keys;
+    a.vals = d_vals;
+    a.n = n;
+    a.vals_out = d_vals_out;
+    a.found = d_found;
+    a.flags = d_status;
+    a.fail_count = d_fail_count;
+    a.with_delete = del;
+    a.deterministic = h->mode == PG_DETERMINISTIC;
+    a.stats = h->d_stats;
+    return run_op(h, pg::Op::Upsert, a, (hipStream_t)stream);
+}
+
+int pg_bulk_insert_if_not_exists(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n,
+                                 void *d_vals_out, uint8_t *d_found, uint8_t *d_status, uint64_t *d_fail_count,
+                                 void *stream) {
+    return do_upsert(h, d_keys, d_vals, n, d_vals_out, d_found, d_status, d_fail_count, stream, 0);
+}
+
+int pg_bulk_insert_if_not_exists_delete(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n,
+                                        void *d_vals_out, uint8_t *d_found, uint8_t *d_status,
+                                        uint64_t *d_fail_count, void *stream) {
+    return do_upsert(h, d_keys, d_vals, n, d_vals_out, d_found, d_status, d_fail_count, stream, 1);
+}
+
 int pg_bulk_query(pg_handle h, const uint64_t *d_keys, uint64_t n, void *d_vals_out, uint8_t *d_found,
                   void *stream) {
     if (!h || (n && !d_keys)) return PG_ERR_INVALID;

@@ -164,7 +164,8 @@ struct OpArgs {
     const void *vals;
     uint64_t n;
     void *vals_out;
-    uint8_t *flags;        // status (insert) or found (query/remove)
+    uint8_t *flags;        // status (insert, upsert) or found (query/remove)
+    uint8_t *found;        // upsert: key was already present
     uint64_t *fail_count;
     uint64_t *stats;       // optional accounting counters (pg_set_stats)
     uint32_t with_delete;
@@ -172,7 +173,7 @@ struct OpArgs {
     uint32_t items;        // keys per tile (pg_set_items), 0 = default
 };
 
-enum class Op { Insert, Query, Remove };
+enum class Op { Insert, Query, Remove, Upsert };
 
 int launch_tcf(const TableDev &T, uint32_t tile, Op op, const OpArgs &a, hipStream_t s);
 int launch_kv(const TableDev &T, uint32_t tile, Op op, const OpArgs &a, hipStream_t s);

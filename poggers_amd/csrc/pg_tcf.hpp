@@ -482,6 +482,85 @@ __device__ int tcf_insert_key(const TableDev &T, uint64_t key, uint32_t val, boo
     }
 }
 
+// ---------------------------------------------------------------------------
+// Upserts: insert_if_not_exists / insert_if_not_exists_delete.
+// Shortcut level (power_of_n_shortcut_buckets.cuh:421-674): look the key up in
+// every probe bucket; a match is returned (and, for _delete, tombstoned);
+// otherwise the shortcut insert (with tombstone reuse for _delete).
+// Linear level (linear_insert_buckets.cuh:249-321): per probe bucket, a match
+// is returned (and removed for _delete), else a free slot takes the key
+// (plain insert in both forms).
+// ---------------------------------------------------------------------------
+template <int P>
+__device__ bool gen_level_upsert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word, bool del,
+                                 const Tile<P> &t, int ai, uint32_t &val, bool &owner, bool &found) {
+    Probe pr;
+    pr.begin(L, key, fp);
+    uint32_t *slots = (uint32_t *)L.slots;
+    const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
+    for (uint32_t j = 0;; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) break;
+        uint32_t *base = slots + fastmod(p, L.nb) * L.B;
+        int f;
+        t.acc[ai]++;
+        if (gen_find<P>(base, L.B, fp, t, val, owner, f)) {
+            found = true;
+            if (del) gen_remove<P>(base, L.B, fp, t, f);
+            return true;
+        }
+        if (linear && gen_claim<P>(base, L.B, word, false, t)) return true;
+    }
+    if (linear) return false;
+    return gen_shortcut_insert<P>(L, key, fp, word, del, t, ai);
+}
+
+template <int P, int B>
+__device__ int tcf_upsert_key(const TableDev &T, uint64_t key, uint32_t v_in, bool del, const Tile<P> &t,
+                              uint32_t &val, bool &owner, bool &found) {
+    const uint32_t fp = fp16_of(key);
+    const uint32_t word = ((v_in & 0xFFFFu) << 16) | fp;
+    found = false;
+    bool ok;
+    if constexpr (B != 0) {
+        const LevelDev &L = T.lv[0];
+        uint32_t *slots = (uint32_t *)L.slots;
+        Probe pr;
+        pr.begin(L, key, fp);
+        Bucket<P, B> bk0;
+        const uint64_t p0 = pr.at(0);
+        const uint64_t b0 = (p0 == kEnd) ? 0 : fastmod(p0, L.nb);
+        ok = false;
+        if (p0 != kEnd) {
+            bk0.load(slots, b0, t);
+            if (bk0.find(fp, t, val, owner)) {
+                found = true;
+                if (del) bk0.remove(slots, b0, fp, t);
+                ok = true;
+            }
+            for (uint32_t j = 1; !ok; j++) {
+                const uint64_t p = pr.at(j);
+                if (p == kEnd) break;
+                const uint64_t b = fastmod(p, L.nb);
+                Bucket<P, B> bk;
+                bk.load(slots, b, t);
+                if (bk.find(fp, t, val, owner)) {
+                    found = true;
+                    if (del) bk.remove(slots, b, fp, t);
+                    ok = true;
+                }
+            }
+        }
+        if (!ok) ok = fast_shortcut_insert<P, B>(L, pr, word, del, t, bk0, b0);
+    } else {
+        ok = gen_level_upsert<P>(T.lv[0], key, fp, word, del, t, 0, val, owner, found);
+    }
+    if (ok) return 1;
+    for (uint32_t l = 1; l < T.nlevels; l++)
+        if (gen_level_upsert<P>(T.lv[l], key, fp, word, del, t, 1, val, owner, found)) return 1 + (int)l;
+    return 0;
+}
+
 template <int P, int B>
 __device__ bool tcf_remove_key(const TableDev &T, uint64_t key, const Tile<P> &t) {
     const uint32_t fp = fp16_of(key);
@@ -563,6 +642,44 @@ __global__ __launch_bounds__(kBlock) void k_tcf_remove(TableDev T, OpArgs a) {
         if (a.flags && t.leader()) a.flags[tid] = ok ? 1 : 0;
     }
     if constexpr (ACC) flush_acc(a.stats, t, tid < a.n);
+}
+
+template <int P, int B>
+__device__ inline void upsert_one(const TableDev &T, const OpArgs &a, uint64_t i, const Tile<P> &t,
+                                  uint64_t &fails) {
+    const uint32_t v_in = a.vals ? ((const uint16_t *)a.vals)[i] : 0u;
+    uint32_t val = 0;
+    bool owner = false, found = false;
+    const int st = tcf_upsert_key<P, B>(T, a.keys[i], v_in, a.with_delete != 0, t, val, owner, found);
+    if (t.leader()) {
+        if (a.flags) a.flags[i] = (uint8_t)st;
+        if (a.found) a.found[i] = found ? 1 : 0;
+        if (a.vals_out && !found) ((uint16_t *)a.vals_out)[i] = 0;
+    }
+    if (found && owner && a.vals_out) ((uint16_t *)a.vals_out)[i] = (uint16_t)val;
+    fails += (st == 0);
+}
+
+template <int P, int B, bool ACC>
+__global__ __launch_bounds__(kBlock) void k_tcf_upsert(TableDev T, OpArgs a) {
+    Tile<P> t;
+    const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
+    uint64_t fails = 0;
+    if (tid < a.n) upsert_one<P, B>(T, a, tid, t, fails);
+    wave_count(a.fail_count, tid < a.n && t.leader() && fails);
+    if constexpr (ACC) flush_acc(a.stats, t, tid < a.n);
+}
+
+template <int P, int B>
+__global__ __launch_bounds__(64) void k_tcf_upsert_seq(TableDev T, OpArgs a) {
+    Tile<P> t;
+    uint64_t fails = 0;
+    for (uint64_t i = 0; i < a.n; i++) {
+        upsert_one<P, B>(T, a, i, t, fails);
+        fence_agent();
+    }
+    if (a.fail_count && t.leader() && fails) atomicAdd((unsigned long long *)a.fail_count, fails);
+    flush_acc(a.stats, t, true);
 }
 
 // Deterministic mode: one tile applies the keys in array order.  The fence
@@ -689,7 +806,11 @@ int launch_tcf_pb(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
     const bool acc = a.stats != nullptr;
     if (op != Op::Query && a.deterministic) {
         if (op == Op::Insert) hipLaunchKernelGGL((k_tcf_insert_seq<P, B>), dim3(1), dim3(P), 0, s, T, a);
+        else if (op == Op::Upsert) hipLaunchKernelGGL((k_tcf_upsert_seq<P, B>), dim3(1), dim3(P), 0, s, T, a);
         else hipLaunchKernelGGL((k_tcf_remove_seq<P, B>), dim3(1), dim3(P), 0, s, T, a);
+    } else if (op == Op::Upsert) {
+        if (acc) hipLaunchKernelGGL((k_tcf_upsert<P, B, true>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
+        else hipLaunchKernelGGL((k_tcf_upsert<P, B, false>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
     } else if (op == Op::Remove) {
         if (acc) hipLaunchKernelGGL((k_tcf_remove<P, B, true>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
         else hipLaunchKernelGGL((k_tcf_remove<P, B, false>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);

@@ -1,0 +1,127 @@
+// Exercises the C++ templated front-end (include/poggers_amd/tables.hpp).
+//   test_frontend cpu   host-only checks (sizing errors, descriptors)
+//   test_frontend gpu   bulk ops through the C ABI on device 0
+// Exit status 0 on success; prints the failing check otherwise.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "poggers_amd/tables.hpp"
+
+using namespace poggers_amd;
+using namespace poggers_amd::data_structs;
+
+#define CHECK(c)                                                        \
+    do {                                                                \
+        if (!(c)) {                                                     \
+            std::fprintf(stderr, "FAILED %s:%d: %s\n", __FILE__, __LINE__, #c); \
+            return 1;                                                   \
+        }                                                               \
+    } while (0)
+
+static int cpu_checks() {
+    // A recursive table needs two sizes (bucketed_table.cuh:162-176).
+    size_in_num_slots<1> one(1 << 12);
+    bool threw = false;
+    try {
+        tcf::generate_on_device(&one, 42);
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    CHECK(threw);
+    size_in_num_slots<3> three(1 << 12, 64, 64);
+    threw = false;
+    try {
+        tcf::generate_on_device(&three, 42);
+    } catch (const std::runtime_error &) {
+        threw = true;
+    }
+    CHECK(threw);
+    size_in_num_slots<2> two(1000, 10);
+    CHECK(two.total() == 1010);
+    pg_desc d{};
+    tcf::describe(d, 0, &two);
+    CHECK(d.level[0].nslots == 1000 && d.level[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT);
+    CHECK(d.level[0].prober == PG_PROBE_XOR_P2 && d.level[0].tile == 4 && d.level[0].bucket_size == 16);
+    CHECK(d.level[1].nslots == 10 && d.level[1].scheme == PG_SCHEME_LINEAR_BUCKETS && d.level[1].max_probes == 4000);
+    static_assert(tcf::levels == 2 && hash_table_64::levels == 1, "levels");
+    std::printf("cpu checks ok\n");
+    return 0;
+}
+
+template <typename Table>
+static int roundtrip(const char *name, uint64_t nslots, uint64_t backing, uint64_t n) {
+    size_in_num_slots<2> sz2(nslots, backing);
+    size_in_num_slots<1> sz1(nslots);
+    Table *t = Table::levels == 2 ? Table::generate_on_device(&sz2, 42) : Table::generate_on_device(&sz1, 42);
+    uint64_t *keys = nullptr, *absent = nullptr, *fails = nullptr;
+    typename Table::val_type *vals = nullptr, *out = nullptr;
+    uint8_t *found = nullptr;
+    CHECK(hipMalloc(&keys, n * 8) == hipSuccess && hipMalloc(&absent, n * 8) == hipSuccess);
+    CHECK(hipMalloc(&vals, n * sizeof(*vals)) == hipSuccess && hipMalloc(&out, n * sizeof(*out)) == hipSuccess);
+    CHECK(hipMalloc(&found, n) == hipSuccess && hipMalloc(&fails, 8) == hipSuccess);
+    CHECK(hipMemset(fails, 0, 8) == hipSuccess);
+    CHECK(pg_gen_keys(7, 0, n, keys, nullptr) == PG_OK);
+    CHECK(pg_gen_keys(7, 1ull << 40, n, absent, nullptr) == PG_OK);
+    // values = low bits of the key
+    std::vector<uint64_t> hk(n);
+    CHECK(hipMemcpy(hk.data(), keys, n * 8, hipMemcpyDeviceToHost) == hipSuccess);
+    std::vector<typename Table::val_type> hv(n);
+    for (uint64_t i = 0; i < n; i++) hv[i] = (typename Table::val_type)(hk[i] * 3);
+    CHECK(hipMemcpy(vals, hv.data(), n * sizeof(*vals), hipMemcpyHostToDevice) == hipSuccess);
+    t->bulk_insert(keys, vals, n, nullptr, fails);
+    uint64_t nfail = 1;
+    CHECK(hipMemcpy(&nfail, fails, 8, hipMemcpyDeviceToHost) == hipSuccess);
+    CHECK(nfail == 0);
+    CHECK(t->get_fill() == n);
+    t->bulk_query(keys, n, out, found);
+    std::vector<uint8_t> hf(n);
+    std::vector<typename Table::val_type> ho(n);
+    CHECK(hipMemcpy(hf.data(), found, n, hipMemcpyDeviceToHost) == hipSuccess);
+    CHECK(hipMemcpy(ho.data(), out, n * sizeof(*out), hipMemcpyDeviceToHost) == hipSuccess);
+    uint64_t fn = 0, wrong = 0;
+    for (uint64_t i = 0; i < n; i++) {
+        fn += hf[i] == 0;
+        wrong += ho[i] != hv[i];
+    }
+    CHECK(fn == 0);
+    if (Table::rep == PG_REP_KV64) CHECK(wrong == 0);
+    t->bulk_remove(keys, n / 2, found);
+    CHECK(t->get_fill() == n - n / 2);
+    t->bulk_query(keys, n, out, found);
+    CHECK(hipMemcpy(hf.data(), found, n, hipMemcpyDeviceToHost) == hipSuccess);
+    uint64_t live = 0;
+    for (uint64_t i = n / 2; i < n; i++) live += hf[i];
+    CHECK(live == n - n / 2);
+    if (Table::rep == PG_REP_KV64) {
+        for (uint64_t i = 0; i < n / 2; i++) CHECK(hf[i] == 0);
+    }
+    CHECK(t->get_num_slots() >= nslots);
+    CHECK(t->host_bytes_in_use() == t->get_num_slots() * (Table::rep == PG_REP_KV64 ? 16 : 4));
+    for (void *p : {(void *)keys, (void *)absent, (void *)vals, (void *)out, (void *)found, (void *)fails})
+        CHECK(hipFree(p) == hipSuccess);
+    Table::free_on_device(t);
+    std::printf("%s ok (%llu keys)\n", name, (unsigned long long)n);
+    return 0;
+}
+
+static int gpu_checks() {
+    if (roundtrip<tcf>("tcf", 1 << 20, 1 << 14, 900000)) return 1;
+    if (roundtrip<fast_tcf>("fast_tcf", 1 << 20, 1 << 14, 900000)) return 1;
+    if (roundtrip<hash_table_64>("hash_table_64", 1 << 20, 0, 900000)) return 1;
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
+    try {
+        if (cpu_checks()) return 1;
+        if (gpu && gpu_checks()) return 1;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "FAILED: exception %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

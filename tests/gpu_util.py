@@ -54,6 +54,17 @@ class Ops:
         torch.cuda.synchronize()
         return found.cpu().numpy(), host(vals, np.uint64 if self.kv else np.uint16)
 
+    def upsert(self, keys, vals=None, delete=False):
+        n = len(keys)
+        k = dev_u64(keys)
+        v = None if vals is None else dev_u16(vals)
+        st = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        found = torch.zeros(n, dtype=torch.uint8, device="cuda")
+        out = torch.zeros(n, dtype=torch.int16, device="cuda")
+        self.t.bulk_insert_if_not_exists(k, v, out, found, st, delete=delete)
+        torch.cuda.synchronize()
+        return st.cpu().numpy(), found.cpu().numpy(), host(out, np.uint16)
+
     def remove(self, keys):
         n = len(keys)
         k = dev_u64(keys)

@@ -151,3 +151,46 @@ def test_full_table_reports_failures():
     assert g.get_fill() == int((st > 0).sum())
     found, _ = ops.query(keys[st > 0])
     assert found.all()
+
+
+@pytest.mark.parametrize("tile,B,prober,hasher", [CASES[0], CASES[2], CASES[6], CASES[3]])
+@pytest.mark.parametrize("delete", [False, True])
+def test_upsert_deterministic_bit_exact(tile, B, prober, hasher, delete):
+    """insert_if_not_exists[_delete] over a stream mixing present and new keys:
+    status, found, values and the slot layout equal the oracle's."""
+    desc = small_tcf(tile, B, prober, hasher)
+    o, g = pair(desc)
+    g.set_mode(pg.DETERMINISTIC)
+    ops = Ops(g)
+    base = oracle.gen_keys(12, 0, 2500)
+    vals = (base >> np.uint64(7)).astype(np.uint16)
+    assert (ops.insert(base, vals)[0] == o.bulk_insert(base, vals)).all()
+    mix = np.concatenate([base[::2], oracle.gen_keys(12, 1 << 40, 1700)])
+    np.random.default_rng(1).shuffle(mix)
+    mv = (mix & np.uint64(0xFFFF)).astype(np.uint16)
+    s_o, f_o, v_o = o.bulk_insert_if_not_exists(mix, mv, delete=delete)
+    s_g, f_g, v_g = ops.upsert(mix, mv, delete=delete)
+    assert (s_g == s_o).all() and (f_g == f_o).all() and (v_g == v_o).all()
+    for lvl in range(2):
+        assert (ops.level(lvl) == o.level_array(lvl)).all()
+
+
+def test_upsert_concurrent_semantics():
+    nslots = 1 << 20
+    desc = pg.tcf_desc(nslots, tile=4)
+    o, g = pair(desc)
+    ops = Ops(g)
+    base = oracle.gen_keys(13, 0, 400_000)
+    vals = (base >> np.uint64(9)).astype(np.uint16)
+    ops.insert(base, vals)
+    new = oracle.gen_keys(13, 1 << 40, 400_000)
+    mix = np.concatenate([base, new])
+    st, found, out = ops.upsert(mix, None)
+    assert (st > 0).all()
+    assert found[:len(base)].all()
+    # a present key reports a stored value of a slot with its fingerprint; with
+    # distinct fingerprints per bucket that is its own value
+    assert (out[:len(base)] == vals).mean() > 0.999
+    assert g.get_fill() == len(base) + int((found[len(base):] == 0).sum())
+    f2, _ = ops.query(mix)
+    assert f2.all()

@@ -242,3 +242,54 @@ def test_fp_rate_matches_closed_form(oracle_lib):
     # zero false negatives
     f2, _ = t.bulk_query(keys)
     assert f2.all()
+
+
+def test_upsert_semantics(oracle_lib):
+    """insert_if_not_exists (power_of_n_shortcut_buckets.cuh:421-544) and
+    insert_if_not_exists_delete (:546-674), hand-derived."""
+    o = oracle_lib
+    t = make(o)
+    pk = Picker(o, seed=31)
+    a, b = pk.key(0, 1), pk.key(0, 1)
+    # absent -> inserted at the lowest empty slot of its first choice
+    assert t.insert_if_not_exists(a, 5) == (1, False, 0)
+    assert bucket(t, 0, 0, B0)[0] == word(a, 5)
+    # present -> reported with its stored value, nothing written
+    before = t.level_array(0).copy()
+    assert t.insert_if_not_exists(a, 9) == (1, True, 5)
+    assert (t.level_array(0) == before).all()
+    t.insert(b, 6)
+    # _delete on a present key tombstones its slot and reports it
+    assert t.insert_if_not_exists(a, 0, delete=True) == (1, True, 5)
+    assert bucket(t, 0, 0, B0)[0] == TOMB and t.fill() == 1
+    # _delete on an absent key inserts, reusing the tombstone
+    c = pk.key(0, 1)
+    assert t.insert_if_not_exists(c, 7, delete=True) == (1, False, 0)
+    assert bucket(t, 0, 0, B0)[0] == word(c, 7)
+    # plain upsert does not reuse tombstones
+    t.remove(c)
+    d = pk.key(0, 1)
+    assert t.insert_if_not_exists(d, 8) == (1, False, 0)
+    assert bucket(t, 0, 0, B0)[0] == TOMB and bucket(t, 0, 0, B0)[2] == word(d, 8)
+
+
+def test_upsert_does_not_see_a_key_held_only_by_the_backing_level(oracle_lib):
+    """bucketed_table.cuh:226-268 runs the level-0 upsert first and only falls
+    back to the backing level when level 0 returns false.  A key that spilled
+    to the backing level is found there by the plain upsert (level 0 has no
+    EMPTY slot for it), but the _delete form reuses a level-0 tombstone and
+    inserts a second copy instead of removing the first."""
+    o = oracle_lib
+    t = make(o)
+    pk = Picker(o, seed=37)
+    full0 = [pk.key(2, 3) for _ in range(16)]
+    full1 = [pk.key(3, 2) for _ in range(16)]
+    for k in full0 + full1:
+        t.insert(k, 1)
+    spill = pk.key(2, 3, bb=0)
+    assert t.insert(spill, 4) == 2
+    t.remove(full0[0])                     # bucket 2 slot 0 becomes a tombstone
+    assert t.insert_if_not_exists(spill, 4) == (2, True, 4)
+    assert t.insert_if_not_exists(spill, 9, delete=True) == (1, False, 0)
+    assert bucket(t, 0, 2, B0)[0] == word(spill, 9)
+    assert bucket(t, 1, 0, B1)[0] == word(spill, 4)
