@@ -206,6 +206,41 @@ int pg_num_slots(pg_handle h, uint64_t *out);
  * exposes its slot arrays outside the device structs. */
 int pg_level_ptr(pg_handle h, uint32_t level, void **d_ptr, uint64_t *bytes);
 
+/* ---- batches: many independent filters, one persistent launch ----
+ * No reference counterpart: the reference launches one kernel per table
+ * (tests/src/test_mhm_tcf.cu:318-487).  A batch holds `ntables` filters of
+ * uneven sizes sharing the prototype's shape (levels, bucket size, tile,
+ * probing, hashing; FP16_V16 with a POWER_OF_N_SHORTCUT level 0).  Keys of all
+ * tables come in one flat device array with CSR device offsets: table t owns
+ * keys[offsets[t] .. offsets[t+1]), offsets[ntables] == total. */
+typedef struct pg_batch_s *pg_batch;
+
+/* nslots[t] = level-0 slots of table t (host array); backing_slots[t] the
+ * backing level's (host array, NULL = 1 % of level 0, at least one bucket). */
+int pg_batch_create(const pg_desc *proto, uint32_t ntables, const uint64_t *nslots,
+                    const uint64_t *backing_slots, uint64_t seed, pg_batch *out);
+int pg_batch_destroy(pg_batch b);
+/* PG_DETERMINISTIC makes pg_batch_insert apply each table's keys in order
+ * (one tile per table, tables in parallel): layouts equal the oracle's. */
+int pg_batch_set_mode(pg_batch b, uint32_t mode);
+/* Persistent launch shape: keys per work chunk and blocks (0 = fill the GPU). */
+int pg_batch_set_launch(pg_batch b, uint32_t chunk, uint32_t blocks);
+int pg_batch_clear(pg_batch b, void *stream);
+int pg_batch_insert(pg_batch b, const uint64_t *d_keys, const uint64_t *d_offsets, uint64_t total,
+                    const uint16_t *d_vals, uint8_t *d_status, uint64_t *d_fail_count, void *stream);
+int pg_batch_query(pg_batch b, const uint64_t *d_keys, const uint64_t *d_offsets, uint64_t total,
+                   uint16_t *d_vals_out, uint8_t *d_found, void *stream);
+/* Record [clear] + insert + query into a hipGraph held by the batch (kernels
+ * only, no memset nodes); pg_batch_replay launches it. */
+int pg_batch_capture(pg_batch b, const uint64_t *d_ins_keys, const uint64_t *d_ins_offsets, uint64_t ins_total,
+                     const uint64_t *d_q_keys, const uint64_t *d_q_offsets, uint64_t q_total, uint8_t *d_status,
+                     uint8_t *d_found, uint64_t *d_fail_count, uint32_t clear_first, void *stream);
+int pg_batch_replay(pg_batch b, void *stream);
+/* Filled slots of one table (synchronises; *out is a host pointer). */
+int pg_batch_fill(pg_batch b, uint32_t table, uint64_t *out, void *stream);
+int pg_batch_level_ptr(pg_batch b, uint32_t table, uint32_t level, void **d_ptr, uint64_t *bytes);
+int pg_batch_bytes(pg_batch b, uint64_t *out);
+
 /* Human-readable name of a status code. */
 const char *pg_error_string(int status);
 

@@ -35,6 +35,9 @@ EXPORTS = [
     "pg_bytes", "pg_num_slots", "pg_level_ptr", "pg_error_string", "pg_gen_keys",
     "pg_random_sector_bench", "pg_fastmod_host", "pg_set_stats", "pg_set_items",
     "pg_bulk_insert_if_not_exists", "pg_bulk_insert_if_not_exists_delete",
+    "pg_batch_create", "pg_batch_destroy", "pg_batch_set_mode", "pg_batch_set_launch", "pg_batch_clear",
+    "pg_batch_insert", "pg_batch_query", "pg_batch_capture", "pg_batch_replay", "pg_batch_fill",
+    "pg_batch_level_ptr", "pg_batch_bytes",
 ]
 
 
@@ -92,6 +95,18 @@ def lib():
     L.pg_random_sector_bench.argtypes = [vp, u64, u64, u32, u64, vp, vp]
     L.pg_set_stats.argtypes = [vp, vp]
     L.pg_set_items.argtypes = [vp, u32, u32]
+    L.pg_batch_create.argtypes = [ctypes.POINTER(Desc), u32, vp, vp, u64, ctypes.POINTER(vp)]
+    L.pg_batch_destroy.argtypes = [vp]
+    L.pg_batch_set_mode.argtypes = [vp, u32]
+    L.pg_batch_set_launch.argtypes = [vp, u32, u32]
+    L.pg_batch_clear.argtypes = [vp, vp]
+    L.pg_batch_insert.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp]
+    L.pg_batch_query.argtypes = [vp, vp, vp, u64, vp, vp, vp]
+    L.pg_batch_capture.argtypes = [vp, vp, vp, u64, vp, vp, u64, vp, vp, vp, u32, vp]
+    L.pg_batch_replay.argtypes = [vp, vp]
+    L.pg_batch_fill.argtypes = [vp, u32, ctypes.POINTER(u64), vp]
+    L.pg_batch_level_ptr.argtypes = [vp, u32, u32, ctypes.POINTER(vp), ctypes.POINTER(u64)]
+    L.pg_batch_bytes.argtypes = [vp, ctypes.POINTER(u64)]
     L.pg_bulk_insert_if_not_exists.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pg_bulk_insert_if_not_exists_delete.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pg_fastmod_host.argtypes = [u64, u64]
@@ -293,6 +308,83 @@ class Table:
         torch.cuda.synchronize()
         buf = ctypes.create_string_buffer(data, n)
         _hip_memcpy(ptr, ctypes.addressof(buf), n, 1)  # hipMemcpyHostToDevice
+
+
+class Batch:
+    """Many independent filters sharing one shape, driven by one persistent
+    kernel per bulk call (``pg_batch_*``).  ``proto`` is a TableDesc whose
+    level sizes are ignored; ``nslots`` lists each table's level-0 slots."""
+
+    def __init__(self, proto: TableDesc, nslots, backing_slots=None, seed: int = 42):
+        import numpy as np
+        self.proto = proto
+        self.ntables = len(nslots)
+        self._ns = np.ascontiguousarray(nslots, dtype=np.uint64)
+        self._bs = None if backing_slots is None else np.ascontiguousarray(backing_slots, dtype=np.uint64)
+        self._cdesc = proto.to_c()
+        h = ctypes.c_void_p()
+        check(lib().pg_batch_create(ctypes.byref(self._cdesc), self.ntables,
+                                    self._ns.ctypes.data_as(ctypes.c_void_p),
+                                    None if self._bs is None else self._bs.ctypes.data_as(ctypes.c_void_p),
+                                    seed, ctypes.byref(h)), "pg_batch_create")
+        self._h = h
+
+    def free(self):
+        if getattr(self, "_h", None):
+            check(lib().pg_batch_destroy(self._h), "pg_batch_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def set_mode(self, mode: int):
+        check(lib().pg_batch_set_mode(self._h, mode), "pg_batch_set_mode")
+
+    def set_launch(self, chunk: int, blocks: int = 0):
+        check(lib().pg_batch_set_launch(self._h, chunk, blocks), "pg_batch_set_launch")
+
+    def clear(self, stream=None):
+        check(lib().pg_batch_clear(self._h, _stream(stream)), "pg_batch_clear")
+
+    def insert(self, keys, offsets, vals=None, status=None, fail_count=None, stream=None):
+        check(lib().pg_batch_insert(self._h, _ptr(keys), _ptr(offsets), keys.numel(), _ptr(vals), _ptr(status),
+                                    _ptr(fail_count), _stream(stream)), "pg_batch_insert")
+
+    def query(self, keys, offsets, vals_out=None, found=None, stream=None):
+        check(lib().pg_batch_query(self._h, _ptr(keys), _ptr(offsets), keys.numel(), _ptr(vals_out), _ptr(found),
+                                   _stream(stream)), "pg_batch_query")
+
+    def capture(self, ins_keys, ins_offsets, q_keys, q_offsets, status=None, found=None, fail_count=None,
+                clear_first=True, stream=None):
+        check(lib().pg_batch_capture(self._h, _ptr(ins_keys), _ptr(ins_offsets), ins_keys.numel(), _ptr(q_keys),
+                                     _ptr(q_offsets), q_keys.numel(), _ptr(status), _ptr(found), _ptr(fail_count),
+                                     int(clear_first), _stream(stream)), "pg_batch_capture")
+
+    def replay(self, stream=None):
+        check(lib().pg_batch_replay(self._h, _stream(stream)), "pg_batch_replay")
+
+    def fill(self, table: int, stream=None) -> int:
+        out = ctypes.c_uint64()
+        check(lib().pg_batch_fill(self._h, table, ctypes.byref(out), _stream(stream)), "pg_batch_fill")
+        return out.value
+
+    def level_bytes(self, table: int, level: int) -> bytes:
+        import torch
+        p = ctypes.c_void_p()
+        n = ctypes.c_uint64()
+        check(lib().pg_batch_level_ptr(self._h, table, level, ctypes.byref(p), ctypes.byref(n)), "level_ptr")
+        torch.cuda.synchronize()
+        buf = ctypes.create_string_buffer(n.value)
+        _hip_memcpy(buf, p.value, n.value, 2)
+        return buf.raw
+
+    def bytes(self) -> int:
+        out = ctypes.c_uint64()
+        check(lib().pg_batch_bytes(self._h, ctypes.byref(out)), "pg_batch_bytes")
+        return out.value
 
 
 _hip = None
