@@ -241,6 +241,36 @@ int pg_batch_fill(pg_batch b, uint32_t table, uint64_t *out, void *stream);
 int pg_batch_level_ptr(pg_batch b, uint32_t table, uint32_t level, void **d_ptr, uint64_t *bytes);
 int pg_batch_bytes(pg_batch b, uint64_t *out);
 
+/* ---- serving: continuous batching over resident tables ----
+ * No reference counterpart (the reference has no host request path).
+ * Requests carry HOST buffers (pinned with hipHostMalloc for asynchronous
+ * copies).  pg_server_progress never blocks: it reports finished requests,
+ * admits queued requests in FIFO order while their staging bytes fit the
+ * device budget and a stream is free, and launches each admitted group as
+ * one batch (copies in, one persistent kernel over all its requests, copies
+ * out).  Up to `nstreams` batches overlap on the device.  Every served table
+ * must use the same tile width (at most 16).  Requests that touch the same
+ * keys of one table from different in-flight batches are not ordered with
+ * each other (as concurrent bulk calls on two streams would not be). */
+enum pg_op { PG_OP_INSERT = 0, PG_OP_QUERY = 1, PG_OP_REMOVE = 2 };
+typedef struct pg_server_s *pg_server;
+
+int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_bytes, uint32_t nstreams,
+                     pg_server *out);
+int pg_server_destroy(pg_server s);
+/* Queue a request; PG_ERR_NOMEM when it could never fit the budget.
+ * h_vals: insert values (uint64 KV / uint16 filter) or NULL; h_vals_out:
+ * query values or NULL; h_flags: per-key status / found or NULL. */
+int pg_server_submit(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,
+                     uint64_t n, void *h_vals_out, uint8_t *h_flags, uint64_t *ticket);
+/* Engine step: writes up to `cap` finished tickets to done[]. */
+int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndone);
+int pg_server_pending(pg_server s, uint64_t *queued, uint64_t *inflight);
+/* One request alone: copies in, the table's bulk kernel, copies out, wait.
+ * The per-request baseline; only when nothing is queued or in flight. */
+int pg_server_run_plain(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,
+                        uint64_t n, void *h_vals_out, uint8_t *h_flags);
+
 /* Human-readable name of a status code. */
 const char *pg_error_string(int status);
 

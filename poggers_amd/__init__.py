@@ -38,7 +38,10 @@ EXPORTS = [
     "pg_batch_create", "pg_batch_destroy", "pg_batch_set_mode", "pg_batch_set_launch", "pg_batch_clear",
     "pg_batch_insert", "pg_batch_query", "pg_batch_capture", "pg_batch_replay", "pg_batch_fill",
     "pg_batch_level_ptr", "pg_batch_bytes",
+    "pg_server_create", "pg_server_destroy", "pg_server_submit", "pg_server_progress", "pg_server_pending",
+    "pg_server_run_plain",
 ]
+OP_INSERT, OP_QUERY, OP_REMOVE = 0, 1, 2
 
 
 class PgError(RuntimeError):
@@ -107,6 +110,12 @@ def lib():
     L.pg_batch_fill.argtypes = [vp, u32, ctypes.POINTER(u64), vp]
     L.pg_batch_level_ptr.argtypes = [vp, u32, u32, ctypes.POINTER(vp), ctypes.POINTER(u64)]
     L.pg_batch_bytes.argtypes = [vp, ctypes.POINTER(u64)]
+    L.pg_server_create.argtypes = [vp, u32, u64, u32, ctypes.POINTER(vp)]
+    L.pg_server_destroy.argtypes = [vp]
+    L.pg_server_submit.argtypes = [vp, u32, u32, vp, vp, u64, vp, vp, ctypes.POINTER(u64)]
+    L.pg_server_progress.argtypes = [vp, vp, u32, ctypes.POINTER(u32)]
+    L.pg_server_pending.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+    L.pg_server_run_plain.argtypes = [vp, u32, u32, vp, vp, u64, vp, vp]
     L.pg_bulk_insert_if_not_exists.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pg_bulk_insert_if_not_exists_delete.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pg_fastmod_host.argtypes = [u64, u64]
@@ -385,6 +394,52 @@ class Batch:
         out = ctypes.c_uint64()
         check(lib().pg_batch_bytes(self._h, ctypes.byref(out)), "pg_batch_bytes")
         return out.value
+
+
+class Server:
+    """Continuous-batching request loop over resident tables (``pg_server_*``).
+    Request buffers are host tensors, ideally pinned (``pin_memory=True``)."""
+
+    def __init__(self, tables, budget_bytes: int = 1 << 30, nstreams: int = 4):
+        self.tables = list(tables)
+        arr = (ctypes.c_void_p * len(self.tables))(*[t._h.value for t in self.tables])
+        h = ctypes.c_void_p()
+        check(lib().pg_server_create(arr, len(self.tables), budget_bytes, nstreams, ctypes.byref(h)),
+              "pg_server_create")
+        self._h = h
+        self._done = (ctypes.c_uint64 * 65536)()
+
+    def free(self):
+        if getattr(self, "_h", None):
+            check(lib().pg_server_destroy(self._h), "pg_server_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def submit(self, table: int, op: int, keys, vals=None, vals_out=None, flags=None) -> int:
+        t = ctypes.c_uint64()
+        check(lib().pg_server_submit(self._h, table, op, _ptr(keys), _ptr(vals), keys.numel(), _ptr(vals_out),
+                                     _ptr(flags), ctypes.byref(t)), "pg_server_submit")
+        return t.value
+
+    def progress(self):
+        """One engine step; returns the tickets that finished."""
+        nd = ctypes.c_uint32()
+        check(lib().pg_server_progress(self._h, self._done, 65536, ctypes.byref(nd)), "pg_server_progress")
+        return [self._done[i] for i in range(nd.value)]
+
+    def pending(self):
+        q, f = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().pg_server_pending(self._h, ctypes.byref(q), ctypes.byref(f)), "pg_server_pending")
+        return q.value, f.value
+
+    def run_plain(self, table: int, op: int, keys, vals=None, vals_out=None, flags=None):
+        check(lib().pg_server_run_plain(self._h, table, op, _ptr(keys), _ptr(vals), keys.numel(), _ptr(vals_out),
+                                        _ptr(flags)), "pg_server_run_plain")
 
 
 _hip = None

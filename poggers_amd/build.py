@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpoggers_amd.so")
 ARCH = os.environ.get("PG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["pg_capi.hip", "pg_tcf.hip", "pg_kv.hip", "pg_util.hip", "pg_batch.hip"] + [
+SOURCES = ["pg_capi.hip", "pg_tcf.hip", "pg_kv.hip", "pg_util.hip", "pg_batch.hip", "pg_serve.hip"] + [
     f"pg_tcf_p{p}.hip" for p in (1, 2, 4, 8, 16, 32, 64)]
 
 # -packed-fp32-ops: no v_pk_*_f32 generation (none of this code is float, the

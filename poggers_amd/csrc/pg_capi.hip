@@ -51,6 +51,14 @@ uint32_t slot_bytes(uint32_t rep) { return rep == PG_REP_KV64 ? 16u : 4u; }
 
 }  // namespace
 
+namespace pg {
+const TableDev *table_dev(pg_handle h, uint32_t *tile) {
+    if (!h) return nullptr;
+    if (tile) *tile = h->desc.level[0].tile;
+    return &h->dev;
+}
+}  // namespace pg
+
 extern "C" {
 
 const char *pg_error_string(int status) {

@@ -135,6 +135,16 @@ struct Probe {
         h1 = hash64(L.hasher, key, L.seed);
         h2 = (prober == PG_PROBE_DOUBLE || prober == PG_PROBE_P2) ? hash64(L.hasher, key, L.seed2) : 0;
     }
+    // Probes worth visiting at a level of nb buckets.  Linear probes h + i
+    // revisit the same buckets with period nb unless h + i wraps past 2^64
+    // (which also covers the ~0 end marker), so when no wrap can occur
+    // within max_probes, probes beyond the first nb repeat buckets already
+    // seen and cannot change any result: the walk stops after nb probes.
+    __host__ __device__ inline uint32_t useful_probes(uint64_t nb) const {
+        if (prober == PG_PROBE_LINEAR && h1 <= kEnd - (uint64_t)max_probes && nb < max_probes)
+            return (uint32_t)nb;
+        return max_probes;
+    }
     __host__ __device__ inline uint64_t at(uint32_t i) const {
         if (i >= max_probes) return kEnd;
         if (i == 0) return prober == PG_PROBE_SPLIT64 ? (h1 & 0xffffffffULL) : h1;
@@ -174,6 +184,10 @@ struct OpArgs {
 };
 
 enum class Op { Insert, Query, Remove, Upsert };
+
+// The device view of a table and its tile width (pg_capi.hip); NULL for a
+// bad handle.
+const TableDev *table_dev(pg_handle h, uint32_t *tile);
 
 int launch_tcf(const TableDev &T, uint32_t tile, Op op, const OpArgs &a, hipStream_t s);
 int launch_kv(const TableDev &T, uint32_t tile, Op op, const OpArgs &a, hipStream_t s);

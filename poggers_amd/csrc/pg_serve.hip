@@ -1,0 +1,537 @@
+// pg_serve.hip -- continuous-batching request loop over resident tables.
+//
+// Requests (table, op, n keys in pinned host memory) are queued by
+// pg_server_submit.  pg_server_progress is the engine: it retires finished
+// batches (event query, no blocking), admits queued requests in FIFO order
+// while their staging bytes fit the device arena (the HBM budget) and a
+// stream slot is free, and launches each admitted group as one batch:
+// host-to-device copies of the keys, one persistent kernel that applies every
+// request of the batch to its own table (mixed tables, mixed ops), and
+// device-to-host copies of the results, all on one stream; up to `nstreams`
+// batches are in flight, so requests that arrive while others run overlap
+// with them on the device.  pg_server_run_plain processes one request with
+// its own copies and kernel launch and waits for it (the per-request
+// baseline).
+//
+// No reference counterpart: the reference has no host request path.
+#include <deque>
+#include <new>
+#include <vector>
+
+#include "pg_kv_ops.hpp"
+#include "pg_tcf.hpp"
+
+namespace pg {
+
+struct ReqDev {
+    const uint64_t *keys;
+    const void *vals;
+    uint8_t *flags;
+    void *vals_out;
+    uint64_t n;
+    uint32_t table;
+    uint32_t op;
+};
+
+struct ServeArgs {
+    const TableDev *tables;
+    const ReqDev *reqs;
+    const uint64_t *off;   // nreq + 1 key offsets
+    uint32_t nreq;
+    uint64_t total;
+    uint64_t *counter;
+    uint32_t chunk;
+};
+
+__device__ inline uint32_t find_req(const uint64_t *off, uint32_t n, uint64_t i) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(64) void k_serve_reset(uint64_t *counter) {
+    if (threadIdx.x == 0) counter[0] = 0;
+}
+
+template <int P>
+__device__ inline void serve_one(const TableDev &T, const ReqDev &r, uint64_t j, const Tile<P> &t) {
+    const uint64_t key = r.keys[j];
+    if (T.rep == PG_REP_KV64) {
+        if (r.op == PG_OP_INSERT) {
+            const uint64_t v = r.vals ? ((const uint64_t *)r.vals)[j] : 0ull;
+            const int st = kv_insert_key<P>(T, key, v, t);
+            if (r.flags && t.leader()) r.flags[j] = (uint8_t)st;
+        } else if (r.op == PG_OP_QUERY) {
+            uint64_t v = 0;
+            bool own = false;
+            const bool f = kv_query_key<P>(T, key, t, v, own);
+            if (f ? own : t.leader()) {
+                if (r.flags) r.flags[j] = f ? 1 : 0;
+                if (r.vals_out) ((uint64_t *)r.vals_out)[j] = f ? v : 0ull;
+            }
+        } else {
+            const bool f = kv_remove_key<P>(T, key, t);
+            if (r.flags && t.leader()) r.flags[j] = f ? 1 : 0;
+        }
+        return;
+    }
+    const bool fast16 = T.lv[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT && T.lv[0].B == 16;
+    if (r.op == PG_OP_INSERT) {
+        const uint32_t v = r.vals ? ((const uint16_t *)r.vals)[j] : 0u;
+        const int st = fast16 ? tcf_insert_key<P, 16>(T, key, v, false, t) : tcf_insert_key<P, 0>(T, key, v, false, t);
+        if (r.flags && t.leader()) r.flags[j] = (uint8_t)st;
+    } else if (r.op == PG_OP_QUERY) {
+        const uint32_t fp = fp16_of(key);
+        uint32_t v = 0;
+        bool own = false, f = false;
+        if (fast16) {
+            const LevelDev &L = T.lv[0];
+            Probe pr;
+            pr.begin(L, key, fp);
+            for (uint32_t q = 0; q < L.max_probes && !f; q++) {
+                const uint64_t p = pr.at(q);
+                if (p == kEnd) break;
+                Bucket<P, 16> bk;
+                bk.load((const uint32_t *)L.slots, fastmod(p, L.nb), t);
+                f = bk.find(fp, t, v, own);
+            }
+        } else {
+            f = gen_level_query<P>(T.lv[0], key, fp, t, v, own, 0);
+        }
+        for (uint32_t l = 1; l < T.nlevels && !f; l++) f = gen_level_query<P>(T.lv[l], key, fp, t, v, own, 1);
+        if (f ? own : t.leader()) {
+            if (r.flags) r.flags[j] = f ? 1 : 0;
+            if (r.vals_out) ((uint16_t *)r.vals_out)[j] = f ? (uint16_t)v : 0;
+        }
+    } else {
+        const bool f = fast16 ? tcf_remove_key<P, 16>(T, key, t) : tcf_remove_key<P, 0>(T, key, t);
+        if (r.flags && t.leader()) r.flags[j] = f ? 1 : 0;
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
+    Tile<P> t;
+    constexpr uint32_t kTilesPerWave = 64 / P;
+    const uint32_t tile_in_wave = (threadIdx.x & 63) / P;
+    while (true) {
+        uint64_t base = 0;
+        uint32_t r0 = 0;
+        if ((threadIdx.x & 63) == 0) {
+            base = atomicAdd((unsigned long long *)a.counter, (unsigned long long)a.chunk);
+            if (base < a.total) r0 = find_req(a.off, a.nreq, base);
+        }
+        base = __shfl(base, 0, 64);
+        r0 = __shfl(r0, 0, 64);
+        if (base >= a.total) break;
+        const uint64_t end = (base + a.chunk < a.total) ? base + a.chunk : a.total;
+        uint32_t r = r0;
+        for (uint64_t i = base + tile_in_wave; i < end; i += kTilesPerWave) {
+            while (a.off[r + 1] <= i) r++;
+            const ReqDev &rq = a.reqs[r];
+            serve_one<P>(a.tables[rq.table], rq, i - a.off[r], t);
+        }
+    }
+}
+
+static int launch_serve(uint32_t P, const ServeArgs &a, unsigned blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_serve_reset, dim3(1), dim3(64), 0, s, a.counter);
+    switch (P) {
+        case 1: hipLaunchKernelGGL((k_serve<1>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_serve<2>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_serve<4>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_serve<8>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 16: hipLaunchKernelGGL((k_serve<16>), dim3(blocks), dim3(256), 0, s, a); break;
+        default: return PG_ERR_INVALID;
+    }
+    return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
+}
+
+// FIFO ring over the staging arena: batches allocate at the head and are
+// released from the tail in launch order.
+struct Ring {
+    uint64_t cap = 0, head = 0;
+    std::deque<std::pair<uint64_t, uint64_t>> live;  // [begin, end) per batch, oldest first
+
+    uint64_t largest_free() const {
+        if (live.empty()) return cap;
+        const uint64_t t = live.front().first;
+        if (live.back().first < t) return t - head;  // wrapped: free is [head, t)
+        const uint64_t a = cap - head, b = t;
+        return a > b ? a : b;
+    }
+    bool alloc(uint64_t need, uint64_t &at) {
+        if (need > cap) return false;
+        if (live.empty()) {
+            at = 0;
+        } else {
+            const uint64_t t = live.front().first;
+            if (live.back().first < t) {
+                if (t - head < need) return false;
+                at = head;
+            } else if (cap - head >= need) {
+                at = head;
+            } else if (t >= need) {
+                at = 0;
+            } else {
+                return false;
+            }
+        }
+        head = at + need;
+        live.push_back({at, head});
+        return true;
+    }
+    void release_front() {
+        live.pop_front();
+        if (live.empty()) head = 0;
+    }
+};
+
+}  // namespace pg
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t kAlign = 256;
+constexpr uint32_t kMaxBatchReqs = 4096;
+
+inline uint64_t al(uint64_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+struct ServeReq {
+    uint64_t ticket;
+    uint32_t table, op;
+    const uint64_t *h_keys;
+    const void *h_vals;
+    uint64_t n;
+    void *h_vals_out;
+    uint8_t *h_flags;
+    uint32_t vbytes;   // value width of the table (8 KV, 2 filter)
+    uint64_t need;     // staging bytes
+};
+
+struct ServeBatch {
+    std::vector<uint64_t> tickets;
+    hipEvent_t done = nullptr;
+    uint32_t stream = 0;
+    bool finished = false;
+    bool reported = false;
+};
+
+uint64_t req_bytes(const ServeReq &r) {
+    uint64_t b = al(r.n * 8);                                     // keys
+    if (r.op == PG_OP_INSERT && r.h_vals) b += al(r.n * r.vbytes);
+    if (r.h_flags) b += al(r.n);
+    if (r.op == PG_OP_QUERY && r.h_vals_out) b += al(r.n * r.vbytes);
+    return b;
+}
+
+}  // namespace
+
+struct pg_server_s {
+    std::vector<pg_handle> handles;
+    std::vector<pg::TableDev> h_tables;
+    pg::TableDev *d_tables = nullptr;
+    uint32_t P = 0;
+    char *arena = nullptr;
+    pg::Ring ring;
+    std::vector<hipStream_t> streams;
+    std::vector<int> busy;
+    std::vector<pg::ReqDev *> h_desc;   // pinned, per stream
+    std::vector<uint64_t *> h_off;      // pinned, per stream
+    std::deque<ServeReq> queue;
+    std::deque<ServeBatch> inflight;
+    uint64_t next_ticket = 1;
+    unsigned blocks = 0;
+    uint32_t chunk = 2048;
+};
+
+extern "C" {
+
+int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_bytes, uint32_t nstreams,
+                     pg_server *out) {
+    if (!tables || !out || ntables == 0 || nstreams == 0 || nstreams > 16 || budget_bytes < (1u << 20))
+        return PG_ERR_INVALID;
+    *out = nullptr;
+    pg_server_s *s = new (std::nothrow) pg_server_s();
+    if (!s) return PG_ERR_NOMEM;
+    s->handles.assign(tables, tables + ntables);
+    s->h_tables.resize(ntables);
+    for (uint32_t i = 0; i < ntables; i++) {
+        uint32_t tile = 0;
+        const pg::TableDev *T = pg::table_dev(tables[i], &tile);
+        if (!T || (i > 0 && tile != s->P) || tile > 16) {
+            delete s;
+            return PG_ERR_UNSUPPORTED;  // one tile width (<= 16) for every served table
+        }
+        s->P = tile;
+        s->h_tables[i] = *T;
+    }
+    int rc = PG_OK;
+    if (hipMalloc((void **)&s->d_tables, sizeof(pg::TableDev) * ntables) != hipSuccess ||
+        hipMalloc((void **)&s->arena, budget_bytes) != hipSuccess)
+        rc = PG_ERR_NOMEM;
+    if (rc == PG_OK &&
+        hipMemcpy(s->d_tables, s->h_tables.data(), sizeof(pg::TableDev) * ntables, hipMemcpyHostToDevice) !=
+            hipSuccess)
+        rc = PG_ERR_HIP;
+    s->ring.cap = budget_bytes;
+    for (uint32_t i = 0; i < nstreams && rc == PG_OK; i++) {
+        hipStream_t st;
+        pg::ReqDev *d = nullptr;
+        uint64_t *o = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+            hipHostMalloc((void **)&d, sizeof(pg::ReqDev) * kMaxBatchReqs) != hipSuccess ||
+            hipHostMalloc((void **)&o, sizeof(uint64_t) * (kMaxBatchReqs + 1)) != hipSuccess) {
+            rc = PG_ERR_HIP;
+            break;
+        }
+        s->streams.push_back(st);
+        s->busy.push_back(0);
+        s->h_desc.push_back(d);
+        s->h_off.push_back(o);
+    }
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    s->blocks = (unsigned)cus * 4u;
+    if (rc != PG_OK) {
+        pg_server_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return PG_OK;
+}
+
+int pg_server_destroy(pg_server s) {
+    if (!s) return PG_ERR_INVALID;
+    for (hipStream_t st : s->streams) hipStreamSynchronize(st);
+    for (auto &b : s->inflight)
+        if (b.done) hipEventDestroy(b.done);
+    for (hipStream_t st : s->streams) hipStreamDestroy(st);
+    for (auto *d : s->h_desc) hipHostFree(d);
+    for (auto *o : s->h_off) hipHostFree(o);
+    if (s->arena) hipFree(s->arena);
+    if (s->d_tables) hipFree(s->d_tables);
+    delete s;
+    return PG_OK;
+}
+
+int pg_server_submit(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,
+                     uint64_t n, void *h_vals_out, uint8_t *h_flags, uint64_t *ticket) {
+    if (!s || table >= s->handles.size() || op > PG_OP_REMOVE || (n && !h_keys)) return PG_ERR_INVALID;
+    const pg::TableDev &T = s->h_tables[table];
+    if (op == PG_OP_REMOVE)
+        for (uint32_t l = 0; l < T.nlevels; l++)
+            if (T.lv[l].scheme == PG_SCHEME_SINGLE_SLOT) return PG_ERR_UNSUPPORTED;
+    ServeReq r{};
+    r.ticket = s->next_ticket++;
+    r.table = table;
+    r.op = op;
+    r.h_keys = h_keys;
+    r.h_vals = h_vals;
+    r.n = n;
+    r.h_vals_out = h_vals_out;
+    r.h_flags = h_flags;
+    r.vbytes = T.rep == PG_REP_KV64 ? 8u : 2u;
+    r.need = req_bytes(r);
+    // a request that can never fit the budget is refused at submission
+    const uint64_t overhead = al(sizeof(pg::ReqDev)) + al(16) + kAlign;
+    if (r.need + overhead > s->ring.cap) return PG_ERR_NOMEM;
+    s->queue.push_back(r);
+    if (ticket) *ticket = r.ticket;
+    return PG_OK;
+}
+
+// Launch queued requests that fit into one batch on a free stream.
+static int launch_batch(pg_server s, uint32_t si) {
+    const uint64_t free_bytes = s->ring.largest_free();
+    uint64_t need = 0;
+    size_t take = 0;
+    const uint64_t fixed = kAlign /* counter */;
+    while (take < s->queue.size() && take < kMaxBatchReqs) {
+        const uint64_t meta = al(sizeof(pg::ReqDev) * (take + 1)) + al(8 * (take + 2));
+        if (fixed + meta + need + s->queue[take].need > free_bytes) break;
+        need += s->queue[take].need;
+        take++;
+    }
+    if (take == 0) return PG_OK;
+    const uint64_t meta = al(sizeof(pg::ReqDev) * take) + al(8 * (take + 1));
+    uint64_t at = 0;
+    if (!s->ring.alloc(fixed + meta + need, at)) return PG_OK;
+    hipStream_t st = s->streams[si];
+    char *base = s->arena + at;
+    uint64_t *d_counter = (uint64_t *)base;
+    pg::ReqDev *d_desc = (pg::ReqDev *)(base + fixed);
+    uint64_t *d_off = (uint64_t *)(base + fixed + al(sizeof(pg::ReqDev) * take));
+    char *p = base + fixed + meta;
+    pg::ReqDev *hd = s->h_desc[si];
+    uint64_t *ho = s->h_off[si];
+    ServeBatch b;
+    b.stream = si;
+    uint64_t total = 0;
+    for (size_t i = 0; i < take; i++) {
+        const ServeReq &r = s->queue[i];
+        pg::ReqDev &d = hd[i];
+        d = pg::ReqDev{};
+        d.n = r.n;
+        d.table = r.table;
+        d.op = r.op;
+        d.keys = (const uint64_t *)p;
+        if (r.n && hipMemcpyAsync(p, r.h_keys, r.n * 8, hipMemcpyHostToDevice, st) != hipSuccess) return PG_ERR_HIP;
+        p += al(r.n * 8);
+        if (r.op == PG_OP_INSERT && r.h_vals) {
+            d.vals = p;
+            if (r.n && hipMemcpyAsync(p, r.h_vals, r.n * r.vbytes, hipMemcpyHostToDevice, st) != hipSuccess)
+                return PG_ERR_HIP;
+            p += al(r.n * r.vbytes);
+        }
+        if (r.h_flags) {
+            d.flags = (uint8_t *)p;
+            p += al(r.n);
+        }
+        if (r.op == PG_OP_QUERY && r.h_vals_out) {
+            d.vals_out = p;
+            p += al(r.n * r.vbytes);
+        }
+        ho[i] = total;
+        total += r.n;
+        b.tickets.push_back(r.ticket);
+    }
+    ho[take] = total;
+    if (hipMemcpyAsync(d_desc, hd, sizeof(pg::ReqDev) * take, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(d_off, ho, 8 * (take + 1), hipMemcpyHostToDevice, st) != hipSuccess)
+        return PG_ERR_HIP;
+    if (total) {
+        pg::ServeArgs a{};
+        a.tables = s->d_tables;
+        a.reqs = d_desc;
+        a.off = d_off;
+        a.nreq = (uint32_t)take;
+        a.total = total;
+        a.counter = d_counter;
+        a.chunk = s->chunk;
+        const uint64_t want = (total + s->chunk - 1) / s->chunk;
+        const unsigned blocks = (unsigned)(want < s->blocks * 4ull ? (want + 3) / 4 : s->blocks);
+        const int rc = pg::launch_serve(s->P, a, blocks ? blocks : 1, st);
+        if (rc != PG_OK) return rc;
+    }
+    for (size_t i = 0; i < take; i++) {
+        const ServeReq &r = s->queue[i];
+        const pg::ReqDev &d = hd[i];
+        if (r.h_flags && r.n &&
+            hipMemcpyAsync(r.h_flags, d.flags, r.n, hipMemcpyDeviceToHost, st) != hipSuccess)
+            return PG_ERR_HIP;
+        if (d.vals_out && r.n &&
+            hipMemcpyAsync(r.h_vals_out, d.vals_out, r.n * r.vbytes, hipMemcpyDeviceToHost, st) != hipSuccess)
+            return PG_ERR_HIP;
+    }
+    if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) return PG_ERR_HIP;
+    if (hipEventRecord(b.done, st) != hipSuccess) return PG_ERR_HIP;
+    s->queue.erase(s->queue.begin(), s->queue.begin() + (long)take);
+    s->busy[si] = 1;
+    s->inflight.push_back(std::move(b));
+    return PG_OK;
+}
+
+int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndone) {
+    if (!s) return PG_ERR_INVALID;
+    uint32_t nd = 0;
+    // retire: report a batch as soon as it finishes (any order); free its
+    // staging memory in launch order
+    for (auto &b : s->inflight) {
+        if (!b.finished) {
+            const hipError_t q = hipEventQuery(b.done);
+            if (q == hipSuccess) {
+                b.finished = true;
+                s->busy[b.stream] = 0;
+            } else if (q != hipErrorNotReady) {
+                return PG_ERR_HIP;
+            }
+        }
+        if (b.finished && !b.reported && (nd + b.tickets.size() <= cap || nd == 0)) {
+            for (uint64_t tk : b.tickets)
+                if (done && nd < cap) done[nd++] = tk;
+            b.reported = true;
+        }
+    }
+    while (!s->inflight.empty() && s->inflight.front().finished && s->inflight.front().reported) {
+        hipEventDestroy(s->inflight.front().done);
+        s->ring.release_front();
+        s->inflight.pop_front();
+    }
+    // admit and launch on every idle stream
+    for (uint32_t si = 0; si < s->streams.size() && !s->queue.empty(); si++) {
+        if (s->busy[si]) continue;
+        const int rc = launch_batch(s, si);
+        if (rc != PG_OK) return rc;
+    }
+    if (ndone) *ndone = nd;
+    return PG_OK;
+}
+
+int pg_server_pending(pg_server s, uint64_t *queued, uint64_t *inflight) {
+    if (!s) return PG_ERR_INVALID;
+    if (queued) *queued = s->queue.size();
+    if (inflight) {
+        uint64_t n = 0;
+        for (auto &b : s->inflight) n += b.tickets.size();
+        *inflight = n;
+    }
+    return PG_OK;
+}
+
+int pg_server_run_plain(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,
+                        uint64_t n, void *h_vals_out, uint8_t *h_flags) {
+    if (!s || table >= s->handles.size() || op > PG_OP_REMOVE || (n && !h_keys)) return PG_ERR_INVALID;
+    if (!s->inflight.empty() || !s->queue.empty()) return PG_ERR_INVALID;  // not while batches run
+    const pg::TableDev &T = s->h_tables[table];
+    const uint32_t vb = T.rep == PG_REP_KV64 ? 8u : 2u;
+    ServeReq r{};
+    r.op = op;
+    r.n = n;
+    r.h_vals = h_vals;
+    r.h_flags = h_flags;
+    r.h_vals_out = h_vals_out;
+    r.vbytes = vb;
+    uint64_t at = 0;
+    if (!s->ring.alloc(req_bytes(r), at)) return PG_ERR_NOMEM;
+    hipStream_t st = s->streams[0];
+    char *p = s->arena + at;
+    uint64_t *d_keys = (uint64_t *)p;
+    p += al(n * 8);
+    void *d_vals = nullptr, *d_out = nullptr;
+    uint8_t *d_flags = nullptr;
+    if (op == PG_OP_INSERT && h_vals) {
+        d_vals = p;
+        p += al(n * vb);
+    }
+    if (h_flags) {
+        d_flags = (uint8_t *)p;
+        p += al(n);
+    }
+    if (op == PG_OP_QUERY && h_vals_out) d_out = p;
+    int rc = PG_OK;
+    if (n && hipMemcpyAsync(d_keys, h_keys, n * 8, hipMemcpyHostToDevice, st) != hipSuccess) rc = PG_ERR_HIP;
+    if (rc == PG_OK && d_vals && n && hipMemcpyAsync(d_vals, h_vals, n * vb, hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = PG_ERR_HIP;
+    if (rc == PG_OK) {
+        pg_handle h = s->handles[table];
+        if (op == PG_OP_INSERT) rc = pg_bulk_insert(h, d_keys, d_vals, n, d_flags, nullptr, st);
+        else if (op == PG_OP_QUERY) rc = pg_bulk_query(h, d_keys, n, d_out, d_flags, st);
+        else rc = pg_bulk_remove(h, d_keys, n, d_flags, st);
+    }
+    if (rc == PG_OK && d_flags && n && hipMemcpyAsync(h_flags, d_flags, n, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = PG_ERR_HIP;
+    if (rc == PG_OK && d_out && n && hipMemcpyAsync(h_vals_out, d_out, n * vb, hipMemcpyDeviceToHost, st) != hipSuccess)
+        rc = PG_ERR_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
+    s->ring.release_front();
+    return rc;
+}
+
+}  // extern "C"

@@ -301,6 +301,7 @@ __device__ bool gen_linear_insert(const LevelDev &L, uint64_t key, uint32_t fp, 
                                   bool wd, const Tile<P> &t, int ai) {
     Probe pr;
     pr.begin(L, key, fp);
+    pr.max_probes = pr.useful_probes(L.nb.d);
     uint32_t *slots = (uint32_t *)L.slots;
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
@@ -328,6 +329,7 @@ __device__ bool gen_level_query(const LevelDev &L, uint64_t key, uint32_t fp, co
     pr.begin(L, key, fp);
     const uint32_t *slots = (const uint32_t *)L.slots;
     const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
+    if (linear) pr.max_probes = pr.useful_probes(L.nb.d);
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
         if (p == kEnd) return false;
@@ -347,6 +349,7 @@ __device__ bool gen_level_remove(const LevelDev &L, uint64_t key, uint32_t fp, c
     pr.begin(L, key, fp);
     uint32_t *slots = (uint32_t *)L.slots;
     const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
+    if (linear) pr.max_probes = pr.useful_probes(L.nb.d);
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
         if (p == kEnd) return false;
@@ -498,6 +501,7 @@ __device__ bool gen_level_upsert(const LevelDev &L, uint64_t key, uint32_t fp, u
     pr.begin(L, key, fp);
     uint32_t *slots = (uint32_t *)L.slots;
     const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
+    if (linear) pr.max_probes = pr.useful_probes(L.nb.d);
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
         if (p == kEnd) break;

@@ -1,0 +1,42 @@
+"""Synthetic workloads on the device (key streams are splitmix64 counters, so
+every table's keys are distinct and reproducible from the seed)."""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import gen_keys
+
+
+def log_uniform_sizes(ntables: int, lo: int, hi: int, seed: int) -> np.ndarray:
+    """Slot counts log-uniform in [2^lo, 2^hi]."""
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(2.0 ** lo), np.log(2.0 ** hi), ntables)).astype(np.uint64)
+
+
+def batched_workload(sizes: np.ndarray, load: float, seed: int, device="cuda"):
+    """Per table an insert batch filling it to `load` and a query batch of the
+    same size, half present (the first half of its inserts) and half absent.
+    Returns device tensors (ins_keys, ins_off, q_keys, q_off) and the host
+    offsets; keys are int64 tensors holding uint64 bit patterns."""
+    n = (np.floor(load * sizes.astype(np.float64))).astype(np.int64)
+    off = np.zeros(len(sizes) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(n)
+    total = int(off[-1])
+    ins = torch.empty(total, dtype=torch.int64, device=device)
+    gen_keys(seed, 0, total, ins)
+    npres = n // 2
+    nabs = n - npres
+    absent = torch.empty(int(nabs.sum()), dtype=torch.int64, device=device)
+    gen_keys(seed, 1 << 40, absent.numel(), absent)
+    q = torch.empty(total, dtype=torch.int64, device=device)
+    a = 0
+    for t in range(len(sizes)):
+        s, p, m = int(off[t]), int(npres[t]), int(nabs[t])
+        if p:
+            q[s:s + p].copy_(ins[s:s + p])
+        if m:
+            q[s + p:s + p + m].copy_(absent[a:a + m])
+        a += m
+    d_off = torch.from_numpy(off.astype(np.int64)).to(device)
+    return ins, d_off, q, d_off.clone(), off, npres

@@ -111,3 +111,29 @@ def test_single_slot_scheme_and_unsupported_ops():
         ops.remove(keys[:10])
     with pytest.raises(pg.PgError):
         ops.insert(keys[:10], keys[:10], with_delete=True)
+
+
+@pytest.mark.parametrize("tile,B", [(1, 1), (4, 1), (4, 4)])
+def test_more_probes_than_buckets_bit_exact(tile, B):
+    """Linear probing with Max_Probes far above the bucket count (the
+    reference's backing tables use 4000): the GPU stops after one sweep of
+    the buckets, the oracle walks all probes; results and layouts agree,
+    including the failures of an overfilled table."""
+    m = 64
+    desc = pg.kv_desc(m, bucket_size=B, tile=tile, prober=pg.PROBE_LINEAR, max_probes=4000)
+    o, g = pair(desc)
+    g.set_mode(pg.DETERMINISTIC)
+    ops = Ops(g)
+    keys = oracle.gen_keys(31, 0, 80)
+    st_o = o.bulk_insert(keys, keys)
+    st_g, _ = ops.insert(keys, keys)
+    assert (st_g == st_o).all() and (st_o == 0).sum() == 16
+    assert (ops.level(0) == o.level_array(0)).all()
+    q = np.concatenate([keys, oracle.gen_keys(31, 1 << 40, 64)])
+    f_o, v_o = o.bulk_query(q)
+    f_g, v_g = ops.query(q)
+    assert (f_g == f_o).all() and (v_g == v_o).all()
+    assert (ops.remove(keys[::5]) == o.bulk_remove(keys[::5])).all()
+    f_o, v_o = o.bulk_query(q)
+    f_g, v_g = ops.query(q)
+    assert (f_g == f_o).all() and (v_g == v_o).all()

@@ -1,0 +1,157 @@
+"""GPU tests of the continuous-batching server (pg_server_*): every request's
+result is the same whether it runs alone (per-request plain launch) or in
+the served mix; admission respects the HBM budget; requests submitted while
+others are in flight complete."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import poggers_amd as pg
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.uint64(0x9E3779B97F4A7C15)
+
+
+def pinned_u64(a):
+    t = torch.empty(len(a), dtype=torch.int64, pin_memory=True)
+    t.numpy()[:] = np.ascontiguousarray(a, dtype=np.uint64).view(np.int64)
+    return t
+
+
+def pinned_u16(a):
+    t = torch.empty(len(a), dtype=torch.int16, pin_memory=True)
+    t.numpy()[:] = np.ascontiguousarray(a, dtype=np.uint16).view(np.int16)
+    return t
+
+
+def build_tables():
+    tabs = [pg.Table(pg.tcf_desc(1 << 19, tile=4)), pg.Table(pg.tcf_desc(1 << 18, tile=4)),
+            pg.Table(pg.kv_desc(1 << 19, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20)),
+            pg.Table(pg.kv_desc(1 << 19, bucket_size=1, tile=4, prober=pg.PROBE_LINEAR))]
+    return tabs
+
+
+def populate(tabs, base):
+    for i, t in enumerate(tabs):
+        k = base[i]
+        dk = torch.from_numpy(k.view(np.int64)).cuda()
+        if t.is_kv:
+            t.bulk_insert(dk, torch.from_numpy((k ^ GOLD).view(np.int64)).cuda())
+        else:
+            t.bulk_insert(dk, torch.from_numpy((k & np.uint64(0xFFFF)).astype(np.uint16).view(np.int16)).cuda())
+    torch.cuda.synchronize()
+
+
+def make_requests(rng, base, ntables, nreq):
+    """Mixed requests whose answers do not depend on their order: queries of
+    pre-populated keys (first half of each base set), deletes of keys from
+    the second quarter, inserts of fresh keys."""
+    reqs = []
+    fresh_start = 1 << 44
+    for r in range(nreq):
+        t = int(rng.integers(ntables))
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(20000))))
+        op = rng.choice([pg.OP_QUERY, pg.OP_QUERY, pg.OP_INSERT, pg.OP_REMOVE])
+        b = base[t]
+        if op == pg.OP_QUERY:
+            keys = b[rng.integers(0, len(b) // 2, n)]
+        elif op == pg.OP_REMOVE:
+            lo = len(b) // 2 + (r * 997) % (len(b) // 4)
+            keys = b[lo:lo + min(n, len(b) // 2 - (lo - len(b) // 2))]
+            keys = np.unique(keys)
+        else:
+            keys = oracle.gen_keys(1000 + r, fresh_start, n)
+        reqs.append((t, int(op), np.ascontiguousarray(keys, dtype=np.uint64)))
+    return reqs
+
+
+def buffers(tab, op, keys):
+    n = len(keys)
+    k = pinned_u64(keys)
+    vals = out = None
+    if op == pg.OP_INSERT:
+        vals = pinned_u64(keys ^ GOLD) if tab.is_kv else pinned_u16((keys & np.uint64(0xFFFF)).astype(np.uint16))
+    if op == pg.OP_QUERY:
+        out = torch.zeros(n, dtype=torch.int64 if tab.is_kv else torch.int16, pin_memory=True)
+    flags = torch.zeros(n, dtype=torch.uint8, pin_memory=True)
+    return k, vals, out, flags
+
+
+def test_results_alone_equal_results_in_mix():
+    rng = np.random.default_rng(4)
+    tabs_mix, tabs_alone = build_tables(), build_tables()
+    base = [oracle.gen_keys(50 + i, 0, int(0.5 * t.get_num_slots())) for i, t in enumerate(tabs_mix)]
+    populate(tabs_mix, base)
+    populate(tabs_alone, base)
+    reqs = make_requests(rng, base, len(tabs_mix), 120)
+    # deletes of one key by two requests would race: keep each key in one delete
+    seen = set()
+    clean = []
+    for t, op, k in reqs:
+        if op == pg.OP_REMOVE:
+            k = np.array([x for x in k if (t, int(x)) not in seen], dtype=np.uint64)
+            seen.update((t, int(x)) for x in k)
+            if len(k) == 0:
+                continue
+        clean.append((t, op, k))
+    reqs = clean
+    srv_mix = pg.Server(tabs_mix, budget_bytes=64 << 20, nstreams=4)
+    bufs_mix = [buffers(tabs_mix[t], op, k) for t, op, k in reqs]
+    tickets = {}
+    for i, ((t, op, k), (kk, vv, oo, ff)) in enumerate(zip(reqs, bufs_mix)):
+        tickets[srv_mix.submit(t, op, kk, vv, oo, ff)] = i
+        if i % 7 == 0:
+            srv_mix.progress()
+    done = set()
+    for _ in range(1_000_000):
+        done.update(srv_mix.progress())
+        if len(done) == len(reqs):
+            break
+    assert len(done) == len(reqs)
+    srv_alone = pg.Server(tabs_alone, budget_bytes=64 << 20, nstreams=1)
+    bufs_alone = [buffers(tabs_alone[t], op, k) for t, op, k in reqs]
+    for (t, op, k), (kk, vv, oo, ff) in zip(reqs, bufs_alone):
+        srv_alone.run_plain(t, op, kk, vv, oo, ff)
+    for i, ((t, op, k), bm, ba) in enumerate(zip(reqs, bufs_mix, bufs_alone)):
+        fm, fa = bm[3].numpy(), ba[3].numpy()
+        if op == pg.OP_QUERY:
+            assert fm.all() and fa.all(), i          # pre-populated keys: no false negatives
+            om, oa = bm[2].numpy(), ba[2].numpy()
+            if tabs_mix[t].is_kv:
+                assert (om == oa).all() and (om.view(np.uint64) == (k ^ GOLD)).all(), i
+            else:
+                assert (om == oa).mean() > 0.995, i   # fingerprint collisions may pick another slot
+        else:
+            assert (fm == fa).all() and fm.all(), (i, op)
+    srv_mix.free()
+    srv_alone.free()
+
+
+def test_admission_respects_budget_and_drains_queue():
+    tabs = build_tables()[:1]
+    srv = pg.Server(tabs, budget_bytes=1 << 20, nstreams=2)
+    big = oracle.gen_keys(1, 0, 1 << 17)       # 1 MiB of keys alone: can never fit
+    kk, _, _, ff = buffers(tabs[0], pg.OP_INSERT, big)
+    with pytest.raises(pg.PgError):
+        srv.submit(0, pg.OP_INSERT, kk, None, None, ff)
+    reqs = [oracle.gen_keys(2, i * 20000, 20000) for i in range(40)]  # ~16 MB in total
+    bufs = [buffers(tabs[0], pg.OP_INSERT, k) for k in reqs]
+    for (kk, vv, oo, ff) in bufs:
+        srv.submit(0, pg.OP_INSERT, kk, vv, oo, ff)
+    q, f = srv.pending()
+    assert q == len(reqs)
+    done = []
+    max_inflight = 0
+    for _ in range(1_000_000):
+        done += srv.progress()
+        max_inflight = max(max_inflight, srv.pending()[1])
+        if len(done) == len(reqs):
+            break
+    assert sorted(done) == sorted(set(done)) and len(done) == len(reqs)
+    assert max_inflight * 20000 * 11 <= (1 << 20)  # keys + vals + flags of in-flight requests fit
+    for (kk, vv, oo, ff) in bufs:
+        assert ff.numpy().all()
+    assert tabs[0].get_fill() == 40 * 20000
+    srv.free()
