@@ -266,6 +266,9 @@ int pg_server_submit(pg_server s, uint32_t table, uint32_t op, const uint64_t *h
 /* Engine step: writes up to `cap` finished tickets to done[]. */
 int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndone);
 int pg_server_pending(pg_server s, uint64_t *queued, uint64_t *inflight);
+/* Batch formation: at most max_batch_keys keys per batch (a larger single
+ * request still forms its own batch) and `chunk` keys per work item. */
+int pg_server_set_batch(pg_server s, uint64_t max_batch_keys, uint32_t chunk);
 /* One request alone: copies in, the table's bulk kernel, copies out, wait.
  * The per-request baseline; only when nothing is queued or in flight. */
 int pg_server_run_plain(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,

@@ -39,7 +39,7 @@ EXPORTS = [
     "pg_batch_insert", "pg_batch_query", "pg_batch_capture", "pg_batch_replay", "pg_batch_fill",
     "pg_batch_level_ptr", "pg_batch_bytes",
     "pg_server_create", "pg_server_destroy", "pg_server_submit", "pg_server_progress", "pg_server_pending",
-    "pg_server_run_plain",
+    "pg_server_run_plain", "pg_server_set_batch",
 ]
 OP_INSERT, OP_QUERY, OP_REMOVE = 0, 1, 2
 
@@ -116,6 +116,7 @@ def lib():
     L.pg_server_progress.argtypes = [vp, vp, u32, ctypes.POINTER(u32)]
     L.pg_server_pending.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     L.pg_server_run_plain.argtypes = [vp, u32, u32, vp, vp, u64, vp, vp]
+    L.pg_server_set_batch.argtypes = [vp, u64, u32]
     L.pg_bulk_insert_if_not_exists.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pg_bulk_insert_if_not_exists_delete.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pg_fastmod_host.argtypes = [u64, u64]
@@ -431,6 +432,9 @@ class Server:
         nd = ctypes.c_uint32()
         check(lib().pg_server_progress(self._h, self._done, 65536, ctypes.byref(nd)), "pg_server_progress")
         return [self._done[i] for i in range(nd.value)]
+
+    def set_batch(self, max_batch_keys: int, chunk: int = 2048):
+        check(lib().pg_server_set_batch(self._h, max_batch_keys, chunk), "pg_server_set_batch")
 
     def pending(self):
         q, f = ctypes.c_uint64(), ctypes.c_uint64()

@@ -249,6 +249,7 @@ struct pg_server_s {
     uint64_t next_ticket = 1;
     unsigned blocks = 0;
     uint32_t chunk = 2048;
+    uint64_t max_batch_keys = 1ull << 22;  // bounds a batch's time on the device
 };
 
 extern "C" {
@@ -350,13 +351,15 @@ int pg_server_submit(pg_server s, uint32_t table, uint32_t op, const uint64_t *h
 // Launch queued requests that fit into one batch on a free stream.
 static int launch_batch(pg_server s, uint32_t si) {
     const uint64_t free_bytes = s->ring.largest_free();
-    uint64_t need = 0;
+    uint64_t need = 0, keys = 0;
     size_t take = 0;
     const uint64_t fixed = kAlign /* counter */;
     while (take < s->queue.size() && take < kMaxBatchReqs) {
         const uint64_t meta = al(sizeof(pg::ReqDev) * (take + 1)) + al(8 * (take + 2));
         if (fixed + meta + need + s->queue[take].need > free_bytes) break;
+        if (take > 0 && keys + s->queue[take].n > s->max_batch_keys) break;
         need += s->queue[take].need;
+        keys += s->queue[take].n;
         take++;
     }
     if (take == 0) return PG_OK;
@@ -471,6 +474,13 @@ int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndon
         if (rc != PG_OK) return rc;
     }
     if (ndone) *ndone = nd;
+    return PG_OK;
+}
+
+int pg_server_set_batch(pg_server s, uint64_t max_batch_keys, uint32_t chunk) {
+    if (!s || max_batch_keys == 0 || chunk == 0) return PG_ERR_INVALID;
+    s->max_batch_keys = max_batch_keys;
+    s->chunk = chunk;
     return PG_OK;
 }
 

@@ -8,6 +8,19 @@ import torch
 from . import gen_keys
 
 
+def keys_at(seed: int, idx: np.ndarray) -> np.ndarray:
+    """Host copy of the device key stream: splitmix64 output for counters
+    `idx` of stream `seed` (the same function as pg_gen_keys)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (idx.astype(np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    z[z == 0] = np.uint64(0x8000000000000000)
+    z[z == np.uint64(0xFFFFFFFFFFFFFFFF)] = np.uint64(0x7FFFFFFFFFFFFFFF)
+    return z
+
+
 def log_uniform_sizes(ntables: int, lo: int, hi: int, seed: int) -> np.ndarray:
     """Slot counts log-uniform in [2^lo, 2^hi]."""
     rng = np.random.default_rng(seed)

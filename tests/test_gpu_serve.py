@@ -100,11 +100,11 @@ def test_results_alone_equal_results_in_mix():
     srv_mix = pg.Server(tabs_mix, budget_bytes=64 << 20, nstreams=4)
     bufs_mix = [buffers(tabs_mix[t], op, k) for t, op, k in reqs]
     tickets = {}
+    done = set()
     for i, ((t, op, k), (kk, vv, oo, ff)) in enumerate(zip(reqs, bufs_mix)):
         tickets[srv_mix.submit(t, op, kk, vv, oo, ff)] = i
         if i % 7 == 0:
-            srv_mix.progress()
-    done = set()
+            done.update(srv_mix.progress())  # requests keep arriving while batches run
     for _ in range(1_000_000):
         done.update(srv_mix.progress())
         if len(done) == len(reqs):
@@ -123,6 +123,9 @@ def test_results_alone_equal_results_in_mix():
                 assert (om == oa).all() and (om.view(np.uint64) == (k ^ GOLD)).all(), i
             else:
                 assert (om == oa).mean() > 0.995, i   # fingerprint collisions may pick another slot
+        elif op == pg.OP_INSERT:
+            # the level a key lands in (status 1 or 2) depends on insertion order
+            assert (fm > 0).all() and (fa > 0).all(), i
         else:
             assert (fm == fa).all() and fm.all(), (i, op)
     srv_mix.free()
@@ -136,7 +139,7 @@ def test_admission_respects_budget_and_drains_queue():
     kk, _, _, ff = buffers(tabs[0], pg.OP_INSERT, big)
     with pytest.raises(pg.PgError):
         srv.submit(0, pg.OP_INSERT, kk, None, None, ff)
-    reqs = [oracle.gen_keys(2, i * 20000, 20000) for i in range(40)]  # ~16 MB in total
+    reqs = [oracle.gen_keys(2, i * 20000, 20000) for i in range(20)]  # 4.4 MB staged in total, 76 % load
     bufs = [buffers(tabs[0], pg.OP_INSERT, k) for k in reqs]
     for (kk, vv, oo, ff) in bufs:
         srv.submit(0, pg.OP_INSERT, kk, vv, oo, ff)
@@ -153,5 +156,5 @@ def test_admission_respects_budget_and_drains_queue():
     assert max_inflight * 20000 * 11 <= (1 << 20)  # keys + vals + flags of in-flight requests fit
     for (kk, vv, oo, ff) in bufs:
         assert ff.numpy().all()
-    assert tabs[0].get_fill() == 40 * 20000
+    assert tabs[0].get_fill() == 20 * 20000
     srv.free()
