@@ -69,8 +69,14 @@ def build_cpp_tests(verbose: bool = False) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"C++ front-end test build failed:\n{r.stdout}\n{r.stderr}")
+    ring_src = os.path.join(root, "tests", "cpp", "test_ring.cpp")
+    ring_out = os.path.join(root, "tests", "cpp", "test_ring")
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", ring_src, "-o", ring_out], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"ring test build failed:\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {out}")
+        print(f"built {out} and {ring_out}")
     return out
 
 

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "pg_kv_ops.hpp"
+#include "pg_ring.hpp"
 #include "pg_tcf.hpp"
 
 namespace pg {
@@ -150,46 +151,6 @@ static int launch_serve(uint32_t P, const ServeArgs &a, unsigned blocks, hipStre
     }
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
 }
-
-// FIFO ring over the staging arena: batches allocate at the head and are
-// released from the tail in launch order.
-struct Ring {
-    uint64_t cap = 0, head = 0;
-    std::deque<std::pair<uint64_t, uint64_t>> live;  // [begin, end) per batch, oldest first
-
-    uint64_t largest_free() const {
-        if (live.empty()) return cap;
-        const uint64_t t = live.front().first;
-        if (live.back().first < t) return t - head;  // wrapped: free is [head, t)
-        const uint64_t a = cap - head, b = t;
-        return a > b ? a : b;
-    }
-    bool alloc(uint64_t need, uint64_t &at) {
-        if (need > cap) return false;
-        if (live.empty()) {
-            at = 0;
-        } else {
-            const uint64_t t = live.front().first;
-            if (live.back().first < t) {
-                if (t - head < need) return false;
-                at = head;
-            } else if (cap - head >= need) {
-                at = head;
-            } else if (t >= need) {
-                at = 0;
-            } else {
-                return false;
-            }
-        }
-        head = at + need;
-        live.push_back({at, head});
-        return true;
-    }
-    void release_front() {
-        live.pop_front();
-        if (live.empty()) head = 0;
-    }
-};
 
 }  // namespace pg
 

@@ -205,6 +205,27 @@ template <int P>
 __device__ bool gen_find(const uint32_t *base, uint32_t B, uint32_t fp, const Tile<P> &t,
                          uint32_t &val, bool &owner, int &fill) {
     fill = 0;
+    if ((B & 3u) == 0 && B <= 4u * P) {
+        // The whole bucket in one row: lane r holds words 4r..4r+3, read with
+        // one 16 B load (B % 4 == 0 keeps every bucket 16 B aligned).
+        const bool ok = 4u * t.rank < B;
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        if (ok) load_words<4>(base + 4 * t.rank, w);
+        uint32_t m = 0;
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+            if (ok && (w[v] & 0xFFFFu) == fp) m |= 1u << v;
+        const uint64_t lanes = t.bits(m != 0);
+        if (lanes) {
+            const uint32_t l = (uint32_t)ctz64(lanes);
+            owner = (t.rank == l);
+            if (owner) val = select_word<4>(w, __builtin_ctz(m)) >> 16;
+            return true;
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) fill += t.popc(ok && w_filled(w[v]));
+        return false;
+    }
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
