@@ -274,6 +274,17 @@ int pg_server_set_batch(pg_server s, uint64_t max_batch_keys, uint32_t chunk);
 int pg_server_run_plain(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,
                         uint64_t n, void *h_vals_out, uint8_t *h_flags);
 
+/* ---- raw bucket-array files (layout checks, checkpoint / resume) ----
+ * No reference counterpart: the reference never serializes a table
+ * (tables/bucketed_table.cuh:194-224 only frees).  Format, little endian:
+ *   "PGAMDTB1" | u32 version (1) | u32 header bytes | pg_desc | u64 seed |
+ *   u64 bytes of level 0 | ... | raw slot array of level 0 | ...
+ * The slot arrays are the device layout byte for byte (see pg_level_ptr). */
+int pg_save(pg_handle h, const char *path);        /* synchronises */
+int pg_load(const char *path, pg_handle *out);     /* pg_create + copy in */
+/* Header only, no device work: descriptor and seed of a saved table. */
+int pg_file_info(const char *path, pg_desc *desc, uint64_t *seed);
+
 /* Human-readable name of a status code. */
 const char *pg_error_string(int status);
 

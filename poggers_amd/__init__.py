@@ -39,7 +39,7 @@ EXPORTS = [
     "pg_batch_insert", "pg_batch_query", "pg_batch_capture", "pg_batch_replay", "pg_batch_fill",
     "pg_batch_level_ptr", "pg_batch_bytes",
     "pg_server_create", "pg_server_destroy", "pg_server_submit", "pg_server_progress", "pg_server_pending",
-    "pg_server_run_plain", "pg_server_set_batch",
+    "pg_server_run_plain", "pg_server_set_batch", "pg_save", "pg_load", "pg_file_info",
 ]
 OP_INSERT, OP_QUERY, OP_REMOVE = 0, 1, 2
 
@@ -117,6 +117,9 @@ def lib():
     L.pg_server_pending.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     L.pg_server_run_plain.argtypes = [vp, u32, u32, vp, vp, u64, vp, vp]
     L.pg_server_set_batch.argtypes = [vp, u64, u32]
+    L.pg_save.argtypes = [vp, ctypes.c_char_p]
+    L.pg_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.pg_file_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(Desc), ctypes.POINTER(u64)]
     L.pg_bulk_insert_if_not_exists.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pg_bulk_insert_if_not_exists_delete.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp, vp]
     L.pg_fastmod_host.argtypes = [u64, u64]
@@ -309,6 +312,21 @@ class Table:
         _hip_memcpy(buf, ptr, n, 2)  # hipMemcpyDeviceToHost
         return buf.raw
 
+    def save(self, path: str):
+        """Write the table (descriptor, seed, raw slot arrays) to `path`."""
+        check(lib().pg_save(self._h, path.encode()), "pg_save")
+
+    @classmethod
+    def load(cls, path: str) -> "Table":
+        """Recreate a table saved with save()."""
+        desc, seed = file_info(path)
+        t = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        check(lib().pg_load(path.encode(), ctypes.byref(h)), "pg_load")
+        t.desc, t.seed, t._h = desc, seed, h
+        t._cdesc = desc.to_c()
+        return t
+
     def level_upload(self, level: int, data: bytes):
         """Overwrite one level's slot array with a raw image (host -> device)."""
         import torch
@@ -459,6 +477,16 @@ def _hip_memcpy(dst, src, n: int, kind: int):
     rc = _hip.hipMemcpy(d, ctypes.c_void_p(src), n, kind)
     if rc != 0:
         raise PgError(f"hipMemcpy failed ({rc})")
+
+
+def file_info(path: str):
+    """(TableDesc, seed) of a table file, read on the host only."""
+    d = Desc()
+    seed = ctypes.c_uint64()
+    check(lib().pg_file_info(path.encode(), ctypes.byref(d), ctypes.byref(seed)), "pg_file_info")
+    levels = [Level(d.level[i].nslots, d.level[i].scheme, d.level[i].bucket_size, d.level[i].tile,
+                    d.level[i].prober, d.level[i].hasher, d.level[i].max_probes) for i in range(d.num_levels)]
+    return TableDesc(d.rep, levels), seed.value
 
 
 def gen_keys(seed: int, start: int, n: int, out, stream=None):

@@ -1,5 +1,6 @@
 // pg_capi.hip -- host side of the C ABI (include/poggers_amd.h).
 #include <new>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -302,6 +303,103 @@ int pg_level_ptr(pg_handle h, uint32_t level, void **d_ptr, uint64_t *bytes) {
     if (!h || level >= h->dev.nlevels) return PG_ERR_INVALID;
     if (d_ptr) *d_ptr = h->dev.lv[level].slots;
     if (bytes) *bytes = h->bytes[level];
+    return PG_OK;
+}
+
+// ---- raw bucket-array files ----
+static const char kMagic[8] = {'P', 'G', 'A', 'M', 'D', 'T', 'B', '1'};
+static const uint32_t kFileVersion = 1;
+
+struct FileHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t header_bytes;
+    pg_desc desc;
+    uint64_t seed;
+    uint64_t level_bytes[PG_MAX_LEVELS];
+};
+
+static int read_header(FILE *f, FileHeader *h) {
+    if (fread(h, sizeof(FileHeader), 1, f) != 1) return PG_ERR_INVALID;
+    if (memcmp(h->magic, kMagic, 8) != 0 || h->version != kFileVersion || h->header_bytes != sizeof(FileHeader))
+        return PG_ERR_INVALID;
+    if (h->desc.num_levels < 1 || h->desc.num_levels > PG_MAX_LEVELS) return PG_ERR_INVALID;
+    return PG_OK;
+}
+
+int pg_file_info(const char *path, pg_desc *desc, uint64_t *seed) {
+    if (!path) return PG_ERR_INVALID;
+    FILE *f = fopen(path, "rb");
+    if (!f) return PG_ERR_INVALID;
+    FileHeader h;
+    const int rc = read_header(f, &h);
+    fclose(f);
+    if (rc != PG_OK) return rc;
+    if (desc) *desc = h.desc;
+    if (seed) *seed = h.seed;
+    return PG_OK;
+}
+
+int pg_save(pg_handle h, const char *path) {
+    if (!h || !path) return PG_ERR_INVALID;
+    FileHeader fh{};
+    memcpy(fh.magic, kMagic, 8);
+    fh.version = kFileVersion;
+    fh.header_bytes = sizeof(FileHeader);
+    fh.desc = h->desc;
+    fh.seed = h->seed;
+    for (uint32_t l = 0; l < h->dev.nlevels; l++) fh.level_bytes[l] = h->bytes[l];
+    if (hipDeviceSynchronize() != hipSuccess) return PG_ERR_HIP;
+    FILE *f = fopen(path, "wb");
+    if (!f) return PG_ERR_INVALID;
+    int rc = fwrite(&fh, sizeof(fh), 1, f) == 1 ? PG_OK : PG_ERR_INVALID;
+    const uint64_t chunk = 64ull << 20;
+    void *buf = nullptr;
+    if (rc == PG_OK && hipHostMalloc(&buf, chunk) != hipSuccess) rc = PG_ERR_NOMEM;
+    for (uint32_t l = 0; l < h->dev.nlevels && rc == PG_OK; l++) {
+        for (uint64_t off = 0; off < h->bytes[l] && rc == PG_OK; off += chunk) {
+            const uint64_t n = h->bytes[l] - off < chunk ? h->bytes[l] - off : chunk;
+            if (hipMemcpy(buf, (char *)h->dev.lv[l].slots + off, n, hipMemcpyDeviceToHost) != hipSuccess)
+                rc = PG_ERR_HIP;
+            else if (fwrite(buf, 1, n, f) != n)
+                rc = PG_ERR_INVALID;
+        }
+    }
+    if (buf) hipHostFree(buf);
+    if (fclose(f) != 0 && rc == PG_OK) rc = PG_ERR_INVALID;
+    return rc;
+}
+
+int pg_load(const char *path, pg_handle *out) {
+    if (!path || !out) return PG_ERR_INVALID;
+    *out = nullptr;
+    FILE *f = fopen(path, "rb");
+    if (!f) return PG_ERR_INVALID;
+    FileHeader fh;
+    int rc = read_header(f, &fh);
+    pg_handle h = nullptr;
+    if (rc == PG_OK) rc = pg_create(&fh.desc, fh.seed, &h);
+    for (uint32_t l = 0; l < fh.desc.num_levels && rc == PG_OK; l++)
+        if (fh.level_bytes[l] != h->bytes[l]) rc = PG_ERR_INVALID;
+    const uint64_t chunk = 64ull << 20;
+    void *buf = nullptr;
+    if (rc == PG_OK && hipHostMalloc(&buf, chunk) != hipSuccess) rc = PG_ERR_NOMEM;
+    for (uint32_t l = 0; l < fh.desc.num_levels && rc == PG_OK; l++) {
+        for (uint64_t off = 0; off < h->bytes[l] && rc == PG_OK; off += chunk) {
+            const uint64_t n = h->bytes[l] - off < chunk ? h->bytes[l] - off : chunk;
+            if (fread(buf, 1, n, f) != n)
+                rc = PG_ERR_INVALID;
+            else if (hipMemcpy((char *)h->dev.lv[l].slots + off, buf, n, hipMemcpyHostToDevice) != hipSuccess)
+                rc = PG_ERR_HIP;
+        }
+    }
+    if (buf) hipHostFree(buf);
+    fclose(f);
+    if (rc != PG_OK) {
+        if (h) pg_destroy(h);
+        return rc;
+    }
+    *out = h;
     return PG_OK;
 }
 
