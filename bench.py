@@ -37,7 +37,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-roofline", action="store_true", help="skip the random-sector microbenchmark")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a 1 GiB device copy (scales PMC byte counters)")
+                    help="also run a 1 GiB streaming read+write kernel and 2^24 random 64 B reads "
+                         "of a 1 GiB buffer (known byte counts that scale the PMC byte counters)")
+    ap.add_argument("--summarize-profiles", nargs=3, metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/ summaries from rocprofv3 csv output (kernel stats, "
+                         "FETCH_SIZE run, WRITE_SIZE run) and exit")
     ap.add_argument("--requests", type=int, default=400, help="serve: requests per experiment")
     ap.add_argument("--workload", choices=["tcf", "batched", "kv", "serve"], default="tcf",
                     help="tcf: the headline 2^28-slot filter; batched: 4096 uneven filters in one graph; "
@@ -377,6 +381,147 @@ def run_batched(args):
     print(json.dumps(out), flush=True)
 
 
+CAL_COPY_BYTES = 1 << 30          # --calibrate-copy: bytes read and written by the copy
+CAL_RANDOM_READS = 1 << 24        # --calibrate-copy: random 64 B reads
+PROFILE_CMDS = {
+    "stats": "rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_stats -o tcf -- "
+             "python3 bench.py --steps 3 --warmup 1 --no-cpu --no-roofline",
+    "fetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/pmc_fetch -o tcf -- "
+             "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-roofline --calibrate-copy",
+    "write": "rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/pmc_write -o tcf -- "
+             "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-roofline --calibrate-copy",
+    "summary": "python3 bench.py --summarize-profiles gpurun_out/prof_stats gpurun_out/pmc_fetch "
+               "gpurun_out/pmc_write",
+}
+
+
+def _csv_rows(directory, suffix):
+    import csv
+    import glob
+    paths = sorted(glob.glob(os.path.join(directory, "**", "*" + suffix), recursive=True))
+    if not paths:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    with open(paths[0], newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _short(name, n=110):
+    return name if len(name) <= n else name[:n - 3] + "..."
+
+
+def summarize_profiles(stats_dir, fetch_dir, write_dir):
+    """Condense rocprofv3 csv output into the committed text summaries:
+    profiles/tcf_2p28_kernel_stats.md (per-kernel launch statistics, and the
+    per-dispatch HBM counters with their calibration) and
+    profiles/tcf_2p28_traffic.json (the insert kernel's HBM bytes per launch,
+    read by the bench line's roofline.traffic)."""
+    stats = _csv_rows(stats_dir, "_kernel_stats.csv")
+    fetch = _csv_rows(fetch_dir, "_counter_collection.csv")
+    write = _csv_rows(write_dir, "_counter_collection.csv")
+
+    def per_kernel(rows):
+        out = {}
+        for r in rows:
+            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return out
+
+    grids = {}
+    for r in fetch:
+        grids.setdefault(r["Kernel_Name"], []).append(int(r["Grid_Size"]))
+
+    fk, wk = per_kernel(fetch), per_kernel(write)
+
+    def pick(table, pred):
+        best = None
+        for name, vals in table.items():
+            if pred(name):
+                v = max(vals)
+                if best is None or v > best[1]:
+                    best = (name, v)
+        return best
+
+    is_cal_copy = lambda n: "neg" in n and "double" in n
+    copy_f = pick(fk, is_cal_copy)
+    copy_w = pick(wk, is_cal_copy)
+    rand_f = pick(fk, lambda n: "k_random_sector" in n)
+    if copy_f is None or copy_w is None:
+        raise SystemExit("the calibration kernel (torch.neg over 1 GiB of float64) is not in the PMC runs")
+    fetch_factor = CAL_COPY_BYTES / (copy_f[1] * 1024.0)
+    write_factor = CAL_COPY_BYTES / (copy_w[1] * 1024.0)
+    rand_factor = CAL_RANDOM_READS * 64 / (rand_f[1] * 1024.0) if rand_f else None
+
+    def mean_of(table, key):
+        names = [n for n in table if key in n and ", true" not in n]
+        vals = [v for n in names for v in table[n]]
+        return (names[0] if names else None), (sum(vals) / len(vals) if vals else 0.0), len(vals)
+
+    rows = []
+    for key in ("k_tcf_insert", "k_tcf_query", "k_zero16"):
+        nf, mf, cf = mean_of(fk, key)
+        _, mw, _ = mean_of(wk, key)
+        if nf is None:
+            continue
+        rows.append((key, nf, cf, mf, mw))
+    ins = next(r for r in rows if r[0] == "k_tcf_insert")
+    # The insert kernel streams its keys (8 B each; threads = keys * tile, the
+    # tile being the kernel's first template argument) and reads buckets at
+    # random in 64 B requests.  FETCH_SIZE counts streamed line fills at half
+    # (the streaming calibration) and random 64 B reads exactly (the random
+    # calibration), so each part takes its own factor.
+    import re
+    tile = int(re.search(r"k_tcf_insert<(\d+),", ins[1]).group(1))
+    keys_per_launch = sum(grids[ins[1]]) / len(grids[ins[1]]) / tile
+    key_bytes = keys_per_launch * 8
+    fetch_b, write_b = ins[3] * 1024, ins[4] * 1024
+    copy_scaled = fetch_b * fetch_factor + write_b * write_factor
+    mixed = ((fetch_b - key_bytes / fetch_factor) * rand_factor + key_bytes + write_b * write_factor
+             if rand_factor else copy_scaled)
+    traffic = {
+        "kernel": "insert",
+        "kernel_name": ins[1],
+        "dispatches": ins[2],
+        "keys_per_launch": int(keys_per_launch),
+        "fetch_kib_per_launch": round(ins[3], 1),
+        "write_kib_per_launch": round(ins[4], 1),
+        "streaming_fetch_factor": round(fetch_factor, 4),
+        "write_factor": round(write_factor, 4),
+        "random64_fetch_factor": round(rand_factor, 4) if rand_factor else None,
+        "hbm_bytes_per_launch": int(mixed),
+        "hbm_bytes_per_launch_all_streaming_scaled": int(copy_scaled),
+        "commands": [PROFILE_CMDS["fetch"], PROFILE_CMDS["write"], PROFILE_CMDS["summary"]],
+    }
+    lines = ["# TCF 2^28 slots, 90 % load: rocprofv3 summaries", "",
+             "Commands (run from the repository root on one MI355X):", ""]
+    lines += [f"    {PROFILE_CMDS[k]}" for k in ("stats", "fetch", "write", "summary")]
+    lines += ["", "## Kernel statistics (`--kernel-trace --stats`)", "",
+              "Kernels whose fourth template argument is `true` are the accounting variants,",
+              "run once, untimed, to count bucket reads; the timed kernels are the `false` ones.", "",
+              "| kernel | calls | average us | total ms | % |", "|---|---|---|---|---|"]
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:12]:
+        lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.2f} |")
+    lines += ["", "## HBM counters per dispatch (`--pmc FETCH_SIZE`, `--pmc WRITE_SIZE`, KiB)", "",
+              "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB |", "|---|---|---|---|"]
+    for key, name, cnt, mf, mw in rows:
+        lines.append(f"| `{_short(name)}` | {cnt} | {mf:.1f} | {mw:.1f} |")
+    lines += ["", "Calibration with kernels of known traffic:", "",
+              f"* 1 GiB streaming read + write (`torch.neg` on float64): FETCH_SIZE {copy_f[1]:.0f} KiB "
+              f"(factor {fetch_factor:.3f}), WRITE_SIZE {copy_w[1]:.0f} KiB (factor {write_factor:.3f}).",
+              (f"* 2^24 random 64 B reads (`k_random_sector`): FETCH_SIZE {rand_f[1]:.0f} KiB "
+               f"(factor {rand_factor:.3f})." if rand_f else "* random 64 B calibration: not in this run."),
+              "", f"Insert kernel, {int(keys_per_launch):,} keys per launch: HBM bytes per launch = "
+              f"(FETCH - keys*8/{fetch_factor:.3f}) * {rand_factor or 0:.3f} + keys*8 + WRITE * {write_factor:.3f} "
+              f"= {traffic['hbm_bytes_per_launch']:,} ({traffic['hbm_bytes_per_launch'] / keys_per_launch:.1f} B "
+              f"per key); with every read scaled as streaming: "
+              f"{traffic['hbm_bytes_per_launch_all_streaming_scaled']:,}.", ""]
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", "tcf_2p28_kernel_stats.md"), "w") as f:
+        f.write("\n".join(lines))
+    with open(os.path.join(ROOT, "profiles", "tcf_2p28_traffic.json"), "w") as f:
+        json.dump(traffic, f, indent=1)
+    print("\n".join(lines))
+
+
 def load_traffic(path):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), committed as a small JSON summary; None when absent."""
@@ -404,7 +549,7 @@ def cpu_baseline(log2_slots=26):
     t0 = time.perf_counter()
     t.bulk_insert(keys)
     t1 = time.perf_counter()
-    t.bulk_query(keys[:nq])
+    t.bulk_query(keys[::max(1, n // nq)][:nq])  # present keys sampled across the insert order
     t.bulk_query(absent)
     t2 = time.perf_counter()
     ops = n + 2 * nq
@@ -423,6 +568,9 @@ def cpu_baseline(log2_slots=26):
 
 def main():
     args = parse()
+    if args.summarize_profiles:
+        summarize_profiles(*args.summarize_profiles)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: run with --gpus 1 (multi-GPU sharding "
                                    "is a design note only)"}))
@@ -593,12 +741,21 @@ def main():
             kernels[q]["frac_random_roofline"] = round(kernels[q]["achieved_gbs"] / rand["read"], 4)
 
     if args.calibrate_copy:
-        # a plain copy of known size, so PMC byte counters can be scaled
-        a = torch.ones(1 << 27, dtype=torch.int64, device=dev)
+        # a streaming kernel of known traffic (1 GiB read, 1 GiB written), so
+        # PMC byte counters can be scaled; an elementwise kernel rather than
+        # copy_(), which becomes a DMA transfer with no kernel to count ...
+        a = torch.ones(1 << 27, dtype=torch.float64, device=dev)
         b = torch.empty_like(a)
-        b.copy_(a)
+        torch.neg(a, out=b)
         torch.cuda.synchronize()
         del a, b
+        # ... and random 64 B reads of known count (read-only, one sector in
+        # flight per lane), the request size of the table kernels
+        buf = torch.zeros(1 << 27, dtype=torch.int64, device=dev)
+        sink = torch.zeros(1, dtype=torch.int64, device=dev)
+        pg.random_sector_bench(buf, CAL_RANDOM_READS, 0, 5, sink, stream)
+        torch.cuda.synchronize()
+        del buf
 
     dom = max(("insert", "query_present", "query_absent"), key=lambda k: acc[k])
     traffic_info = load_traffic(os.path.join(ROOT, "profiles", "tcf_2p28_traffic.json"))

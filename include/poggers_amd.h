@@ -54,8 +54,21 @@ enum pg_status {
  *   FP16_V16  dynamic_container<bit_grouped_container<16,16>, uint16_t>:
  *             one uint32 word = val16 << 16 | fp16
  *             (representations/dynamic_container.cuh:23-141,
- *              grouped_storage_sub_bits.cuh:99-260)                      */
-enum pg_rep { PG_REP_KV64 = 0, PG_REP_FP16_V16 = 1 };
+ *              grouped_storage_sub_bits.cuh:99-260)
+ *   FP8_V8    dynamic_container<bit_grouped_container<8,8>, uint8_t>:
+ *             one uint16 word = val8 << 8 | fp8; fp8 = low byte of the key,
+ *             0 -> 1, 0xFF -> 0xFE (tests/src/delete_tests.cu:165-179)
+ *   FP10_V6   dynamic_container<bit_grouped_container<10,6>, uint16_t>:
+ *             one uint16 word = val6 << 10 | (fp16 & 0x3FF); fp16 is
+ *             remapped BEFORE the masking, as in the reference, so a key
+ *             whose low 10 bits are 0 (or 0x3FF) stores as an empty-looking
+ *             (or tombstone-looking) word (tests/src/test_mhm_tcf.cu:105-116).
+ *             The probe tag (XOR_P2) is the unmasked fp16.
+ *   Value buffers hold the reference's Val type: uint64 (KV64), uint16
+ *   (FP16_V16), uint8 (FP8_V8 and FP10_V6, which keep the low 8 / 6 bits).
+ *   The 8/8 and 10/6 formats run the generic (runtime bucket size) kernels;
+ *   batches and the server take KV64 and FP16_V16 tables only.            */
+enum pg_rep { PG_REP_KV64 = 0, PG_REP_FP16_V16 = 1, PG_REP_FP8_V8 = 2, PG_REP_FP10_V6 = 3 };
 
 /* Insert scheme of a level.
  *   BUCKET        insert_schemes/bucket_insert.cuh:38-361
@@ -142,7 +155,8 @@ int pg_set_mode(pg_handle h, uint32_t mode);
 /* Bulk insert: replaces the per-key device op insert(tile, key, val)
  * (tables/bucketed_table.cuh:271-291) driven by speed_insert_kernel
  * (tests/src/test_mhm_tcf.cu:318-345).
- *   d_vals       uint64 per key (KV64) or uint16 per key (FP16_V16);
+ *   d_vals       one Val per key: uint64 (KV64), uint16 (FP16_V16) or
+ *                uint8 (FP8_V8, FP10_V6);
  *                NULL inserts value 0 (bucketed_table.cuh:324-329).
  *   d_status     optional, one byte per key: 0 failed, 1 + level index.
  *   d_fail_count optional uint64 device counter, incremented per failure

@@ -30,9 +30,11 @@ struct key_val_pair {  // key_val_pair<Key, Val>: 16 B {u64 key, u64 val}
 };
 template <std::size_t KeyBits, std::size_t ValBits>
 struct bit_grouped_container {  // dynamic_container<bit_grouped_container<K,V>, SmallKey>
-    static_assert(KeyBits == 16 && ValBits == 16,
-                  "the MI355X build stores 16-bit fingerprints with 16-bit values");
-    static constexpr uint32_t rep = PG_REP_FP16_V16;
+    static_assert((KeyBits == 16 && ValBits == 16) || (KeyBits == 8 && ValBits == 8) ||
+                      (KeyBits == 10 && ValBits == 6),
+                  "packed slots are 16/16, 8/8 or 10/6 (the reference's compositions)");
+    static constexpr uint32_t rep =
+        KeyBits == 16 ? PG_REP_FP16_V16 : (KeyBits == 8 ? PG_REP_FP8_V8 : PG_REP_FP10_V6);
 };
 
 // ---- insert schemes (insert_schemes/*.cuh) ----
@@ -116,7 +118,10 @@ class bucketed_table {
 
     static_assert(std::is_same<Key, uint64_t>::value, "keys are uint64_t");
     static_assert(rep != PG_REP_KV64 || std::is_same<Val, uint64_t>::value, "key_val_pair values are uint64_t");
-    static_assert(rep != PG_REP_FP16_V16 || sizeof(Val) <= 2, "filter values are at most 16 bits");
+    static_assert(rep != PG_REP_FP16_V16 || std::is_same<Val, uint16_t>::value,
+                  "16/16 filter values are uint16_t");
+    static_assert(!(rep == PG_REP_FP8_V8 || rep == PG_REP_FP10_V6) || std::is_same<Val, uint8_t>::value,
+                  "8/8 and 10/6 filter values are uint8_t");
     static_assert(Partition_Size >= 1 && Partition_Size <= 64 && (Partition_Size & (Partition_Size - 1)) == 0,
                   "Partition_Size is a power of two of at most one wavefront (64 lanes)");
     static_assert(Bucket_Size >= 1 && Bucket_Size <= 64, "Bucket_Size in 1..64");
@@ -240,6 +245,20 @@ using tcf_backing = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16,
 using tcf = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16, 16>, 4, 16,
                            power_of_n_insert_shortcut_bucket_scheme, 2, XORPowerOfTwoHasher, murmurHasher, true,
                            tcf_backing>;
+
+// The reference's 8/8 and 10/6 compositions: del_TCF_small
+// (tests/src/delete_tests.cu:165-179) and the TCF of
+// tests/src/test_mhm_tcf.cu:105-116.
+using small_tcf_backing = bucketed_table<uint64_t, uint8_t, bit_grouped_container<8, 8>, 4, 8,
+                                         linear_insert_bucket_scheme, 400, linearProber, murmurHasher>;
+using small_tcf = bucketed_table<uint64_t, uint8_t, bit_grouped_container<8, 8>, 4, 16,
+                                 power_of_n_insert_shortcut_bucket_scheme, 2, XORPowerOfTwoHasher, murmurHasher,
+                                 true, small_tcf_backing>;
+using tcf_10_6_backing = bucketed_table<uint64_t, uint8_t, bit_grouped_container<10, 6>, 1, 8,
+                                        linear_insert_bucket_scheme, 20, doubleHasher, murmurHasher>;
+using tcf_10_6 = bucketed_table<uint64_t, uint8_t, bit_grouped_container<10, 6>, 1, 8,
+                                power_of_n_insert_shortcut_bucket_scheme, 2, doubleHasher, murmurHasher, true,
+                                tcf_10_6_backing>;
 
 // The MI355X default: two choices from one fmix64 hash split into 32-bit
 // halves, tile 2 (the fastest measured shape at 2^28 slots).

@@ -19,7 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-REP_KV64, REP_FP16_V16, REP_FP8_V8 = 0, 1, 2
+REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6 = 0, 1, 2, 3
 SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS = 0, 1, 2, 3
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
@@ -171,7 +171,7 @@ class Table:
         self._h = lib().or_create(rep, len(levels), arr, seed)
         if not self._h:
             raise ValueError("oracle table creation failed")
-        self.val_bytes = 8 if rep == REP_KV64 else (1 if rep == REP_FP8_V8 else 2)
+        self.val_bytes = 8 if rep == REP_KV64 else (1 if rep in (REP_FP8_V8, REP_FP10_V6) else 2)
         self.val_dtype = {8: np.uint64, 2: np.uint16, 1: np.uint8}[self.val_bytes]
 
     def __del__(self):
@@ -245,7 +245,8 @@ class Table:
         return ctypes.string_at(p, n)
 
     def level_array(self, lvl: int) -> np.ndarray:
-        dt = {REP_KV64: np.uint64, REP_FP16_V16: np.uint32, REP_FP8_V8: np.uint16}[self.rep]
+        dt = {REP_KV64: np.uint64, REP_FP16_V16: np.uint32, REP_FP8_V8: np.uint16,
+              REP_FP10_V6: np.uint16}[self.rep]
         return np.frombuffer(self.level_bytes(lvl), dtype=dt).copy()
 
     def num_buckets(self, lvl: int) -> int:

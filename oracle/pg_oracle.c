@@ -36,7 +36,7 @@
 
 /* ---- enums (values mirror include/poggers_amd.h so tests can share one
  *      descriptor layout; the oracle does not include that header) ---- */
-enum { OR_REP_KV64 = 0, OR_REP_FP16_V16 = 1, OR_REP_FP8_V8 = 2 };
+enum { OR_REP_KV64 = 0, OR_REP_FP16_V16 = 1, OR_REP_FP8_V8 = 2, OR_REP_FP10_V6 = 3 };
 enum { OR_SCHEME_BUCKET = 0, OR_SCHEME_SINGLE_SLOT = 1,
        OR_SCHEME_POWER_OF_N_SHORTCUT = 2, OR_SCHEME_LINEAR_BUCKETS = 3 };
 enum { OR_PROBE_LINEAR = 0, OR_PROBE_DOUBLE = 1, OR_PROBE_P2 = 2,
@@ -225,8 +225,12 @@ int or_probes(uint32_t prober, uint32_t hasher, uint32_t max_probes,
 /* Slot encodings                                                      */
 /* ------------------------------------------------------------------ */
 
-/* Fingerprint: the low bits of the RAW key, 0 -> 1, all-ones -> all-ones-1.
- * representations/dynamic_container.cuh:35-50. */
+/* Fingerprint: the low bits of the RAW key, 0 -> 1, all-ones -> all-ones-1,
+ * at the width of the SmallKey type (representations/dynamic_container.cuh:
+ * 35-50): uint8 for the 8/8 slot, uint16 for 16/16 and for 10/6.  The 10/6
+ * slot keeps only the low 10 bits of that (grouped_storage_sub_bits.cuh:
+ * 85-87), so the remap happens before the masking; the probe tag is the
+ * unmasked SmallKey. */
 static uint32_t fp_of(uint32_t rep, uint64_t key)
 {
     if (rep == OR_REP_FP8_V8) {
@@ -283,22 +287,31 @@ typedef struct {
     uint64_t buckets_read[2];  /* distinct bucket examinations per level */
 } or_table;
 
-/* Word access for the packed (filter) representations. */
+/* Word access for the packed (filter) representations: 16-bit words for the
+ * 8/8 and 10/6 slots, 32-bit words for 16/16. */
+static inline int narrow(const or_table *t) { return t->rep == OR_REP_FP8_V8 || t->rep == OR_REP_FP10_V6; }
 static inline uint32_t wget(const or_table *t, const or_level *L, uint64_t i)
 {
-    if (t->rep == OR_REP_FP8_V8) return ((const uint16_t *)L->slots)[i];
+    if (narrow(t)) return ((const uint16_t *)L->slots)[i];
     return ((const uint32_t *)L->slots)[i];
 }
 static inline void wset(const or_table *t, or_level *L, uint64_t i, uint32_t w)
 {
-    if (t->rep == OR_REP_FP8_V8) ((uint16_t *)L->slots)[i] = (uint16_t)w;
+    if (narrow(t)) ((uint16_t *)L->slots)[i] = (uint16_t)w;
     else ((uint32_t *)L->slots)[i] = w;
 }
-static inline uint32_t kbits_of(const or_table *t) { return t->rep == OR_REP_FP8_V8 ? 8 : 16; }
-static inline uint32_t kmask_of(const or_table *t) { return t->rep == OR_REP_FP8_V8 ? 0xFFu : 0xFFFFu; }
+static inline uint32_t kbits_of(const or_table *t)
+{
+    return t->rep == OR_REP_FP8_V8 ? 8 : (t->rep == OR_REP_FP10_V6 ? 10 : 16);
+}
+static inline uint32_t vbits_of(const or_table *t)
+{
+    return t->rep == OR_REP_FP8_V8 ? 8 : (t->rep == OR_REP_FP10_V6 ? 6 : 16);
+}
+static inline uint32_t kmask_of(const or_table *t) { return (1u << kbits_of(t)) - 1u; }
 static inline uint32_t slot_bytes(const or_table *t)
 {
-    return t->rep == OR_REP_KV64 ? 16u : (t->rep == OR_REP_FP8_V8 ? 2u : 4u);
+    return t->rep == OR_REP_KV64 ? 16u : (narrow(t) ? 2u : 4u);
 }
 
 /* Packed-slot predicates: grouped_storage_sub_bits.cuh:160-195.
@@ -382,7 +395,7 @@ static int pb_query(const or_table *t, const or_level *L, uint64_t b, uint32_t f
 {
     for (uint32_t i = 0; i < L->B; i++) {
         uint32_t w = wget(t, L, b * L->B + i);
-        if ((w & kmask_of(t)) == fp) {
+        if ((w & kmask_of(t)) == (fp & kmask_of(t))) {
             *val = w >> kbits_of(t);
             return 1;
         }
@@ -397,7 +410,7 @@ static int pb_remove(or_table *t, or_level *L, uint64_t b, uint32_t fp)
     for (uint32_t i = 0; i < L->B; i++) {
         uint64_t s = b * L->B + i;
         uint32_t w = wget(t, L, s);
-        if ((w & kmask_of(t)) == fp) {
+        if ((w & kmask_of(t)) == (fp & kmask_of(t))) {
             wset(t, L, s, tomb_word(t));
             t->sectors_written += 1;
             return 1;
@@ -729,8 +742,7 @@ int or_insert(or_table *t, uint64_t key, uint64_t val, int with_delete)
             if (kv_insert(t, &t->lv[l], (int)l, key, val)) return 1 + (int)l;
         return 0;
     }
-    uint32_t vbits = kbits_of(t);
-    uint32_t word = or_pack(vbits, vbits, fp_of(t->rep, key), (uint32_t)val);
+    uint32_t word = or_pack(kbits_of(t), vbits_of(t), fp_of(t->rep, key), (uint32_t)val);
     for (uint32_t l = 0; l < t->nlevels; l++)
         if (lvl_insert(t, (int)l, key, word, with_delete)) return 1 + (int)l;
     return 0;
@@ -820,8 +832,7 @@ int or_insert_if_not_exists(or_table *t, uint64_t key, uint64_t val, int del, in
     *found = 0;
     *val_out = 0;
     if (t->rep == OR_REP_KV64) return -1; /* bucket_insert has no upsert */
-    uint32_t vbits = kbits_of(t);
-    uint32_t word = or_pack(vbits, vbits, fp_of(t->rep, key), (uint32_t)val);
+    uint32_t word = or_pack(kbits_of(t), vbits_of(t), fp_of(t->rep, key), (uint32_t)val);
     for (uint32_t l = 0; l < t->nlevels; l++) {
         or_level *L = &t->lv[l];
         uint32_t v = 0;

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libpoggers_amd.so")
 
 # enum values of include/poggers_amd.h
 PG_OK = 0
-REP_KV64, REP_FP16_V16 = 0, 1
+REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6 = 0, 1, 2, 3
 SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS = 0, 1, 2, 3
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
@@ -185,18 +185,20 @@ class TableDesc:
 
 def tcf_desc(nslots: int, backing_slots: int | None = None, bucket_size: int = 16, tile: int = 4,
              prober: int = PROBE_SPLIT64, hasher: int = HASH_FMIX64, backing_bucket: int = 8,
-             backing_prober: int = PROBE_LINEAR, backing_probes: int = 4000) -> TableDesc:
-    """Two-Choice Filter: a POWER_OF_N_SHORTCUT level with 16-bit fingerprints
-    and 16-bit values plus a LINEAR_BUCKETS backing level (the reference's
-    del_TCF composition, tests/src/delete_tests.cu:138-152).  By default the
-    backing level holds 1 % of the primary's slots."""
+             backing_prober: int = PROBE_LINEAR, backing_probes: int = 4000,
+             rep: int = REP_FP16_V16) -> TableDesc:
+    """Two-Choice Filter: a POWER_OF_N_SHORTCUT level plus a LINEAR_BUCKETS
+    backing level (the reference's del_TCF composition,
+    tests/src/delete_tests.cu:138-152).  ``rep`` picks the slot: 16-bit
+    fingerprints and values (default), 8/8 or 10/6.  By default the backing
+    level holds 1 % of the primary's slots."""
     if backing_slots is None:
         backing_slots = nslots // 100
     levels = [Level(nslots, SCHEME_SHORTCUT, bucket_size, tile, prober, hasher, 2)]
     if backing_slots:
         levels.append(Level(backing_slots, SCHEME_LINEAR_BUCKETS, backing_bucket, tile, backing_prober,
                             hasher, backing_probes))
-    return TableDesc(REP_FP16_V16, levels)
+    return TableDesc(rep, levels)
 
 
 def kv_desc(nslots: int, bucket_size: int = 1, tile: int = 4, prober: int = PROBE_LINEAR,
@@ -215,7 +217,7 @@ def kv_desc(nslots: int, bucket_size: int = 1, tile: int = 4, prober: int = PROB
 class Table:
     """A device-resident table (``pg_create``).  Key/value/result arguments
     are torch CUDA tensors: keys int64 (uint64 bit patterns), values int64
-    (KV) or int16 (filters), flags uint8."""
+    (KV), int16 (16/16 filters) or uint8 (8/8 and 10/6 filters), flags uint8."""
 
     def __init__(self, desc: TableDesc, seed: int = 42):
         self.desc = desc

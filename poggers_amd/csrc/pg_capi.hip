@@ -35,7 +35,7 @@ int check_level(uint32_t rep, const pg_level_desc &l, uint32_t index) {
     if (rep == PG_REP_KV64) {
         if (l.scheme != PG_SCHEME_BUCKET && l.scheme != PG_SCHEME_SINGLE_SLOT) return PG_ERR_UNSUPPORTED;
         if (l.scheme == PG_SCHEME_BUCKET && (l.bucket_size == 0 || l.bucket_size > 64)) return PG_ERR_INVALID;
-    } else if (rep == PG_REP_FP16_V16) {
+    } else if (rep == PG_REP_FP16_V16 || rep == PG_REP_FP8_V8 || rep == PG_REP_FP10_V6) {
         if (l.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT && l.scheme != PG_SCHEME_LINEAR_BUCKETS)
             return PG_ERR_UNSUPPORTED;
         if (l.bucket_size == 0 || l.bucket_size > 64) return PG_ERR_INVALID;
@@ -48,7 +48,9 @@ int check_level(uint32_t rep, const pg_level_desc &l, uint32_t index) {
     return PG_OK;
 }
 
-uint32_t slot_bytes(uint32_t rep) { return rep == PG_REP_KV64 ? 16u : 4u; }
+uint32_t slot_bytes(uint32_t rep) {
+    return rep == PG_REP_KV64 ? 16u : (rep == PG_REP_FP16_V16 ? 4u : 2u);
+}
 
 }  // namespace
 

@@ -168,6 +168,44 @@ __host__ __device__ inline uint32_t fp16_of(uint64_t key) {
 }
 constexpr uint32_t kTomb16 = 0x0000FFFFu;  // join(tombstone fp 0xFFFF, val 0)
 
+// Packed filter slot formats: one W-bit word = (val & vmask) << kbits |
+// (fp & kmask) (grouped_storage_sub_bits.cuh:77-95 join, :35-74 retrieve).
+// The fingerprint is the SmallKey-wide remap of the raw key's low bits
+// (dynamic_container.cuh:35-50): 16 bits for the 16/16 and 10/6 slots, 8 bits
+// for 8/8.  The probe tag is that remapped SmallKey; only the stored and
+// compared key bits are masked to kbits (grouped_storage_sub_bits.cuh:85-87,
+// :186-195), so a 10/6 key whose masked bits are 0 or 0x3FF is stored as an
+// empty- or tombstone-looking word, exactly as the reference stores it.
+// Empty word 0; tombstone word = kmask (join(get_empty()-1, 0)).
+struct Fmt16 {  // bit_grouped_container<16,16>, SmallKey uint16, 32-bit word
+    using W = uint32_t;
+    using V = uint16_t;  // value type of the bulk value buffers
+    static constexpr uint32_t small_bits = 16, kbits = 16, kmask = 0xFFFFu, vmask = 0xFFFFu;
+};
+struct Fmt8 {  // bit_grouped_container<8,8>, SmallKey uint8, 16-bit word
+    using W = uint16_t;
+    using V = uint8_t;
+    static constexpr uint32_t small_bits = 8, kbits = 8, kmask = 0xFFu, vmask = 0xFFu;
+};
+struct Fmt10 {  // bit_grouped_container<10,6>, SmallKey uint16, 16-bit word
+    using W = uint16_t;
+    using V = uint8_t;
+    static constexpr uint32_t small_bits = 16, kbits = 10, kmask = 0x3FFu, vmask = 0x3Fu;
+};
+
+template <class F>
+__host__ __device__ inline uint32_t fp_of(uint64_t key) {
+    constexpr uint32_t all = (1u << F::small_bits) - 1u;
+    uint32_t f = (uint32_t)key & all;
+    f += (f == 0u);
+    f -= (f == all);
+    return f;
+}
+template <class F>
+__host__ __device__ inline uint32_t pack_word(uint32_t fp, uint32_t val) {
+    return ((val & F::vmask) << F::kbits) | (fp & F::kmask);
+}
+
 // ---- launchers (defined in the .hip files) ----
 struct OpArgs {
     const uint64_t *keys;

@@ -227,9 +227,11 @@ int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_
     for (uint32_t i = 0; i < ntables; i++) {
         uint32_t tile = 0;
         const pg::TableDev *T = pg::table_dev(tables[i], &tile);
-        if (!T || (i > 0 && tile != s->P) || tile > 16) {
+        // one tile width (<= 16) for every served table; KV64 and 16/16 filters
+        if (!T || (i > 0 && tile != s->P) || tile > 16 ||
+            (T->rep != PG_REP_KV64 && T->rep != PG_REP_FP16_V16)) {
             delete s;
-            return PG_ERR_UNSUPPORTED;  // one tile width (<= 16) for every served table
+            return PG_ERR_UNSUPPORTED;
         }
         s->P = tile;
         s->h_tables[i] = *T;

@@ -23,6 +23,36 @@ __device__ inline bool w_empty(uint32_t w) { return (w & 0xFFFFu) == 0u; }
 __device__ inline bool w_tomb(uint32_t w) { return (w & 0xFFFFu) == 0xFFFFu; }
 __device__ inline bool w_filled(uint32_t w) { return !(w_empty(w) || w_tomb(w)); }
 
+// The same predicates for any slot format (the key bits are the low kbits).
+template <class F>
+__device__ inline bool f_empty(uint32_t w) { return (w & F::kmask) == 0u; }
+template <class F>
+__device__ inline bool f_tomb(uint32_t w) { return (w & F::kmask) == F::kmask; }
+template <class F>
+__device__ inline bool f_filled(uint32_t w) { return !(f_empty<F>(w) || f_tomb<F>(w)); }
+
+// Compare-and-swap of one slot word.  16-bit words are swapped through the
+// aligned 32-bit word that holds them (global atomics are 32/64-bit): the
+// neighbour half is carried over unchanged, and the loop only repeats while
+// that neighbour changes under us, so it ends as soon as either our half
+// differs from `expect` or the swap lands.  Slot arrays are 256 B aligned and
+// padded to 256 B, so the containing word is always inside the allocation.
+__device__ inline bool cas_word(uint32_t *p, uint32_t expect, uint32_t desired) {
+    return atomicCAS(p, expect, desired) == expect;
+}
+__device__ inline bool cas_word(uint16_t *p, uint32_t expect, uint32_t desired) {
+    uint32_t *w32 = (uint32_t *)((uintptr_t)p & ~uintptr_t(3));
+    const uint32_t sh = ((uintptr_t)p & 2u) ? 16u : 0u;
+    uint32_t old = __hip_atomic_load(w32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        if (((old >> sh) & 0xFFFFu) != (expect & 0xFFFFu)) return false;
+        const uint32_t nw = (old & ~(0xFFFFu << sh)) | ((desired & 0xFFFFu) << sh);
+        const uint32_t prev = atomicCAS(w32, old, nw);
+        if (prev == old) return true;
+        old = prev;
+    }
+}
+
 template <int V>
 __device__ inline uint32_t select_word(const uint32_t (&w)[V], int v) {
     uint32_t x = w[0];
@@ -162,32 +192,35 @@ struct Bucket {
 // pass (the reference reads the bucket once for its fill and again for the
 // claim).  Used by the backing level and by shapes without a fast path.
 // ---------------------------------------------------------------------------
-template <int P>
-__device__ int gen_fill(const uint32_t *base, uint32_t B, const Tile<P> &t) {
+template <int P, class F = Fmt16>
+__device__ int gen_fill(const typename F::W *base, uint32_t B, const Tile<P> &t) {
     int f = 0;
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
-        const uint32_t w = ok ? base[i] : 0u;
-        f += t.popc(ok && w_filled(w));
+        const uint32_t w = ok ? (uint32_t)base[i] : 0u;
+        f += t.popc(ok && f_filled<F>(w));
     }
     return f;
 }
 
-template <int P>
-__device__ bool gen_claim(uint32_t *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t) {
+// Claim the lowest empty slot (with `wd`, the lowest empty-or-tombstone one)
+// by a CAS from the empty word (or from the tombstone word).
+// packed_bucket.cuh:64-180, grouped_storage_sub_bits.cuh:127-157.
+template <int P, class F = Fmt16>
+__device__ bool gen_claim(typename F::W *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t) {
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
-        const uint32_t w = ok ? base[i] : 0u;
-        bool cand = ok && (w_empty(w) || (wd && w_tomb(w)));
+        const uint32_t w = ok ? (uint32_t)base[i] : 0u;
+        bool cand = ok && (f_empty<F>(w) || (wd && f_tomb<F>(w)));
         uint64_t lanes = t.bits(cand);
         while (lanes) {
             const uint32_t l = (uint32_t)ctz64(lanes);
             bool success = false;
             if (t.rank == l) {
-                const uint32_t expect = (wd && w_tomb(w)) ? kTomb16 : 0u;
-                success = atomicCAS(base + i, expect, word) == expect;
+                const uint32_t expect = (wd && f_tomb<F>(w)) ? F::kmask : 0u;
+                success = cas_word(base + i, expect, word);
                 cand = false;
             }
             if (t.any(success)) {
@@ -200,62 +233,69 @@ __device__ bool gen_claim(uint32_t *base, uint32_t B, uint32_t word, bool wd, co
     return false;
 }
 
-// Lowest match; when none, *fill receives the bucket's fill (same pass).
-template <int P>
-__device__ bool gen_find(const uint32_t *base, uint32_t B, uint32_t fp, const Tile<P> &t,
+// Lowest slot whose key bits equal fp's (fp & kmask); when none, *fill
+// receives the bucket's fill (same pass).  packed_bucket.cuh:182-219.
+template <int P, class F = Fmt16>
+__device__ bool gen_find(const typename F::W *base, uint32_t B, uint32_t fp, const Tile<P> &t,
                          uint32_t &val, bool &owner, int &fill) {
     fill = 0;
-    if ((B & 3u) == 0 && B <= 4u * P) {
-        // The whole bucket in one row: lane r holds words 4r..4r+3, read with
-        // one 16 B load (B % 4 == 0 keeps every bucket 16 B aligned).
-        const bool ok = 4u * t.rank < B;
-        uint32_t w[4] = {0u, 0u, 0u, 0u};
-        if (ok) load_words<4>(base + 4 * t.rank, w);
-        uint32_t m = 0;
+    if constexpr (F::kbits == 16) {
+        if ((B & 3u) == 0 && B <= 4u * P) {
+            // The whole bucket in one row: lane r holds words 4r..4r+3, read
+            // with one 16 B load (B % 4 == 0 keeps every bucket 16 B aligned).
+            const bool ok = 4u * t.rank < B;
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            if (ok) load_words<4>(base + 4 * t.rank, w);
+            uint32_t m = 0;
 #pragma unroll
-        for (int v = 0; v < 4; v++)
-            if (ok && (w[v] & 0xFFFFu) == fp) m |= 1u << v;
-        const uint64_t lanes = t.bits(m != 0);
-        if (lanes) {
-            const uint32_t l = (uint32_t)ctz64(lanes);
-            owner = (t.rank == l);
-            if (owner) val = select_word<4>(w, __builtin_ctz(m)) >> 16;
-            return true;
+            for (int v = 0; v < 4; v++)
+                if (ok && (w[v] & 0xFFFFu) == fp) m |= 1u << v;
+            const uint64_t lanes = t.bits(m != 0);
+            if (lanes) {
+                const uint32_t l = (uint32_t)ctz64(lanes);
+                owner = (t.rank == l);
+                if (owner) val = select_word<4>(w, __builtin_ctz(m)) >> 16;
+                return true;
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) fill += t.popc(ok && w_filled(w[v]));
+            return false;
         }
-#pragma unroll
-        for (int v = 0; v < 4; v++) fill += t.popc(ok && w_filled(w[v]));
-        return false;
     }
+    const uint32_t fpm = fp & F::kmask;
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
-        const uint32_t w = ok ? base[i] : 0u;
-        const uint64_t lanes = t.bits(ok && (w & 0xFFFFu) == fp);
+        const uint32_t w = ok ? (uint32_t)base[i] : 0u;
+        const uint64_t lanes = t.bits(ok && (w & F::kmask) == fpm);
         if (lanes) {
             const uint32_t l = (uint32_t)ctz64(lanes);
             owner = (t.rank == l);
-            if (owner) val = w >> 16;
+            if (owner) val = (w >> F::kbits) & F::vmask;
             return true;
         }
-        fill += t.popc(ok && w_filled(w));
+        fill += t.popc(ok && f_filled<F>(w));
     }
     return false;
 }
 
-template <int P>
-__device__ bool gen_remove(uint32_t *base, uint32_t B, uint32_t fp, const Tile<P> &t, int &fill) {
+// Tombstone the lowest matching slot: CAS from the word as read to the
+// tombstone word.  packed_bucket.cuh:227-271, grouped_storage_sub_bits.cuh:217-234.
+template <int P, class F = Fmt16>
+__device__ bool gen_remove(typename F::W *base, uint32_t B, uint32_t fp, const Tile<P> &t, int &fill) {
     fill = 0;
+    const uint32_t fpm = fp & F::kmask;
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
-        const uint32_t w = ok ? base[i] : 0u;
-        bool m = ok && (w & 0xFFFFu) == fp;
+        const uint32_t w = ok ? (uint32_t)base[i] : 0u;
+        bool m = ok && (w & F::kmask) == fpm;
         uint64_t lanes = t.bits(m);
         while (lanes) {
             const uint32_t l = (uint32_t)ctz64(lanes);
             bool success = false;
             if (t.rank == l) {
-                success = atomicCAS(base + i, w, kTomb16) == w;
+                success = cas_word(base + i, w, F::kmask);
                 m = false;
             }
             if (t.any(success)) {
@@ -264,7 +304,7 @@ __device__ bool gen_remove(uint32_t *base, uint32_t B, uint32_t fp, const Tile<P
             }
             lanes = t.bits(m);
         }
-        fill += t.popc(ok && w_filled(w));
+        fill += t.popc(ok && f_filled<F>(w));
     }
     return false;
 }
@@ -275,12 +315,12 @@ __device__ bool gen_remove(uint32_t *base, uint32_t B, uint32_t fp, const Tile<P
 
 // power_of_n_shortcut_buckets.cuh:198-306 (insert) and :309-417
 // (insert_with_delete), runtime B.
-template <int P>
+template <int P, class F = Fmt16>
 __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word,
                                     bool wd, const Tile<P> &t, int ai) {
     Probe pr;
     pr.begin(L, key, fp);
-    uint32_t *slots = (uint32_t *)L.slots;
+    typename F::W *slots = (typename F::W *)L.slots;
     const int B = (int)L.B;
     uint64_t bidx[kMaxShortcutProbes];
     int fill[kMaxShortcutProbes];
@@ -292,9 +332,9 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
         const uint64_t b = fastmod(p, L.nb);
         bidx[j] = b;
         t.acc[ai]++;
-        int f = gen_fill<P>(slots + b * L.B, L.B, t);
+        int f = gen_fill<P, F>(slots + b * L.B, L.B, t);
         if (4 * f < 3 * B) {
-            if (gen_claim<P>(slots + b * L.B, L.B, word, wd, t)) return true;
+            if (gen_claim<P, F>(slots + b * L.B, L.B, word, wd, t)) return true;
             f = B;
         }
         fill[j] = f;
@@ -307,7 +347,7 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
 #pragma unroll
         for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
             if (j < np) {
-                if (fill[j] == lvl && gen_claim<P>(slots + bidx[j] * L.B, L.B, word, wd, t)) return true;
+                if (fill[j] == lvl && gen_claim<P, F>(slots + bidx[j] * L.B, L.B, word, wd, t)) return true;
                 if (fill[j] > lvl && fill[j] < next) next = fill[j];
             }
         }
@@ -317,38 +357,38 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
 }
 
 // linear_insert_buckets.cuh:200-244: first probe bucket with a free slot.
-template <int P>
+template <int P, class F = Fmt16>
 __device__ bool gen_linear_insert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word,
                                   bool wd, const Tile<P> &t, int ai) {
     Probe pr;
     pr.begin(L, key, fp);
     pr.max_probes = pr.useful_probes(L.nb.d);
-    uint32_t *slots = (uint32_t *)L.slots;
+    typename F::W *slots = (typename F::W *)L.slots;
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
         if (p == kEnd) return false;
         const uint64_t b = fastmod(p, L.nb);
         t.acc[ai]++;
-        if (gen_claim<P>(slots + b * L.B, L.B, word, wd, t)) return true;
+        if (gen_claim<P, F>(slots + b * L.B, L.B, word, wd, t)) return true;
     }
 }
 
-template <int P>
+template <int P, class F = Fmt16>
 __device__ bool gen_level_insert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word, bool wd,
                                  const Tile<P> &t, int ai) {
-    if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT) return gen_shortcut_insert<P>(L, key, fp, word, wd, t, ai);
-    return gen_linear_insert<P>(L, key, fp, word, wd, t, ai);
+    if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT) return gen_shortcut_insert<P, F>(L, key, fp, word, wd, t, ai);
+    return gen_linear_insert<P, F>(L, key, fp, word, wd, t, ai);
 }
 
 // Shortcut query (power_of_n_shortcut_buckets.cuh:676-707): every probe
 // bucket, first match wins.  Linear query (linear_insert_buckets.cuh:362-398):
 // stop at the first probe bucket whose fill != B.
-template <int P>
+template <int P, class F = Fmt16>
 __device__ bool gen_level_query(const LevelDev &L, uint64_t key, uint32_t fp, const Tile<P> &t,
                                 uint32_t &val, bool &owner, int ai) {
     Probe pr;
     pr.begin(L, key, fp);
-    const uint32_t *slots = (const uint32_t *)L.slots;
+    const typename F::W *slots = (const typename F::W *)L.slots;
     const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
     if (linear) pr.max_probes = pr.useful_probes(L.nb.d);
     for (uint32_t j = 0;; j++) {
@@ -357,18 +397,18 @@ __device__ bool gen_level_query(const LevelDev &L, uint64_t key, uint32_t fp, co
         const uint64_t b = fastmod(p, L.nb);
         int f;
         t.acc[ai]++;
-        if (gen_find<P>(slots + b * L.B, L.B, fp, t, val, owner, f)) return true;
+        if (gen_find<P, F>(slots + b * L.B, L.B, fp, t, val, owner, f)) return true;
         if (linear && f != (int)L.B) return false;
     }
 }
 
 // Shortcut remove (power_of_n_shortcut_buckets.cuh:709-740); linear remove
 // (linear_insert_buckets.cuh:323-360) stops at the first bucket with fill != B.
-template <int P>
+template <int P, class F = Fmt16>
 __device__ bool gen_level_remove(const LevelDev &L, uint64_t key, uint32_t fp, const Tile<P> &t, int ai) {
     Probe pr;
     pr.begin(L, key, fp);
-    uint32_t *slots = (uint32_t *)L.slots;
+    typename F::W *slots = (typename F::W *)L.slots;
     const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
     if (linear) pr.max_probes = pr.useful_probes(L.nb.d);
     for (uint32_t j = 0;; j++) {
@@ -377,7 +417,7 @@ __device__ bool gen_level_remove(const LevelDev &L, uint64_t key, uint32_t fp, c
         const uint64_t b = fastmod(p, L.nb);
         int f;
         t.acc[ai]++;
-        if (gen_remove<P>(slots + b * L.B, L.B, fp, t, f)) return true;
+        if (gen_remove<P, F>(slots + b * L.B, L.B, fp, t, f)) return true;
         if (linear && f != (int)L.B) return false;
     }
 }
@@ -484,12 +524,14 @@ __device__ int tcf_insert_prepared(const TableDev &T, uint64_t key, uint32_t fp,
     return 0;
 }
 
-// Whole-table insert of one key; B == 0 selects the generic level-0 path.
-template <int P, int B>
+// Whole-table insert of one key; B == 0 selects the generic level-0 path (the
+// only one for the 8/8 and 10/6 formats).
+template <int P, int B, class F = Fmt16>
 __device__ int tcf_insert_key(const TableDev &T, uint64_t key, uint32_t val, bool wd, const Tile<P> &t) {
-    const uint32_t fp = fp16_of(key);
-    const uint32_t word = ((val & 0xFFFFu) << 16) | fp;
+    const uint32_t fp = fp_of<F>(key);
+    const uint32_t word = pack_word<F>(fp, val);
     if constexpr (B != 0) {
+        static_assert(F::kbits == 16, "register-bucket path is for 16/16 slots");
         const LevelDev &L = T.lv[0];
         Probe pr;
         pr.begin(L, key, fp);
@@ -499,9 +541,9 @@ __device__ int tcf_insert_key(const TableDev &T, uint64_t key, uint32_t val, boo
         if (p0 != kEnd) bk0.load((const uint32_t *)L.slots, b0, t);
         return tcf_insert_prepared<P, B>(T, key, fp, word, wd, t, pr, bk0, b0);
     } else {
-        if (gen_level_insert<P>(T.lv[0], key, fp, word, wd, t, 0)) return 1;
+        if (gen_level_insert<P, F>(T.lv[0], key, fp, word, wd, t, 0)) return 1;
         for (uint32_t l = 1; l < T.nlevels; l++)
-            if (gen_level_insert<P>(T.lv[l], key, fp, word, wd, t, 1)) return 1 + (int)l;
+            if (gen_level_insert<P, F>(T.lv[l], key, fp, word, wd, t, 1)) return 1 + (int)l;
         return 0;
     }
 }
@@ -515,39 +557,40 @@ __device__ int tcf_insert_key(const TableDev &T, uint64_t key, uint32_t val, boo
 // is returned (and removed for _delete), else a free slot takes the key
 // (plain insert in both forms).
 // ---------------------------------------------------------------------------
-template <int P>
+template <int P, class F = Fmt16>
 __device__ bool gen_level_upsert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word, bool del,
                                  const Tile<P> &t, int ai, uint32_t &val, bool &owner, bool &found) {
     Probe pr;
     pr.begin(L, key, fp);
-    uint32_t *slots = (uint32_t *)L.slots;
+    typename F::W *slots = (typename F::W *)L.slots;
     const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
     if (linear) pr.max_probes = pr.useful_probes(L.nb.d);
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
         if (p == kEnd) break;
-        uint32_t *base = slots + fastmod(p, L.nb) * L.B;
+        typename F::W *base = slots + fastmod(p, L.nb) * L.B;
         int f;
         t.acc[ai]++;
-        if (gen_find<P>(base, L.B, fp, t, val, owner, f)) {
+        if (gen_find<P, F>(base, L.B, fp, t, val, owner, f)) {
             found = true;
-            if (del) gen_remove<P>(base, L.B, fp, t, f);
+            if (del) gen_remove<P, F>(base, L.B, fp, t, f);
             return true;
         }
-        if (linear && gen_claim<P>(base, L.B, word, false, t)) return true;
+        if (linear && gen_claim<P, F>(base, L.B, word, false, t)) return true;
     }
     if (linear) return false;
-    return gen_shortcut_insert<P>(L, key, fp, word, del, t, ai);
+    return gen_shortcut_insert<P, F>(L, key, fp, word, del, t, ai);
 }
 
-template <int P, int B>
+template <int P, int B, class F = Fmt16>
 __device__ int tcf_upsert_key(const TableDev &T, uint64_t key, uint32_t v_in, bool del, const Tile<P> &t,
                               uint32_t &val, bool &owner, bool &found) {
-    const uint32_t fp = fp16_of(key);
-    const uint32_t word = ((v_in & 0xFFFFu) << 16) | fp;
+    const uint32_t fp = fp_of<F>(key);
+    const uint32_t word = pack_word<F>(fp, v_in);
     found = false;
     bool ok;
     if constexpr (B != 0) {
+        static_assert(F::kbits == 16, "register-bucket path is for 16/16 slots");
         const LevelDev &L = T.lv[0];
         uint32_t *slots = (uint32_t *)L.slots;
         Probe pr;
@@ -578,23 +621,23 @@ __device__ int tcf_upsert_key(const TableDev &T, uint64_t key, uint32_t v_in, bo
         }
         if (!ok) ok = fast_shortcut_insert<P, B>(L, pr, word, del, t, bk0, b0);
     } else {
-        ok = gen_level_upsert<P>(T.lv[0], key, fp, word, del, t, 0, val, owner, found);
+        ok = gen_level_upsert<P, F>(T.lv[0], key, fp, word, del, t, 0, val, owner, found);
     }
     if (ok) return 1;
     for (uint32_t l = 1; l < T.nlevels; l++)
-        if (gen_level_upsert<P>(T.lv[l], key, fp, word, del, t, 1, val, owner, found)) return 1 + (int)l;
+        if (gen_level_upsert<P, F>(T.lv[l], key, fp, word, del, t, 1, val, owner, found)) return 1 + (int)l;
     return 0;
 }
 
-template <int P, int B>
+template <int P, int B, class F = Fmt16>
 __device__ bool tcf_remove_key(const TableDev &T, uint64_t key, const Tile<P> &t) {
-    const uint32_t fp = fp16_of(key);
+    const uint32_t fp = fp_of<F>(key);
     bool ok;
     if constexpr (B != 0) ok = fast_shortcut_remove<P, B>(T.lv[0], key, fp, t);
-    else ok = gen_level_remove<P>(T.lv[0], key, fp, t, 0);
+    else ok = gen_level_remove<P, F>(T.lv[0], key, fp, t, 0);
     if (ok) return true;
     for (uint32_t l = 1; l < T.nlevels; l++)
-        if (gen_level_remove<P>(T.lv[l], key, fp, t, 1)) return true;
+        if (gen_level_remove<P, F>(T.lv[l], key, fp, t, 1)) return true;
     return false;
 }
 
@@ -605,7 +648,7 @@ __device__ bool tcf_remove_key(const TableDev &T, uint64_t key, const Tile<P> &t
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
 
-template <int P, int B, int I, bool ACC>
+template <int P, int B, int I, bool ACC, class F = Fmt16>
 __global__ __launch_bounds__(kBlock) void k_tcf_insert(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tile_id = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
@@ -626,7 +669,7 @@ __global__ __launch_bounds__(kBlock) void k_tcf_insert(TableDev T, OpArgs a) {
             live[i] = idx[i] < a.n;
             key[i] = live[i] ? a.keys[idx[i]] : 0;
             const uint32_t val = (live[i] && a.vals) ? ((const uint16_t *)a.vals)[idx[i]] : 0u;
-            fp[i] = fp16_of(key[i]);
+            fp[i] = fp16_of(key[i]);  // B != 0: 16/16 slots only
             word[i] = (val << 16) | fp[i];
             pr[i].begin(L, key[i], fp[i]);
             const uint64_t p0 = pr[i].at(0);
@@ -645,8 +688,8 @@ __global__ __launch_bounds__(kBlock) void k_tcf_insert(TableDev T, OpArgs a) {
             live[i] = idx[i] < a.n;
             st[i] = 1;
             if (live[i]) {
-                const uint32_t val = a.vals ? ((const uint16_t *)a.vals)[idx[i]] : 0u;
-                st[i] = tcf_insert_key<P, 0>(T, a.keys[idx[i]], val, wd, t);
+                const uint32_t val = a.vals ? ((const typename F::V *)a.vals)[idx[i]] : 0u;
+                st[i] = tcf_insert_key<P, 0, F>(T, a.keys[idx[i]], val, wd, t);
             }
         }
     }
@@ -658,49 +701,49 @@ __global__ __launch_bounds__(kBlock) void k_tcf_insert(TableDev T, OpArgs a) {
     if constexpr (ACC) flush_acc(a.stats, t, live[0]);
 }
 
-template <int P, int B, bool ACC>
+template <int P, int B, bool ACC, class F = Fmt16>
 __global__ __launch_bounds__(kBlock) void k_tcf_remove(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
     if (tid < a.n) {
-        const bool ok = tcf_remove_key<P, B>(T, a.keys[tid], t);
+        const bool ok = tcf_remove_key<P, B, F>(T, a.keys[tid], t);
         if (a.flags && t.leader()) a.flags[tid] = ok ? 1 : 0;
     }
     if constexpr (ACC) flush_acc(a.stats, t, tid < a.n);
 }
 
-template <int P, int B>
+template <int P, int B, class F = Fmt16>
 __device__ inline void upsert_one(const TableDev &T, const OpArgs &a, uint64_t i, const Tile<P> &t,
                                   uint64_t &fails) {
-    const uint32_t v_in = a.vals ? ((const uint16_t *)a.vals)[i] : 0u;
+    const uint32_t v_in = a.vals ? ((const typename F::V *)a.vals)[i] : 0u;
     uint32_t val = 0;
     bool owner = false, found = false;
-    const int st = tcf_upsert_key<P, B>(T, a.keys[i], v_in, a.with_delete != 0, t, val, owner, found);
+    const int st = tcf_upsert_key<P, B, F>(T, a.keys[i], v_in, a.with_delete != 0, t, val, owner, found);
     if (t.leader()) {
         if (a.flags) a.flags[i] = (uint8_t)st;
         if (a.found) a.found[i] = found ? 1 : 0;
-        if (a.vals_out && !found) ((uint16_t *)a.vals_out)[i] = 0;
+        if (a.vals_out && !found) ((typename F::V *)a.vals_out)[i] = 0;
     }
-    if (found && owner && a.vals_out) ((uint16_t *)a.vals_out)[i] = (uint16_t)val;
+    if (found && owner && a.vals_out) ((typename F::V *)a.vals_out)[i] = (typename F::V)val;
     fails += (st == 0);
 }
 
-template <int P, int B, bool ACC>
+template <int P, int B, bool ACC, class F = Fmt16>
 __global__ __launch_bounds__(kBlock) void k_tcf_upsert(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
     uint64_t fails = 0;
-    if (tid < a.n) upsert_one<P, B>(T, a, tid, t, fails);
+    if (tid < a.n) upsert_one<P, B, F>(T, a, tid, t, fails);
     wave_count(a.fail_count, tid < a.n && t.leader() && fails);
     if constexpr (ACC) flush_acc(a.stats, t, tid < a.n);
 }
 
-template <int P, int B>
+template <int P, int B, class F = Fmt16>
 __global__ __launch_bounds__(64) void k_tcf_upsert_seq(TableDev T, OpArgs a) {
     Tile<P> t;
     uint64_t fails = 0;
     for (uint64_t i = 0; i < a.n; i++) {
-        upsert_one<P, B>(T, a, i, t, fails);
+        upsert_one<P, B, F>(T, a, i, t, fails);
         fence_agent();
     }
     if (a.fail_count && t.leader() && fails) atomicAdd((unsigned long long *)a.fail_count, fails);
@@ -709,14 +752,14 @@ __global__ __launch_bounds__(64) void k_tcf_upsert_seq(TableDev T, OpArgs a) {
 
 // Deterministic mode: one tile applies the keys in array order.  The fence
 // after every key makes the next key's loads observe this key's CAS.
-template <int P, int B>
+template <int P, int B, class F = Fmt16>
 __global__ __launch_bounds__(64) void k_tcf_insert_seq(TableDev T, OpArgs a) {
     Tile<P> t;
     uint64_t fails = 0;
     for (uint64_t i = 0; i < a.n; i++) {
         const uint64_t key = a.keys[i];
-        const uint32_t val = a.vals ? ((const uint16_t *)a.vals)[i] : 0u;
-        const int st = tcf_insert_key<P, B>(T, key, val, a.with_delete != 0, t);
+        const uint32_t val = a.vals ? ((const typename F::V *)a.vals)[i] : 0u;
+        const int st = tcf_insert_key<P, B, F>(T, key, val, a.with_delete != 0, t);
         if (a.flags && t.leader()) a.flags[i] = (uint8_t)st;
         fails += (st == 0);
         fence_agent();
@@ -725,11 +768,11 @@ __global__ __launch_bounds__(64) void k_tcf_insert_seq(TableDev T, OpArgs a) {
     flush_acc(a.stats, t, true);
 }
 
-template <int P, int B>
+template <int P, int B, class F = Fmt16>
 __global__ __launch_bounds__(64) void k_tcf_remove_seq(TableDev T, OpArgs a) {
     Tile<P> t;
     for (uint64_t i = 0; i < a.n; i++) {
-        const bool ok = tcf_remove_key<P, B>(T, a.keys[i], t);
+        const bool ok = tcf_remove_key<P, B, F>(T, a.keys[i], t);
         if (a.flags && t.leader()) a.flags[i] = ok ? 1 : 0;
         fence_agent();
     }
@@ -740,7 +783,7 @@ __global__ __launch_bounds__(64) void k_tcf_remove_seq(TableDev T, OpArgs a) {
 // bucketed_table.cuh:332-349).  The level-0 bucket loads of all I keys are
 // issued before any is examined, so a wave keeps 64/P * I random 64 B reads
 // in flight.
-template <int P, int B, int I, bool ACC>
+template <int P, int B, int I, bool ACC, class F = Fmt16>
 __global__ __launch_bounds__(kBlock) void k_tcf_query(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tile_id = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
@@ -757,7 +800,7 @@ __global__ __launch_bounds__(kBlock) void k_tcf_query(TableDev T, OpArgs a) {
         idx[i] = tile_id + (uint64_t)i * tiles;
         live[i] = idx[i] < a.n;
         key[i] = live[i] ? a.keys[idx[i]] : 0;
-        fp[i] = fp16_of(key[i]);
+        fp[i] = fp_of<F>(key[i]);
         pr[i].begin(L, key[i], fp[i]);
         pend[i] = live[i];
         found[i] = false;
@@ -789,19 +832,19 @@ __global__ __launch_bounds__(kBlock) void k_tcf_query(TableDev T, OpArgs a) {
     } else {
 #pragma unroll
         for (int i = 0; i < I; i++)
-            if (live[i]) found[i] = gen_level_query<P>(L, key[i], fp[i], t, val[i], own[i], 0);
+            if (live[i]) found[i] = gen_level_query<P, F>(L, key[i], fp[i], t, val[i], own[i], 0);
     }
 #pragma unroll
     for (int i = 0; i < I; i++) {
         if (live[i] && !found[i]) {
             for (uint32_t l = 1; l < T.nlevels && !found[i]; l++)
-                found[i] = gen_level_query<P>(T.lv[l], key[i], fp[i], t, val[i], own[i], 1);
+                found[i] = gen_level_query<P, F>(T.lv[l], key[i], fp[i], t, val[i], own[i], 1);
         }
         if (live[i]) {
             const bool writer = found[i] ? own[i] : t.leader();
             if (writer) {
                 if (a.flags) a.flags[idx[i]] = found[i] ? 1 : 0;
-                if (a.vals_out) ((uint16_t *)a.vals_out)[idx[i]] = found[i] ? (uint16_t)val[i] : 0;
+                if (a.vals_out) ((typename F::V *)a.vals_out)[idx[i]] = found[i] ? (typename F::V)val[i] : 0;
             }
         }
     }
@@ -816,38 +859,38 @@ inline unsigned tcf_grid(uint64_t tiles, int P) {
     return (unsigned)((threads + kBlock - 1) / kBlock);
 }
 
-template <int P, int B, int I, bool ACC>
+template <int P, int B, int I, bool ACC, class F = Fmt16>
 void launch_tcf_kernel(Op op, const TableDev &T, const OpArgs &a, hipStream_t s) {
     const uint64_t tiles = (a.n + I - 1) / I;
     if (op == Op::Query)
-        hipLaunchKernelGGL((k_tcf_query<P, B, I, ACC>), dim3(tcf_grid(tiles, P)), dim3(kBlock), 0, s, T, a);
+        hipLaunchKernelGGL((k_tcf_query<P, B, I, ACC, F>), dim3(tcf_grid(tiles, P)), dim3(kBlock), 0, s, T, a);
     else
-        hipLaunchKernelGGL((k_tcf_insert<P, B, I, ACC>), dim3(tcf_grid(tiles, P)), dim3(kBlock), 0, s, T, a);
+        hipLaunchKernelGGL((k_tcf_insert<P, B, I, ACC, F>), dim3(tcf_grid(tiles, P)), dim3(kBlock), 0, s, T, a);
 }
 
-template <int P, int B>
+template <int P, int B, class F = Fmt16>
 int launch_tcf_pb(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
     if (a.n == 0) return PG_OK;
     const bool acc = a.stats != nullptr;
     if (op != Op::Query && a.deterministic) {
-        if (op == Op::Insert) hipLaunchKernelGGL((k_tcf_insert_seq<P, B>), dim3(1), dim3(P), 0, s, T, a);
-        else if (op == Op::Upsert) hipLaunchKernelGGL((k_tcf_upsert_seq<P, B>), dim3(1), dim3(P), 0, s, T, a);
-        else hipLaunchKernelGGL((k_tcf_remove_seq<P, B>), dim3(1), dim3(P), 0, s, T, a);
+        if (op == Op::Insert) hipLaunchKernelGGL((k_tcf_insert_seq<P, B, F>), dim3(1), dim3(P), 0, s, T, a);
+        else if (op == Op::Upsert) hipLaunchKernelGGL((k_tcf_upsert_seq<P, B, F>), dim3(1), dim3(P), 0, s, T, a);
+        else hipLaunchKernelGGL((k_tcf_remove_seq<P, B, F>), dim3(1), dim3(P), 0, s, T, a);
     } else if (op == Op::Upsert) {
-        if (acc) hipLaunchKernelGGL((k_tcf_upsert<P, B, true>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
-        else hipLaunchKernelGGL((k_tcf_upsert<P, B, false>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
+        if (acc) hipLaunchKernelGGL((k_tcf_upsert<P, B, true, F>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
+        else hipLaunchKernelGGL((k_tcf_upsert<P, B, false, F>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
     } else if (op == Op::Remove) {
-        if (acc) hipLaunchKernelGGL((k_tcf_remove<P, B, true>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
-        else hipLaunchKernelGGL((k_tcf_remove<P, B, false>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
+        if (acc) hipLaunchKernelGGL((k_tcf_remove<P, B, true, F>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
+        else hipLaunchKernelGGL((k_tcf_remove<P, B, false, F>), dim3(tcf_grid(a.n, P)), dim3(kBlock), 0, s, T, a);
     } else if (acc) {
-        launch_tcf_kernel<P, B, 1, true>(op, T, a, s);
+        launch_tcf_kernel<P, B, 1, true, F>(op, T, a, s);
     } else if constexpr (B == 0) {
-        launch_tcf_kernel<P, B, 1, false>(op, T, a, s);
+        launch_tcf_kernel<P, B, 1, false, F>(op, T, a, s);
     } else {
         switch (a.items) {
-            case 2: launch_tcf_kernel<P, B, 2, false>(op, T, a, s); break;
-            case 4: launch_tcf_kernel<P, B, 4, false>(op, T, a, s); break;
-            default: launch_tcf_kernel<P, B, 1, false>(op, T, a, s); break;
+            case 2: launch_tcf_kernel<P, B, 2, false, F>(op, T, a, s); break;
+            case 4: launch_tcf_kernel<P, B, 4, false, F>(op, T, a, s); break;
+            default: launch_tcf_kernel<P, B, 1, false, F>(op, T, a, s); break;
         }
     }
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
@@ -856,6 +899,8 @@ int launch_tcf_pb(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
 template <int P>
 int launch_tcf_p(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
     const LevelDev &L = T.lv[0];
+    if (T.rep == PG_REP_FP8_V8) return launch_tcf_pb<P, 0, Fmt8>(T, op, a, s);
+    if (T.rep == PG_REP_FP10_V6) return launch_tcf_pb<P, 0, Fmt10>(T, op, a, s);
     if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT) {
         switch (L.B) {
             case 8: return launch_tcf_pb<P, 8>(T, op, a, s);

@@ -26,6 +26,19 @@ __global__ __launch_bounds__(kBlock) void k_fill_packed(const uint32_t *w, uint6
     if (__lane_id() == 0 && c) atomicAdd((unsigned long long *)counter, (unsigned long long)c);
 }
 
+// 16-bit words of the 8/8 and 10/6 formats: the key bits are the low
+// popcount(kmask) bits.
+__global__ __launch_bounds__(kBlock) void k_fill_packed16(const uint16_t *w, uint64_t n, uint32_t kmask,
+                                                           uint64_t *counter) {
+    uint64_t c = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
+        const uint32_t f = w[i] & kmask;
+        c += (f != 0u && f != kmask);
+    }
+    c = wave_sum(c);
+    if (__lane_id() == 0 && c) atomicAdd((unsigned long long *)counter, (unsigned long long)c);
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_kv(const ulonglong2 *s, uint64_t n, uint64_t *counter) {
     uint64_t c = 0;
     for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
@@ -51,6 +64,10 @@ int launch_fill(const TableDev &T, uint64_t *d_counter, hipStream_t s) {
         if (T.rep == PG_REP_KV64)
             hipLaunchKernelGGL(k_fill_kv, dim3(stride_grid(nslots)), dim3(kBlock), 0, s,
                                (const ulonglong2 *)L.slots, nslots, d_counter);
+        else if (T.rep == PG_REP_FP8_V8 || T.rep == PG_REP_FP10_V6)
+            hipLaunchKernelGGL(k_fill_packed16, dim3(stride_grid(nslots)), dim3(kBlock), 0, s,
+                               (const uint16_t *)L.slots, nslots, T.rep == PG_REP_FP8_V8 ? 0xFFu : 0x3FFu,
+                               d_counter);
         else
             hipLaunchKernelGGL(k_fill_packed, dim3(stride_grid(nslots)), dim3(kBlock), 0, s,
                                (const uint32_t *)L.slots, nslots, d_counter);

@@ -47,6 +47,11 @@ static int cpu_checks() {
     CHECK(d.level[0].prober == PG_PROBE_XOR_P2 && d.level[0].tile == 4 && d.level[0].bucket_size == 16);
     CHECK(d.level[1].nslots == 10 && d.level[1].scheme == PG_SCHEME_LINEAR_BUCKETS && d.level[1].max_probes == 4000);
     static_assert(tcf::levels == 2 && hash_table_64::levels == 1, "levels");
+    static_assert(small_tcf::rep == PG_REP_FP8_V8 && tcf_10_6::rep == PG_REP_FP10_V6, "packed formats");
+    size_in_num_slots<2> two_b(1000, 10);  // a sizing is consumed as levels take their sizes
+    tcf_10_6::describe(d, 0, &two_b);
+    CHECK(d.level[0].tile == 1 && d.level[0].bucket_size == 8);
+    CHECK(d.level[0].prober == PG_PROBE_DOUBLE && d.level[1].max_probes == 20);
     std::printf("cpu checks ok\n");
     return 0;
 }
@@ -88,6 +93,17 @@ static int roundtrip(const char *name, uint64_t nslots, uint64_t backing, uint64
     }
     CHECK(fn == 0);
     if (Table::rep == PG_REP_KV64) CHECK(wrong == 0);
+    const uint64_t slot_bytes = Table::rep == PG_REP_KV64 ? 16 : (Table::rep == PG_REP_FP16_V16 ? 4 : 2);
+    CHECK(t->host_bytes_in_use() == t->get_num_slots() * slot_bytes);
+    if (Table::rep == PG_REP_FP8_V8) {
+        // 8-bit fingerprints collide often: a remove may take another key's
+        // word, so the exact post-remove counts below do not apply.
+        for (void *p : {(void *)keys, (void *)absent, (void *)vals, (void *)out, (void *)found, (void *)fails})
+            CHECK(hipFree(p) == hipSuccess);
+        Table::free_on_device(t);
+        std::printf("%s ok (%llu keys)\n", name, (unsigned long long)n);
+        return 0;
+    }
     t->bulk_remove(keys, n / 2, found);
     CHECK(t->get_fill() == n - n / 2);
     t->bulk_query(keys, n, out, found);
@@ -99,7 +115,6 @@ static int roundtrip(const char *name, uint64_t nslots, uint64_t backing, uint64
         for (uint64_t i = 0; i < n / 2; i++) CHECK(hf[i] == 0);
     }
     CHECK(t->get_num_slots() >= nslots);
-    CHECK(t->host_bytes_in_use() == t->get_num_slots() * (Table::rep == PG_REP_KV64 ? 16 : 4));
     for (void *p : {(void *)keys, (void *)absent, (void *)vals, (void *)out, (void *)found, (void *)fails})
         CHECK(hipFree(p) == hipSuccess);
     Table::free_on_device(t);
@@ -111,6 +126,7 @@ static int gpu_checks() {
     if (roundtrip<tcf>("tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<fast_tcf>("fast_tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<hash_table_64>("hash_table_64", 1 << 20, 0, 900000)) return 1;
+    if (roundtrip<small_tcf>("small_tcf", 1 << 20, 1 << 14, 900000)) return 1;
     return 0;
 }
 

@@ -15,6 +15,10 @@ def dev_u16(a: np.ndarray) -> torch.Tensor:
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint16).view(np.int16)).cuda()
 
 
+def dev_u8(a: np.ndarray) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.uint8)).cuda()
+
+
 def host(t: torch.Tensor, dtype) -> np.ndarray:
     return t.cpu().numpy().view(dtype)
 
@@ -32,13 +36,27 @@ class Ops:
     def __init__(self, table: pg.Table):
         self.t = table
         self.kv = table.is_kv
+        # value buffers hold the format's Val type: uint64 / uint16 / uint8
+        self.narrow = table.desc.rep in (pg.REP_FP8_V8, pg.REP_FP10_V6)
+
+    def _dev_vals(self, vals):
+        if self.kv:
+            return dev_u64(vals)
+        return dev_u8(vals) if self.narrow else dev_u16(vals)
+
+    def _out_vals(self, n):
+        dt = torch.int64 if self.kv else (torch.uint8 if self.narrow else torch.int16)
+        return torch.zeros(n, dtype=dt, device="cuda")
+
+    def _host_vals(self, t):
+        return host(t, np.uint64 if self.kv else (np.uint8 if self.narrow else np.uint16))
 
     def insert(self, keys, vals=None, with_delete=False):
         n = len(keys)
         k = dev_u64(keys)
         v = None
         if vals is not None:
-            v = dev_u64(vals) if self.kv else dev_u16(vals)
+            v = self._dev_vals(vals)
         st = torch.zeros(n, dtype=torch.uint8, device="cuda")
         fails = torch.zeros(1, dtype=torch.int64, device="cuda")
         self.t.bulk_insert(k, v, st, fails, with_delete=with_delete)
@@ -49,21 +67,21 @@ class Ops:
         n = len(keys)
         k = dev_u64(keys)
         found = torch.zeros(n, dtype=torch.uint8, device="cuda")
-        vals = torch.zeros(n, dtype=torch.int64 if self.kv else torch.int16, device="cuda")
+        vals = self._out_vals(n)
         self.t.bulk_query(k, vals, found)
         torch.cuda.synchronize()
-        return found.cpu().numpy(), host(vals, np.uint64 if self.kv else np.uint16)
+        return found.cpu().numpy(), self._host_vals(vals)
 
     def upsert(self, keys, vals=None, delete=False):
         n = len(keys)
         k = dev_u64(keys)
-        v = None if vals is None else dev_u16(vals)
+        v = None if vals is None else self._dev_vals(vals)
         st = torch.zeros(n, dtype=torch.uint8, device="cuda")
         found = torch.zeros(n, dtype=torch.uint8, device="cuda")
-        out = torch.zeros(n, dtype=torch.int16, device="cuda")
+        out = self._out_vals(n)
         self.t.bulk_insert_if_not_exists(k, v, out, found, st, delete=delete)
         torch.cuda.synchronize()
-        return st.cpu().numpy(), found.cpu().numpy(), host(out, np.uint16)
+        return st.cpu().numpy(), found.cpu().numpy(), self._host_vals(out)
 
     def remove(self, keys):
         n = len(keys)
@@ -74,5 +92,6 @@ class Ops:
         return found.cpu().numpy()
 
     def level(self, lvl):
-        dt = np.uint64 if self.kv else np.uint32
+        dt = {pg.REP_KV64: np.uint64, pg.REP_FP16_V16: np.uint32, pg.REP_FP8_V8: np.uint16,
+              pg.REP_FP10_V6: np.uint16}[self.t.desc.rep]
         return np.frombuffer(self.t.level_bytes(lvl), dtype=dt)
