@@ -64,11 +64,27 @@ enum pg_status {
  *             whose low 10 bits are 0 (or 0x3FF) stores as an empty-looking
  *             (or tombstone-looking) word (tests/src/test_mhm_tcf.cu:105-116).
  *             The probe tag (XOR_P2) is the unmasked fp16.
+ *   FP12      twelve_bucket<uint64_t, uint16_t, uint16_t, P, B> through
+ *             wrapper_half_bucket<uint16_t> (representations/12_bit_bucket.cuh:
+ *             33-342): key-only, four slots in three uint16 words, slot 4g+o
+ *             keeping its own 12 bits of the key's low 16 bits
+ *             (representation_helpers.cuh:600-903); a bucket is
+ *             2*((12B-1)/16+1) bytes.  Insert and query only: the
+ *             reference's remove and insert_delete for this bucket do not
+ *             compile, so remove, insert_with_delete and the upserts return
+ *             PG_ERR_UNSUPPORTED; queries report value 0 (uint16 buffers).
  *   Value buffers hold the reference's Val type: uint64 (KV64), uint16
- *   (FP16_V16), uint8 (FP8_V8 and FP10_V6, which keep the low 8 / 6 bits).
- *   The 8/8 and 10/6 formats run the generic (runtime bucket size) kernels;
- *   batches and the server take KV64 and FP16_V16 tables only.            */
-enum pg_rep { PG_REP_KV64 = 0, PG_REP_FP16_V16 = 1, PG_REP_FP8_V8 = 2, PG_REP_FP10_V6 = 3 };
+ *   (FP16_V16, FP12), uint8 (FP8_V8 and FP10_V6, which keep the low 8 / 6
+ *   bits).  The 8/8, 10/6 and 12-bit formats and grouped levels run the
+ *   generic (runtime bucket size) kernels; batches and the server take KV64
+ *   and FP16_V16 tables only.                                             */
+enum pg_rep {
+    PG_REP_KV64 = 0,
+    PG_REP_FP16_V16 = 1,
+    PG_REP_FP8_V8 = 2,
+    PG_REP_FP10_V6 = 3,
+    PG_REP_FP12 = 4
+};
 
 /* Insert scheme of a level.
  *   BUCKET        insert_schemes/bucket_insert.cuh:38-361
@@ -76,12 +92,19 @@ enum pg_rep { PG_REP_KV64 = 0, PG_REP_FP16_V16 = 1, PG_REP_FP8_V8 = 2, PG_REP_FP
  *   POWER_OF_N_SHORTCUT  insert_schemes/power_of_n_shortcut_buckets.cuh:35-783
  *                 (the TCF primary: two choices + 75 % shortcut)
  *   LINEAR_BUCKETS insert_schemes/linear_insert_buckets.cuh:35-445
- *                 (the TCF backing table)                                */
+ *                 (the TCF backing table)
+ *   GROUPED_SHORTCUT  blocked_bucket_insert<K> =
+ *                 power_of_n_insert_grouped_buckets
+ *                 (insert_schemes/grouped_power_buckets.cuh:46-388): the
+ *                 shortcut scheme over buckets packed `group` = K to a
+ *                 128-byte line (bucket b at byte 128*(b/K) + size*(b%K)),
+ *                 packed filter slots only                               */
 enum pg_scheme {
     PG_SCHEME_BUCKET = 0,
     PG_SCHEME_SINGLE_SLOT = 1,
     PG_SCHEME_POWER_OF_N_SHORTCUT = 2,
-    PG_SCHEME_LINEAR_BUCKETS = 3
+    PG_SCHEME_LINEAR_BUCKETS = 3,
+    PG_SCHEME_GROUPED_SHORTCUT = 4
 };
 
 /* Probing scheme (the probing_schemes directory).  Probe i of a key, before the
@@ -123,6 +146,10 @@ typedef struct pg_level_desc {
     uint32_t prober;       /* enum pg_prober */
     uint32_t hasher;       /* enum pg_hasher */
     uint32_t max_probes;   /* Max_Probes */
+    uint32_t group;        /* GROUPED_SHORTCUT: buckets per 128-byte line (K of
+                              blocked_bucket_insert<K>, K * bucket bytes <= 128);
+                              0 for every other scheme */
+    uint32_t reserved;     /* 0 */
 } pg_level_desc;
 
 /* A table: level 0 plus an optional backing level (Is_Recursive /

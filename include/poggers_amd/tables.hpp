@@ -37,6 +37,14 @@ struct bit_grouped_container {  // dynamic_container<bit_grouped_container<K,V>,
         KeyBits == 16 ? PG_REP_FP16_V16 : (KeyBits == 8 ? PG_REP_FP8_V8 : PG_REP_FP10_V6);
 };
 
+// wrapper_half_bucket<uint16_t> (representations/12_bit_bucket.cuh:319-323):
+// the 12-bit sub-word bucket, key-only (Val = uint16_t, queries report 0).
+template <typename Internal_Storage>
+struct wrapper_half_bucket {
+    static_assert(std::is_same<Internal_Storage, uint16_t>::value, "the 12-bit bucket stores uint16_t words");
+    static constexpr uint32_t rep = PG_REP_FP12;
+};
+
 // ---- insert schemes (insert_schemes/*.cuh) ----
 struct bucket_insert { static constexpr uint32_t scheme = PG_SCHEME_BUCKET; };
 struct single_slot_insert { static constexpr uint32_t scheme = PG_SCHEME_SINGLE_SLOT; };
@@ -44,6 +52,14 @@ struct power_of_n_insert_shortcut_bucket_scheme {
     static constexpr uint32_t scheme = PG_SCHEME_POWER_OF_N_SHORTCUT;
 };
 struct linear_insert_bucket_scheme { static constexpr uint32_t scheme = PG_SCHEME_LINEAR_BUCKETS; };
+// blocked_bucket_insert<K> (insert_schemes/grouped_power_buckets.cuh:388-393):
+// the shortcut scheme over K buckets per 128-byte line.
+template <std::size_t Buckets_per_cache_line>
+struct blocked_bucket_insert {
+    static_assert(Buckets_per_cache_line >= 1, "at least one bucket per line");
+    static constexpr uint32_t scheme = PG_SCHEME_GROUPED_SHORTCUT;
+    static constexpr uint32_t group = (uint32_t)Buckets_per_cache_line;
+};
 
 // ---- probing schemes (probing_schemes/*.cuh) ----
 struct linearProber { static constexpr uint32_t prober = PG_PROBE_LINEAR; };
@@ -104,6 +120,14 @@ inline void check(int rc, const char *what) {
     if (rc != PG_OK) throw pg_error(what, rc);
 }
 
+// Buckets per 128-byte line of a grouped scheme (0 for the others).
+template <typename S, typename = void>
+struct group_trait { static constexpr uint32_t value = 0; };
+template <typename S>
+struct group_trait<S, std::void_t<decltype(S::group)>> { static constexpr uint32_t value = S::group; };
+template <typename S>
+constexpr uint32_t group_of() { return group_trait<S>::value; }
+
 template <typename Key, typename Val, typename Internal_Rep, std::size_t Partition_Size, std::size_t Bucket_Size,
           typename Insert_Scheme, std::size_t Max_Probes, typename Probing_Scheme, typename Hasher,
           bool Is_Recursive = false, typename Recursive_Type = recursive_end_table>
@@ -118,6 +142,9 @@ class bucketed_table {
 
     static_assert(std::is_same<Key, uint64_t>::value, "keys are uint64_t");
     static_assert(rep != PG_REP_KV64 || std::is_same<Val, uint64_t>::value, "key_val_pair values are uint64_t");
+    static_assert(rep != PG_REP_FP12 || std::is_same<Val, uint16_t>::value, "12-bit bucket values are uint16_t");
+    static_assert(rep != PG_REP_FP12 || !std::is_same<Probing_Scheme, XORPowerOfTwoHasher>::value,
+                  "twelve_bucket::tag() returns nothing (12_bit_bucket.cuh:313): no XOR probing");
     static_assert(rep != PG_REP_FP16_V16 || std::is_same<Val, uint16_t>::value,
                   "16/16 filter values are uint16_t");
     static_assert(!(rep == PG_REP_FP8_V8 || rep == PG_REP_FP10_V6) || std::is_same<Val, uint8_t>::value,
@@ -134,7 +161,8 @@ class bucketed_table {
     static_assert(rep == PG_REP_KV64
                       ? (Insert_Scheme::scheme == PG_SCHEME_BUCKET || Insert_Scheme::scheme == PG_SCHEME_SINGLE_SLOT)
                       : (Insert_Scheme::scheme == PG_SCHEME_POWER_OF_N_SHORTCUT ||
-                         Insert_Scheme::scheme == PG_SCHEME_LINEAR_BUCKETS),
+                         Insert_Scheme::scheme == PG_SCHEME_LINEAR_BUCKETS ||
+                         Insert_Scheme::scheme == PG_SCHEME_GROUPED_SHORTCUT),
                   "insert scheme does not apply to this representation");
     static_assert(!(Insert_Scheme::scheme == PG_SCHEME_SINGLE_SLOT) || Partition_Size == Bucket_Size,
                   "single_slot_insert needs Partition_Size == Bucket_Size (single_slot_insert.cuh:43)");
@@ -154,6 +182,8 @@ class bucketed_table {
         l.prober = Probing_Scheme::prober;
         l.hasher = Hasher::hasher;
         l.max_probes = (uint32_t)Max_Probes;
+        l.group = group_of<Insert_Scheme>();
+        l.reserved = 0;
         if constexpr (Is_Recursive) {
             static_assert(Recursive_Type::rep == rep, "levels share one representation");
             Recursive_Type::describe(d, first + 1, sizing);
@@ -259,6 +289,15 @@ using tcf_10_6_backing = bucketed_table<uint64_t, uint8_t, bit_grouped_container
 using tcf_10_6 = bucketed_table<uint64_t, uint8_t, bit_grouped_container<10, 6>, 1, 8,
                                 power_of_n_insert_shortcut_bucket_scheme, 2, doubleHasher, murmurHasher, true,
                                 tcf_10_6_backing>;
+
+// The reference's 12-bit bucket benchmarks (tests/src/bucket_speed_tests.cu:
+// 82-102): plain and cache-line-grouped (blocked_bucket_insert<K>).
+using tcqf_twelve = bucketed_table<uint64_t, uint16_t, wrapper_half_bucket<uint16_t>, 4, 16,
+                                   power_of_n_insert_shortcut_bucket_scheme, 2, doubleHasher, murmurHasher>;
+using tcqf_twelve_16 = bucketed_table<uint64_t, uint16_t, wrapper_half_bucket<uint16_t>, 4, 16,
+                                      blocked_bucket_insert<5>, 2, doubleHasher, murmurHasher>;
+using tcqf_twelve_8 = bucketed_table<uint64_t, uint16_t, wrapper_half_bucket<uint16_t>, 2, 8,
+                                     blocked_bucket_insert<10>, 2, doubleHasher, murmurHasher>;
 
 // The MI355X default: two choices from one fmix64 hash split into 32-bit
 // halves, tile 2 (the fastest measured shape at 2^28 slots).

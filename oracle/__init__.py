@@ -19,8 +19,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6 = 0, 1, 2, 3
-SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS = 0, 1, 2, 3
+REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12 = 0, 1, 2, 3, 4
+SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS, SCHEME_GROUPED = 0, 1, 2, 3, 4
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
 
@@ -34,6 +34,8 @@ class LevelDesc(ctypes.Structure):
         ("prober", ctypes.c_uint32),
         ("hasher", ctypes.c_uint32),
         ("max_probes", ctypes.c_uint32),
+        ("group", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -153,9 +155,10 @@ def gen_keys(seed: int, start: int, n: int) -> np.ndarray:
     return out
 
 
-def level(nslots, scheme, bucket_size, tile=4, prober=PROBE_SPLIT64, hasher=HASH_FMIX64, max_probes=2):
+def level(nslots, scheme, bucket_size, tile=4, prober=PROBE_SPLIT64, hasher=HASH_FMIX64, max_probes=2,
+          group=0):
     return dict(nslots=int(nslots), scheme=scheme, bucket_size=bucket_size, tile=tile,
-                prober=prober, hasher=hasher, max_probes=max_probes)
+                prober=prober, hasher=hasher, max_probes=max_probes, group=group)
 
 
 class Table:
@@ -172,6 +175,7 @@ class Table:
         if not self._h:
             raise ValueError("oracle table creation failed")
         self.val_bytes = 8 if rep == REP_KV64 else (1 if rep in (REP_FP8_V8, REP_FP10_V6) else 2)
+        # REP_FP12 stores no values: queries return 0
         self.val_dtype = {8: np.uint64, 2: np.uint16, 1: np.uint8}[self.val_bytes]
 
     def __del__(self):
@@ -246,7 +250,7 @@ class Table:
 
     def level_array(self, lvl: int) -> np.ndarray:
         dt = {REP_KV64: np.uint64, REP_FP16_V16: np.uint32, REP_FP8_V8: np.uint16,
-              REP_FP10_V6: np.uint16}[self.rep]
+              REP_FP10_V6: np.uint16, REP_FP12: np.uint16}[self.rep]
         return np.frombuffer(self.level_bytes(lvl), dtype=dt).copy()
 
     def num_buckets(self, lvl: int) -> int:

@@ -36,9 +36,10 @@
 
 /* ---- enums (values mirror include/poggers_amd.h so tests can share one
  *      descriptor layout; the oracle does not include that header) ---- */
-enum { OR_REP_KV64 = 0, OR_REP_FP16_V16 = 1, OR_REP_FP8_V8 = 2, OR_REP_FP10_V6 = 3 };
+enum { OR_REP_KV64 = 0, OR_REP_FP16_V16 = 1, OR_REP_FP8_V8 = 2, OR_REP_FP10_V6 = 3, OR_REP_FP12 = 4 };
 enum { OR_SCHEME_BUCKET = 0, OR_SCHEME_SINGLE_SLOT = 1,
-       OR_SCHEME_POWER_OF_N_SHORTCUT = 2, OR_SCHEME_LINEAR_BUCKETS = 3 };
+       OR_SCHEME_POWER_OF_N_SHORTCUT = 2, OR_SCHEME_LINEAR_BUCKETS = 3,
+       OR_SCHEME_GROUPED = 4 };
 enum { OR_PROBE_LINEAR = 0, OR_PROBE_DOUBLE = 1, OR_PROBE_P2 = 2,
        OR_PROBE_XOR_P2 = 3, OR_PROBE_SPLIT64 = 4 };
 enum { OR_HASH_FMIX64 = 0, OR_HASH_MURMUR64A = 1 };
@@ -233,6 +234,10 @@ int or_probes(uint32_t prober, uint32_t hasher, uint32_t max_probes,
  * unmasked SmallKey. */
 static uint32_t fp_of(uint32_t rep, uint64_t key)
 {
+    /* twelve_bucket takes (uint16_t)key with no remap (12_bit_bucket.cuh:
+     * 104-122, representation_helpers.cuh:930-962); each slot remaps its
+     * own 12 bits (or_fp12_proj). */
+    if (rep == OR_REP_FP12) return (uint16_t)key;
     if (rep == OR_REP_FP8_V8) {
         uint32_t f = (uint8_t)key;
         if (f == 0) f += 1;
@@ -266,6 +271,8 @@ uint32_t or_unpack_val(uint32_t kbits, uint32_t vbits, uint32_t w)
 typedef struct {
     uint64_t nslots;
     uint32_t scheme, bucket_size, tile, prober, hasher, max_probes;
+    uint32_t group;      /* GROUPED: buckets per 128-byte line; else 0 */
+    uint32_t reserved;
 } or_level_desc;
 
 typedef struct {
@@ -273,8 +280,10 @@ typedef struct {
     uint64_t seed;
     uint64_t nb;        /* buckets (for SINGLE_SLOT: slots) */
     uint32_t B;         /* slots per bucket */
-    void *slots;        /* uint32_t[] (FP16_V16), uint16_t[] (FP8_V8), uint64_t[2*] (KV64) */
+    void *slots;        /* uint32_t[] (FP16_V16), uint16_t[] (FP8_V8, FP10_V6, FP12), uint64_t[2*] (KV64) */
     uint64_t nslot_alloc;
+    uint64_t bucket_bytes;
+    uint64_t bytes;     /* size of the slot array */
 } or_level;
 
 typedef struct {
@@ -287,18 +296,78 @@ typedef struct {
     uint64_t buckets_read[2];  /* distinct bucket examinations per level */
 } or_table;
 
-/* Word access for the packed (filter) representations: 16-bit words for the
- * 8/8 and 10/6 slots, 32-bit words for 16/16. */
-static inline int narrow(const or_table *t) { return t->rep == OR_REP_FP8_V8 || t->rep == OR_REP_FP10_V6; }
-static inline uint32_t wget(const or_table *t, const or_level *L, uint64_t i)
+static inline int narrow(const or_table *t)
 {
-    if (narrow(t)) return ((const uint16_t *)L->slots)[i];
-    return ((const uint32_t *)L->slots)[i];
+    return t->rep == OR_REP_FP8_V8 || t->rep == OR_REP_FP10_V6 || t->rep == OR_REP_FP12;
 }
-static inline void wset(const or_table *t, or_level *L, uint64_t i, uint32_t w)
+
+/* Bucket b of a level.  Plain levels are arrays of buckets; GROUPED levels
+ * put `group` buckets in each 128-byte line and pad the rest
+ * (grouped_power_buckets.cuh:27-37 bucket_wrapper, :150-160 slot b / K, b % K). */
+static inline uint8_t *bucket_at(const or_level *L, uint64_t b)
 {
-    if (narrow(t)) ((uint16_t *)L->slots)[i] = (uint16_t)w;
-    else ((uint32_t *)L->slots)[i] = w;
+    uint64_t off = L->d.group ? (b / L->d.group) * 128 + (b % L->d.group) * L->bucket_bytes
+                              : b * L->bucket_bytes;
+    return (uint8_t *)L->slots + off;
+}
+
+/* The 12-bit sub-word bucket (12_bit_bucket.cuh:33-313): four slots in three
+ * uint16 words; slot 4g+o keeps these bits of the 16-bit item, and remaps a
+ * zero first part so a filled slot never reads as empty
+ * (representation_helpers.cuh:600-777 read / match, :779-903 replace):
+ *   o=0  w0[0:12]              m = item & 0x0FFF,  0 -> 1
+ *   o=1  w0[12:16] + w1[0:8]   m = item & 0xF0FF,  (item & 0xF000) == 0 -> |= 0x1000
+ *   o=2  w1[8:16] + w2[0:4]    m = item & 0xFF0F,  (item & 0xFF00) == 0 -> |= 0x0100
+ *   o=3  w2[4:16]              m = item & 0xFFF0,  0 -> 0x0010              */
+uint32_t or_fp12_proj(uint32_t item, uint32_t o)
+{
+    item &= 0xFFFFu;
+    switch (o & 3u) {
+    case 0: { uint32_t m = item & 0x0FFFu; return m ? m : 1u; }
+    case 1: { uint32_t m = item & 0xF0FFu; return (item & 0xF000u) ? m : (m | 0x1000u); }
+    case 2: { uint32_t m = item & 0xFF0Fu; return (item & 0xFF00u) ? m : (m | 0x0100u); }
+    default: { uint32_t m = item & 0xFFF0u; return m ? m : 0x0010u; }
+    }
+}
+static uint32_t fp12_get(const uint16_t *w, uint32_t i)
+{
+    w += 3 * (i / 4);
+    switch (i & 3u) {
+    case 0: return w[0] & 0x0FFFu;
+    case 1: return (w[0] & 0xF000u) | (w[1] & 0x00FFu);
+    case 2: return (w[1] & 0xFF00u) | (w[2] & 0x000Fu);
+    default: return w[2] & 0xFFF0u;
+    }
+}
+static void fp12_put(uint16_t *w, uint32_t i, uint32_t m)
+{
+    w += 3 * (i / 4);
+    switch (i & 3u) {
+    case 0: w[0] = (uint16_t)((w[0] & 0xF000u) | (m & 0x0FFFu)); break;
+    case 1: w[0] = (uint16_t)((w[0] & 0x0FFFu) | (m & 0xF000u));
+            w[1] = (uint16_t)((w[1] & 0xFF00u) | (m & 0x00FFu)); break;
+    case 2: w[1] = (uint16_t)((w[1] & 0x00FFu) | (m & 0xFF00u));
+            w[2] = (uint16_t)((w[2] & 0xFFF0u) | (m & 0x000Fu)); break;
+    default: w[2] = (uint16_t)((w[2] & 0x000Fu) | (m & 0xFFF0u)); break;
+    }
+}
+
+/* Slot i of bucket b: the stored word (packed slots) or the slot's 12-bit
+ * composite (FP12). */
+static inline uint32_t sget(const or_table *t, const or_level *L, uint64_t b, uint32_t i)
+{
+    const uint8_t *p = bucket_at(L, b);
+    if (t->rep == OR_REP_FP12) return fp12_get((const uint16_t *)p, i);
+    if (narrow(t)) return ((const uint16_t *)p)[i];
+    return ((const uint32_t *)p)[i];
+}
+/* Store: a packed word, or for FP12 the item's projection for slot i. */
+static inline void sset(const or_table *t, or_level *L, uint64_t b, uint32_t i, uint32_t w)
+{
+    uint8_t *p = bucket_at(L, b);
+    if (t->rep == OR_REP_FP12) fp12_put((uint16_t *)p, i, or_fp12_proj(w, i & 3u));
+    else if (narrow(t)) ((uint16_t *)p)[i] = (uint16_t)w;
+    else ((uint32_t *)p)[i] = w;
 }
 static inline uint32_t kbits_of(const or_table *t)
 {
@@ -313,11 +382,37 @@ static inline uint32_t slot_bytes(const or_table *t)
 {
     return t->rep == OR_REP_KV64 ? 16u : (narrow(t) ? 2u : 4u);
 }
+/* Bytes of one bucket: B slots, or (12*B - 1)/16 + 1 uint16 words for the
+ * 12-bit bucket (12_bit_bucket.cuh:46). */
+static inline uint64_t bucket_bytes_of(uint32_t rep, uint32_t B)
+{
+    if (rep == OR_REP_FP12) return 2ull * ((12ull * B - 1) / 16 + 1);
+    if (rep == OR_REP_KV64) return 16ull * B;
+    return (uint64_t)B * ((rep == OR_REP_FP16_V16) ? 4u : 2u);
+}
 
 /* Packed-slot predicates: grouped_storage_sub_bits.cuh:160-195.
  * empty == fp bits 0; tombstone == fp bits all-ones (get_empty()-1). */
-static inline int p_empty(const or_table *t, uint32_t w) { return (w & kmask_of(t)) == 0; }
-static inline int p_tomb(const or_table *t, uint32_t w) { return (w & kmask_of(t)) == kmask_of(t); }
+static inline int p_empty(const or_table *t, uint32_t w)
+{
+    return t->rep == OR_REP_FP12 ? w == 0 : (w & kmask_of(t)) == 0;
+}
+/* The 12-bit bucket has no tombstones (its remove does not compile in the
+ * reference, 12_bit_bucket.cuh:207-246). */
+static inline int p_tomb(const or_table *t, uint32_t w)
+{
+    return t->rep == OR_REP_FP12 ? 0 : (w & kmask_of(t)) == kmask_of(t);
+}
+/* Query match of slot i (contains / sub_byte_match) and its value. */
+static inline int p_match(const or_table *t, uint32_t w, uint32_t fp, uint32_t i)
+{
+    if (t->rep == OR_REP_FP12) return w == or_fp12_proj(fp, i & 3u);
+    return (w & kmask_of(t)) == (fp & kmask_of(t));
+}
+static inline uint32_t p_val(const or_table *t, uint32_t w)
+{
+    return t->rep == OR_REP_FP12 ? 0u : w >> kbits_of(t);  /* 12-bit query returns 0 */
+}
 static inline uint32_t tomb_word(const or_table *t) { return kmask_of(t); } /* join(tombstone, 0) */
 
 /* KV predicates: representations/key_val_pair.cuh:70,111-125 (empty key 0,
@@ -329,8 +424,7 @@ static void account_bucket(or_table *t, int level, uint64_t b)
 {
     (void)b;
     t->buckets_read[level] += 1;
-    uint64_t bytes = (uint64_t)t->lv[level].B * slot_bytes(t);
-    t->sectors_read += (bytes + 63) / 64;
+    t->sectors_read += (t->lv[level].bucket_bytes + 63) / 64;
 }
 
 /* ------------------------------------------------------------------ */
@@ -345,7 +439,7 @@ static int pb_fill(const or_table *t, const or_level *L, uint64_t b)
 {
     int f = 0;
     for (uint32_t i = 0; i < L->B; i++) {
-        uint32_t w = wget(t, L, b * L->B + i);
+        uint32_t w = sget(t, L, b, i);
         if (!(p_empty(t, w) || p_tomb(t, w))) f++;
     }
     return f;
@@ -356,11 +450,10 @@ static int pb_fill(const or_table *t, const or_level *L, uint64_t b)
 static int pb_insert(or_table *t, or_level *L, uint64_t b, uint32_t word)
 {
     for (uint32_t i = 0; i < L->B; i++) {
-        uint64_t s = b * L->B + i;
-        uint32_t w = wget(t, L, s);
+        uint32_t w = sget(t, L, b, i);
         if (p_empty(t, w)) {
             if (w == 0) { /* CAS(0 -> word) */
-                wset(t, L, s, word);
+                sset(t, L, b, i, word);
                 t->sectors_written += 1;
                 return 1;
             }
@@ -375,12 +468,11 @@ static int pb_insert(or_table *t, or_level *L, uint64_t b, uint32_t word)
 static int pb_insert_delete(or_table *t, or_level *L, uint64_t b, uint32_t word)
 {
     for (uint32_t i = 0; i < L->B; i++) {
-        uint64_t s = b * L->B + i;
-        uint32_t w = wget(t, L, s);
+        uint32_t w = sget(t, L, b, i);
         if (p_empty(t, w) || p_tomb(t, w)) {
             uint32_t expect = p_empty(t, w) ? 0u : tomb_word(t);
             if (w == expect) {
-                wset(t, L, s, word);
+                sset(t, L, b, i, word);
                 t->sectors_written += 1;
                 return 1;
             }
@@ -394,9 +486,9 @@ static int pb_query(const or_table *t, const or_level *L, uint64_t b, uint32_t f
                     uint32_t *val)
 {
     for (uint32_t i = 0; i < L->B; i++) {
-        uint32_t w = wget(t, L, b * L->B + i);
-        if ((w & kmask_of(t)) == (fp & kmask_of(t))) {
-            *val = w >> kbits_of(t);
+        uint32_t w = sget(t, L, b, i);
+        if (p_match(t, w, fp, i)) {
+            *val = p_val(t, w);
             return 1;
         }
     }
@@ -408,10 +500,9 @@ static int pb_query(const or_table *t, const or_level *L, uint64_t b, uint32_t f
 static int pb_remove(or_table *t, or_level *L, uint64_t b, uint32_t fp)
 {
     for (uint32_t i = 0; i < L->B; i++) {
-        uint64_t s = b * L->B + i;
-        uint32_t w = wget(t, L, s);
-        if ((w & kmask_of(t)) == (fp & kmask_of(t))) {
-            wset(t, L, s, tomb_word(t));
+        uint32_t w = sget(t, L, b, i);
+        if (p_match(t, w, fp, i)) {
+            sset(t, L, b, i, tomb_word(t));
             t->sectors_written += 1;
             return 1;
         }
@@ -674,7 +765,17 @@ static int kv_remove(or_table *t, or_level *L, int lvl, uint64_t key)
  * slot scheme allocates nslots slots (single_slot_insert.cuh:89-96). */
 or_table *or_create(uint32_t rep, uint32_t nlevels, const or_level_desc *lv, uint64_t seed)
 {
-    if (nlevels < 1 || nlevels > 2) return NULL;
+    if (nlevels < 1 || nlevels > 2 || rep > OR_REP_FP12) return NULL;
+    for (uint32_t l = 0; l < nlevels; l++) {
+        /* GROUPED levels hold `group` packed buckets per 128-byte line
+         * (grouped_power_buckets.cuh:58: sizeof(bucket) * K <= 128). */
+        int grouped = lv[l].scheme == OR_SCHEME_GROUPED;
+        if (grouped != (lv[l].group != 0)) return NULL;
+        if (grouped && (rep == OR_REP_KV64 || lv[l].bucket_size == 0 ||
+                        lv[l].group * bucket_bytes_of(rep, lv[l].bucket_size) > 128))
+            return NULL;
+        if ((rep == OR_REP_KV64) != (lv[l].scheme <= OR_SCHEME_SINGLE_SLOT)) return NULL;
+    }
     or_table *t = (or_table *)calloc(1, sizeof(or_table));
     if (!t) return NULL;
     t->rep = rep;
@@ -694,7 +795,9 @@ or_table *or_create(uint32_t rep, uint32_t nlevels, const or_level_desc *lv, uin
             L->nb = (lv[l].nslots - 1) / L->B + 1;
         }
         L->nslot_alloc = L->nb * L->B;
-        L->slots = calloc(L->nslot_alloc, slot_bytes(t));
+        L->bucket_bytes = bucket_bytes_of(rep, L->B);
+        L->bytes = lv[l].group ? ((L->nb - 1) / lv[l].group + 1) * 128 : L->nb * L->bucket_bytes;
+        L->slots = calloc(L->bytes, 1);
         if (!L->slots) { for (uint32_t q = 0; q < l; q++) free(t->lv[q].slots); free(t); return NULL; }
     }
     return t;
@@ -710,27 +813,29 @@ void or_destroy(or_table *t)
 void or_clear(or_table *t)
 {
     for (uint32_t l = 0; l < t->nlevels; l++)
-        memset(t->lv[l].slots, 0, t->lv[l].nslot_alloc * slot_bytes(t));
+        memset(t->lv[l].slots, 0, t->lv[l].bytes);
 }
 
 /* Per-level dispatch for the packed representations.  Every table op
- * tries level 0 and, on false, the backing level (bucketed_table.cuh:271-381). */
+ * tries level 0 and, on false, the backing level (bucketed_table.cuh:271-381).
+ * GROUPED levels run the shortcut scheme's ops: power_of_n_insert_grouped_buckets
+ * (grouped_power_buckets.cuh:164-384) repeats them over grouped buckets. */
 static int lvl_insert(or_table *t, int l, uint64_t key, uint32_t word, int with_delete)
 {
     or_level *L = &t->lv[l];
-    if (L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT) return shortcut_insert(t, L, key, word, with_delete);
+    if (L->d.scheme != OR_SCHEME_LINEAR_BUCKETS) return shortcut_insert(t, L, key, word, with_delete);
     return linear_insert(t, L, l, key, word, with_delete);
 }
 static int lvl_query(or_table *t, int l, uint64_t key, uint32_t *val)
 {
     or_level *L = &t->lv[l];
-    if (L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT) return shortcut_query(t, L, key, val);
+    if (L->d.scheme != OR_SCHEME_LINEAR_BUCKETS) return shortcut_query(t, L, key, val);
     return linear_query(t, L, l, key, val);
 }
 static int lvl_remove(or_table *t, int l, uint64_t key)
 {
     or_level *L = &t->lv[l];
-    if (L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT) return shortcut_remove(t, L, key);
+    if (L->d.scheme != OR_SCHEME_LINEAR_BUCKETS) return shortcut_remove(t, L, key);
     return linear_remove(t, L, l, key);
 }
 
@@ -742,7 +847,11 @@ int or_insert(or_table *t, uint64_t key, uint64_t val, int with_delete)
             if (kv_insert(t, &t->lv[l], (int)l, key, val)) return 1 + (int)l;
         return 0;
     }
-    uint32_t word = or_pack(kbits_of(t), vbits_of(t), fp_of(t->rep, key), (uint32_t)val);
+    /* twelve_bucket has no insert_delete that compiles (12_bit_bucket.cuh:
+     * 148-203 calls a sub_byte_match overload that does not exist) */
+    if (t->rep == OR_REP_FP12 && with_delete) return -1;
+    uint32_t word = t->rep == OR_REP_FP12 ? fp_of(t->rep, key)
+                                          : or_pack(kbits_of(t), vbits_of(t), fp_of(t->rep, key), (uint32_t)val);
     for (uint32_t l = 0; l < t->nlevels; l++)
         if (lvl_insert(t, (int)l, key, word, with_delete)) return 1 + (int)l;
     return 0;
@@ -771,6 +880,7 @@ int or_remove(or_table *t, uint64_t key)
         }
         return 0;
     }
+    if (t->rep == OR_REP_FP12) return -1; /* twelve_bucket::remove does not compile (12_bit_bucket.cuh:207-246) */
     for (uint32_t l = 0; l < t->nlevels; l++)
         if (lvl_remove(t, (int)l, key)) return 1;
     return 0;
@@ -832,11 +942,12 @@ int or_insert_if_not_exists(or_table *t, uint64_t key, uint64_t val, int del, in
     *found = 0;
     *val_out = 0;
     if (t->rep == OR_REP_KV64) return -1; /* bucket_insert has no upsert */
+    if (t->rep == OR_REP_FP12) return -1; /* not offered for the 12-bit bucket */
     uint32_t word = or_pack(kbits_of(t), vbits_of(t), fp_of(t->rep, key), (uint32_t)val);
     for (uint32_t l = 0; l < t->nlevels; l++) {
         or_level *L = &t->lv[l];
         uint32_t v = 0;
-        int r = (L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT)
+        int r = (L->d.scheme != OR_SCHEME_LINEAR_BUCKETS)
                     ? shortcut_upsert(t, L, key, word, del, found, &v)
                     : linear_upsert(t, L, key, word, del, found, &v);
         if (r) {
@@ -908,15 +1019,18 @@ uint64_t or_fill(const or_table *t)
     uint64_t f = 0;
     for (uint32_t l = 0; l < t->nlevels; l++) {
         const or_level *L = &t->lv[l];
-        for (uint64_t s = 0; s < L->nslot_alloc; s++) {
-            if (t->rep == OR_REP_KV64) {
+        if (t->rep == OR_REP_KV64) {
+            for (uint64_t s = 0; s < L->nslot_alloc; s++) {
                 uint64_t k = ((const uint64_t *)L->slots)[2 * s];
                 if (k != KV_EMPTY && k != KV_TOMB) f++;
-            } else {
-                uint32_t w = wget(t, L, s);
+            }
+            continue;
+        }
+        for (uint64_t b = 0; b < L->nb; b++)
+            for (uint32_t i = 0; i < L->B; i++) {
+                uint32_t w = sget(t, L, b, i);
                 if (!(p_empty(t, w) || p_tomb(t, w))) f++;
             }
-        }
     }
     return f;
 }
@@ -924,7 +1038,7 @@ uint64_t or_fill(const or_table *t)
 uint64_t or_level_bytes(const or_table *t, uint32_t l)
 {
     if (l >= t->nlevels) return 0;
-    return t->lv[l].nslot_alloc * slot_bytes(t);
+    return t->lv[l].bytes;
 }
 const void *or_level_data(const or_table *t, uint32_t l)
 {

@@ -21,8 +21,8 @@ LIB_PATH = os.path.join(_HERE, "libpoggers_amd.so")
 
 # enum values of include/poggers_amd.h
 PG_OK = 0
-REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6 = 0, 1, 2, 3
-SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS = 0, 1, 2, 3
+REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12 = 0, 1, 2, 3, 4
+SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS, SCHEME_GROUPED = 0, 1, 2, 3, 4
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
 CONCURRENT, DETERMINISTIC = 0, 1
@@ -57,6 +57,8 @@ class LevelDesc(ctypes.Structure):
         ("prober", ctypes.c_uint32),
         ("hasher", ctypes.c_uint32),
         ("max_probes", ctypes.c_uint32),
+        ("group", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -162,10 +164,12 @@ class Level:
     prober: int = PROBE_SPLIT64
     hasher: int = HASH_FMIX64
     max_probes: int = 2
+    group: int = 0  # SCHEME_GROUPED: buckets per 128-byte line
 
     def as_dict(self):
         return dict(nslots=int(self.nslots), scheme=self.scheme, bucket_size=self.bucket_size,
-                    tile=self.tile, prober=self.prober, hasher=self.hasher, max_probes=self.max_probes)
+                    tile=self.tile, prober=self.prober, hasher=self.hasher, max_probes=self.max_probes,
+                    group=self.group)
 
 
 @dataclass
@@ -487,7 +491,8 @@ def file_info(path: str):
     seed = ctypes.c_uint64()
     check(lib().pg_file_info(path.encode(), ctypes.byref(d), ctypes.byref(seed)), "pg_file_info")
     levels = [Level(d.level[i].nslots, d.level[i].scheme, d.level[i].bucket_size, d.level[i].tile,
-                    d.level[i].prober, d.level[i].hasher, d.level[i].max_probes) for i in range(d.num_levels)]
+                    d.level[i].prober, d.level[i].hasher, d.level[i].max_probes, d.level[i].group)
+              for i in range(d.num_levels)]
     return TableDesc(d.rep, levels), seed.value
 
 

@@ -64,7 +64,8 @@ def build_cpp_tests(verbose: bool = False) -> str:
     root = os.path.dirname(HERE)
     src = os.path.join(root, "tests", "cpp", "test_frontend.cpp")
     out = os.path.join(root, "tests", "cpp", "test_frontend")
-    cmd = [hipcc(), "-O2", "-std=c++17", "-Wno-unused-command-line-argument", "-I", os.path.join(root, "include"),
+    cmd = [hipcc(), "-O2", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-Wno-unused-command-line-argument", "-I", os.path.join(root, "include"),
            src, "-o", out, "-L", HERE, "-lpoggers_amd", "-Wl,-rpath,$ORIGIN/../../poggers_amd"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

@@ -219,6 +219,8 @@ int pg_batch_create(const pg_desc *proto, uint32_t ntables, const uint64_t *nslo
     if (l0.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT) return PG_ERR_UNSUPPORTED;
     for (uint32_t i = 0; i < proto->num_levels; i++) {
         if (proto->level[i].tile != l0.tile) return PG_ERR_INVALID;
+        if (i > 0 && proto->level[i].scheme != PG_SCHEME_LINEAR_BUCKETS) return PG_ERR_UNSUPPORTED;
+        if (proto->level[i].group != 0) return PG_ERR_UNSUPPORTED;
         if (proto->level[i].bucket_size == 0 || proto->level[i].bucket_size > 64) return PG_ERR_INVALID;
     }
     const uint32_t P = l0.tile;
@@ -260,6 +262,8 @@ int pg_batch_create(const pg_desc *proto, uint32_t ntables, const uint64_t *nslo
             pg::LevelDev &L = T.lv[l];
             L.nb = pg::make_fastmod(nb);
             L.B = B;
+            L.group = 0;
+            L.bucket_bytes = B * 4;
             L.scheme = proto->level[l].scheme;
             L.prober = proto->level[l].prober;
             L.hasher = proto->level[l].hasher;

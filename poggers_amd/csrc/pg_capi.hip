@@ -35,21 +35,33 @@ int check_level(uint32_t rep, const pg_level_desc &l, uint32_t index) {
     if (rep == PG_REP_KV64) {
         if (l.scheme != PG_SCHEME_BUCKET && l.scheme != PG_SCHEME_SINGLE_SLOT) return PG_ERR_UNSUPPORTED;
         if (l.scheme == PG_SCHEME_BUCKET && (l.bucket_size == 0 || l.bucket_size > 64)) return PG_ERR_INVALID;
-    } else if (rep == PG_REP_FP16_V16 || rep == PG_REP_FP8_V8 || rep == PG_REP_FP10_V6) {
-        if (l.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT && l.scheme != PG_SCHEME_LINEAR_BUCKETS)
+    } else if (rep <= PG_REP_FP12) {
+        if (l.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT && l.scheme != PG_SCHEME_LINEAR_BUCKETS &&
+            l.scheme != PG_SCHEME_GROUPED_SHORTCUT)
             return PG_ERR_UNSUPPORTED;
         if (l.bucket_size == 0 || l.bucket_size > 64) return PG_ERR_INVALID;
-        if (l.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT && l.max_probes > pg::kMaxShortcutProbes)
+        if (l.scheme != PG_SCHEME_LINEAR_BUCKETS && l.max_probes > pg::kMaxShortcutProbes)
             return PG_ERR_UNSUPPORTED;
+        // twelve_bucket::tag() returns nothing (12_bit_bucket.cuh:313-315), so
+        // a tag-driven probe is undefined in the reference: not offered.
+        if (rep == PG_REP_FP12 && l.prober == PG_PROBE_XOR_P2) return PG_ERR_UNSUPPORTED;
     } else {
         return PG_ERR_INVALID;
     }
+    // blocked_bucket_insert<K>: K buckets in one 128-byte line
+    // (grouped_power_buckets.cuh:58).
+    if ((l.scheme == PG_SCHEME_GROUPED_SHORTCUT) != (l.group != 0)) return PG_ERR_INVALID;
+    if (l.group && l.group * pg::bucket_bytes(rep, l.bucket_size) > 128) return PG_ERR_INVALID;
+    if (l.reserved != 0) return PG_ERR_INVALID;
     (void)index;
     return PG_OK;
 }
 
-uint32_t slot_bytes(uint32_t rep) {
-    return rep == PG_REP_KV64 ? 16u : (rep == PG_REP_FP16_V16 ? 4u : 2u);
+// Bytes of one level's slot array: nb buckets, or ceil(nb / group) 128-byte
+// lines for a grouped level.
+uint64_t level_bytes(uint32_t rep, const pg_level_desc &l, uint32_t B, uint64_t nb) {
+    if (l.group) return ((nb - 1) / l.group + 1) * 128;
+    return nb * pg::bucket_bytes(rep, B);
 }
 
 }  // namespace
@@ -124,9 +136,12 @@ int pg_create(const pg_desc *desc, uint64_t seed, pg_handle *out) {
         L.prober = l.prober;
         L.hasher = l.hasher;
         L.max_probes = l.max_probes;
+        L.group = l.group;
+        L.bucket_bytes = (uint32_t)pg::bucket_bytes(desc->rep, B);
         s = s * s;  // recursive level seeded seed*seed (bucketed_table.cuh:170)
-        const uint64_t bytes = ((nb * B * slot_bytes(desc->rep)) + 255) & ~uint64_t(255);
-        t->bytes[i] = nb * B * slot_bytes(desc->rep);
+        t->bytes[i] = level_bytes(desc->rep, l, B, nb);
+        // padded to 256 B: 16-bit slots are swapped through their 32-bit word
+        const uint64_t bytes = (t->bytes[i] + 255) & ~uint64_t(255);
         if (hipMalloc(&L.slots, bytes) != hipSuccess) {
             L.slots = nullptr;
             rc = PG_ERR_NOMEM;
@@ -157,7 +172,7 @@ int pg_destroy(pg_handle h) {
 int pg_clear(pg_handle h, void *stream) {
     if (!h) return PG_ERR_INVALID;
     for (uint32_t i = 0; i < h->dev.nlevels; i++) {
-        const int rc = pg::launch_zero(h->dev.lv[i].slots, (h->bytes[i] + 15) & ~uint64_t(15),
+        const int rc = pg::launch_zero(h->dev.lv[i].slots, (h->bytes[i] + 255) & ~uint64_t(255),
                                        (hipStream_t)stream);
         if (rc != PG_OK) return rc;
     }
@@ -199,16 +214,18 @@ int pg_bulk_insert(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint
 int pg_bulk_insert_with_delete(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n,
                                uint8_t *d_status, uint64_t *d_fail_count, void *stream) {
     if (!h) return PG_ERR_INVALID;
-    // bucket_insert / single_slot_insert have no insert_with_delete.
-    if (h->dev.rep == PG_REP_KV64) return PG_ERR_UNSUPPORTED;
+    // bucket_insert / single_slot_insert have no insert_with_delete, and
+    // twelve_bucket's does not compile (12_bit_bucket.cuh:148-203).
+    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_FP12) return PG_ERR_UNSUPPORTED;
     return do_insert(h, d_keys, d_vals, n, d_status, d_fail_count, stream, 1);
 }
 
 static int do_upsert(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n, void *d_vals_out,
                      uint8_t *d_found, uint8_t *d_status, uint64_t *d_fail_count, void *stream, uint32_t del) {
     if (!h || (n && !d_keys)) return PG_ERR_INVALID;
-    // bucket_insert / single_slot_insert have no insert_if_not_exists.
-    if (h->dev.rep == PG_REP_KV64) return PG_ERR_UNSUPPORTED;
+    // bucket_insert / single_slot_insert have no insert_if_not_exists; not
+    // offered for the 12-bit bucket (its _delete form needs remove).
+    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_FP12) return PG_ERR_UNSUPPORTED;
     pg::OpArgs a{};
     a.keys = d_keys;
     a.vals = d_vals;
@@ -252,6 +269,8 @@ int pg_bulk_remove(pg_handle h, const uint64_t *d_keys, uint64_t n, uint8_t *d_f
     if (!h || (n && !d_keys)) return PG_ERR_INVALID;
     for (uint32_t i = 0; i < h->dev.nlevels; i++)
         if (h->dev.lv[i].scheme == PG_SCHEME_SINGLE_SLOT) return PG_ERR_UNSUPPORTED;
+    // twelve_bucket::remove does not compile (12_bit_bucket.cuh:207-246)
+    if (h->dev.rep == PG_REP_FP12) return PG_ERR_UNSUPPORTED;
     pg::OpArgs a{};
     a.keys = d_keys;
     a.n = n;
@@ -310,7 +329,7 @@ int pg_level_ptr(pg_handle h, uint32_t level, void **d_ptr, uint64_t *bytes) {
 
 // ---- raw bucket-array files ----
 static const char kMagic[8] = {'P', 'G', 'A', 'M', 'D', 'T', 'B', '1'};
-static const uint32_t kFileVersion = 1;
+static const uint32_t kFileVersion = 2;  // 2: pg_level_desc carries group + reserved
 
 struct FileHeader {
     char magic[8];

@@ -112,8 +112,32 @@ struct LevelDev {
     uint32_t prober;
     uint32_t hasher;
     uint32_t max_probes;
+    uint32_t group;      // GROUPED_SHORTCUT: buckets per 128-byte line, else 0
+    uint32_t bucket_bytes;
     uint32_t pad;
 };
+
+// Bytes of one bucket of B slots; the 12-bit bucket packs four slots in
+// three uint16 words: (12B - 1)/16 + 1 words (12_bit_bucket.cuh:46).
+__host__ __device__ inline uint64_t bucket_bytes(uint32_t rep, uint32_t B) {
+    if (rep == PG_REP_FP12) return 2ull * ((12ull * B - 1) / 16 + 1);
+    if (rep == PG_REP_KV64) return 16ull * B;
+    return (uint64_t)B * (rep == PG_REP_FP16_V16 ? 4u : 2u);
+}
+
+// Start of bucket b: plain levels are arrays of buckets; a grouped level puts
+// `group` buckets in each 128-byte line and pads the rest
+// (grouped_power_buckets.cuh:27-37 bucket_wrapper, :150-160 b / K, b % K).
+__host__ __device__ inline void *bucket_ptr(const LevelDev &L, uint64_t b) {
+    uint64_t off;
+    if (L.group) {
+        const uint64_t line = b / L.group;
+        off = line * 128 + (b - line * L.group) * L.bucket_bytes;
+    } else {
+        off = b * L.bucket_bytes;
+    }
+    return (char *)L.slots + off;
+}
 
 struct TableDev {
     LevelDev lv[PG_MAX_LEVELS];
@@ -192,18 +216,47 @@ struct Fmt10 {  // bit_grouped_container<10,6>, SmallKey uint16, 16-bit word
     using V = uint8_t;
     static constexpr uint32_t small_bits = 16, kbits = 10, kmask = 0x3FFu, vmask = 0x3Fu;
 };
+// The 12-bit sub-word bucket (12_bit_bucket.cuh:33-342): key-only, no
+// tombstones; the "fingerprint" is the key's low 16 bits with no remap, and
+// slot 4g+o keeps and remaps its own 12 of them (twelve_proj).
+struct Fmt12 {
+    using W = uint16_t;
+    using V = uint16_t;
+    static constexpr uint32_t small_bits = 16, kbits = 16, kmask = 0xFFFFu, vmask = 0u;
+};
+template <class F> inline constexpr bool is_twelve = false;
+template <> inline constexpr bool is_twelve<Fmt12> = true;
 
 template <class F>
 __host__ __device__ inline uint32_t fp_of(uint64_t key) {
-    constexpr uint32_t all = (1u << F::small_bits) - 1u;
-    uint32_t f = (uint32_t)key & all;
-    f += (f == 0u);
-    f -= (f == all);
-    return f;
+    if constexpr (is_twelve<F>) {
+        return (uint32_t)key & 0xFFFFu;
+    } else {
+        constexpr uint32_t all = (1u << F::small_bits) - 1u;
+        uint32_t f = (uint32_t)key & all;
+        f += (f == 0u);
+        f -= (f == all);
+        return f;
+    }
 }
 template <class F>
 __host__ __device__ inline uint32_t pack_word(uint32_t fp, uint32_t val) {
-    return ((val & F::vmask) << F::kbits) | (fp & F::kmask);
+    if constexpr (is_twelve<F>) return fp;  // the item; each slot stores its projection
+    else return ((val & F::vmask) << F::kbits) | (fp & F::kmask);
+}
+
+// 12-bit slot o's stored bits of item, remapped so a filled slot never reads
+// as empty (representation_helpers.cuh:684-777 match_*, :779-903 replace_*):
+//   o=0 item & 0x0FFF (0 -> 1)        o=1 item & 0xF0FF (| 0x1000 if bits 12-15 are 0)
+//   o=2 item & 0xFF0F (| 0x0100 if bits 8-15 are 0)   o=3 item & 0xFFF0 (0 -> 0x0010)
+__host__ __device__ inline uint32_t twelve_proj(uint32_t item, uint32_t o) {
+    item &= 0xFFFFu;
+    switch (o & 3u) {
+        case 0: { const uint32_t m = item & 0x0FFFu; return m ? m : 1u; }
+        case 1: { const uint32_t m = item & 0xF0FFu; return (item & 0xF000u) ? m : (m | 0x1000u); }
+        case 2: { const uint32_t m = item & 0xFF0Fu; return (item & 0xFF00u) ? m : (m | 0x0100u); }
+        default: { const uint32_t m = item & 0xFFF0u; return m ? m : 0x0010u; }
+    }
 }
 
 // ---- launchers (defined in the .hip files) ----

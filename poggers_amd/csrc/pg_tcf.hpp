@@ -14,6 +14,8 @@
 // contiguous words (64 B at B = 16), bucket b at word b*B.
 #pragma once
 
+#include <type_traits>
+
 #include "pg_internal.hpp"
 #include "pg_tile.hpp"
 
@@ -40,17 +42,24 @@ __device__ inline bool f_filled(uint32_t w) { return !(f_empty<F>(w) || f_tomb<F
 __device__ inline bool cas_word(uint32_t *p, uint32_t expect, uint32_t desired) {
     return atomicCAS(p, expect, desired) == expect;
 }
-__device__ inline bool cas_word(uint16_t *p, uint32_t expect, uint32_t desired) {
+// Replace the `mask` bits of a 16-bit word if they equal expect's
+// (replace_uint16, representation_helpers.cuh:451-489, on the aligned 32-bit
+// word that holds it).
+__device__ inline bool cas_bits16(uint16_t *p, uint32_t mask, uint32_t expect, uint32_t desired) {
     uint32_t *w32 = (uint32_t *)((uintptr_t)p & ~uintptr_t(3));
     const uint32_t sh = ((uintptr_t)p & 2u) ? 16u : 0u;
+    const uint32_t m32 = (mask & 0xFFFFu) << sh;
+    const uint32_t e32 = (expect & mask & 0xFFFFu) << sh, d32 = (desired & mask & 0xFFFFu) << sh;
     uint32_t old = __hip_atomic_load(w32, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (;;) {
-        if (((old >> sh) & 0xFFFFu) != (expect & 0xFFFFu)) return false;
-        const uint32_t nw = (old & ~(0xFFFFu << sh)) | ((desired & 0xFFFFu) << sh);
-        const uint32_t prev = atomicCAS(w32, old, nw);
+        if ((old & m32) != e32) return false;
+        const uint32_t prev = atomicCAS(w32, old, (old & ~m32) | d32);
         if (prev == old) return true;
         old = prev;
     }
+}
+__device__ inline bool cas_word(uint16_t *p, uint32_t expect, uint32_t desired) {
+    return cas_bits16(p, 0xFFFFu, expect, desired);
 }
 
 template <int V>
@@ -188,39 +197,116 @@ struct Bucket {
 };
 
 // ---------------------------------------------------------------------------
-// Generic bucket routines: runtime B, one word per lane per row, re-read per
+// Slot access for the generic routines, per slot format.  A slot reads as a
+// "composite": the stored word (packed formats) or, for the 12-bit bucket,
+// the slot's 12 bits gathered from its three-word group
+// (representation_helpers.cuh:600-682 read_*).
+// ---------------------------------------------------------------------------
+template <class F>
+__device__ inline uint32_t s_get(const void *bk, uint32_t i) {
+    if constexpr (is_twelve<F>) {
+        const uint16_t *w = (const uint16_t *)bk + 3 * (i >> 2);
+        switch (i & 3u) {
+            case 0: return w[0] & 0x0FFFu;
+            case 1: return (w[0] & 0xF000u) | (w[1] & 0x00FFu);
+            case 2: return (w[1] & 0xFF00u) | (w[2] & 0x000Fu);
+            default: return w[2] & 0xFFF0u;
+        }
+    } else {
+        return (uint32_t)((const typename F::W *)bk)[i];
+    }
+}
+template <class F>
+__device__ inline bool s_empty(uint32_t c) {
+    if constexpr (is_twelve<F>) return c == 0u;
+    else return f_empty<F>(c);
+}
+template <class F>
+__device__ inline bool s_tomb(uint32_t c) {
+    if constexpr (is_twelve<F>) return false;  // no tombstones
+    else return f_tomb<F>(c);
+}
+template <class F>
+__device__ inline bool s_filled(uint32_t c) { return !(s_empty<F>(c) || s_tomb<F>(c)); }
+// contains(): the key bits equal fp's; for the 12-bit bucket the composite
+// equals slot i's projection of the item (sub_byte_match).
+template <class F>
+__device__ inline bool s_match(uint32_t c, uint32_t fp, uint32_t i) {
+    if constexpr (is_twelve<F>) return c == twelve_proj(fp, i);
+    else return (c & F::kmask) == (fp & F::kmask);
+}
+template <class F>
+__device__ inline uint32_t s_value(uint32_t c) {
+    if constexpr (is_twelve<F>) return 0u;  // twelve_bucket::query reports 0
+    else return (c >> F::kbits) & F::vmask;
+}
+// Claim slot i, seen empty (or, with from_tomb, seen as a tombstone), for
+// `word`.  Packed formats CAS the whole word from the empty / tombstone word
+// (grouped_storage_sub_bits.cuh:127-157).  The 12-bit bucket replaces the
+// slot's first part only if it is still zero, then writes the second part
+// (representation_helpers.cuh:779-903 replace_*: two masked swaps, the
+// second asserted to succeed).
+template <class F>
+__device__ inline bool s_claim(void *bk, uint32_t i, bool from_tomb, uint32_t word) {
+    if constexpr (is_twelve<F>) {
+        uint16_t *w = (uint16_t *)bk + 3 * (i >> 2);
+        const uint32_t m = twelve_proj(word, i);
+        switch (i & 3u) {
+            case 0: return cas_bits16(w, 0x0FFFu, 0u, m);
+            case 1:
+                if (!cas_bits16(w, 0xF000u, 0u, m)) return false;
+                cas_bits16(w + 1, 0x00FFu, 0u, m);
+                return true;
+            case 2:
+                if (!cas_bits16(w + 1, 0xFF00u, 0u, m)) return false;
+                cas_bits16(w + 2, 0x000Fu, 0u, m);
+                return true;
+            default: return cas_bits16(w + 2, 0xFFF0u, 0u, m);
+        }
+    } else {
+        return cas_word((typename F::W *)bk + i, from_tomb ? F::kmask : 0u, word);
+    }
+}
+// Tombstone slot i seen as c (grouped_storage_sub_bits.cuh:217-234).
+template <class F>
+__device__ inline bool s_bury(void *bk, uint32_t i, uint32_t c) {
+    if constexpr (is_twelve<F>) return false;  // removal is not offered
+    else return cas_word((typename F::W *)bk + i, c, F::kmask);
+}
+
+// ---------------------------------------------------------------------------
+// Generic bucket routines: runtime B, one slot per lane per row, re-read per
 // pass (the reference reads the bucket once for its fill and again for the
-// claim).  Used by the backing level and by shapes without a fast path.
+// claim).  Used by the backing level, grouped levels, the 8/8, 10/6 and
+// 12-bit slots and the shapes without a fast path.
 // ---------------------------------------------------------------------------
 template <int P, class F = Fmt16>
-__device__ int gen_fill(const typename F::W *base, uint32_t B, const Tile<P> &t) {
+__device__ int gen_fill(const void *base, uint32_t B, const Tile<P> &t) {
     int f = 0;
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
-        const uint32_t w = ok ? (uint32_t)base[i] : 0u;
-        f += t.popc(ok && f_filled<F>(w));
+        const uint32_t w = ok ? s_get<F>(base, i) : 0u;
+        f += t.popc(ok && s_filled<F>(w));
     }
     return f;
 }
 
-// Claim the lowest empty slot (with `wd`, the lowest empty-or-tombstone one)
-// by a CAS from the empty word (or from the tombstone word).
-// packed_bucket.cuh:64-180, grouped_storage_sub_bits.cuh:127-157.
+// Claim the lowest empty slot (with `wd`, the lowest empty-or-tombstone one).
+// packed_bucket.cuh:64-180, 12_bit_bucket.cuh:94-144.
 template <int P, class F = Fmt16>
-__device__ bool gen_claim(typename F::W *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t) {
+__device__ bool gen_claim(void *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t) {
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
-        const uint32_t w = ok ? (uint32_t)base[i] : 0u;
-        bool cand = ok && (f_empty<F>(w) || (wd && f_tomb<F>(w)));
+        const uint32_t w = ok ? s_get<F>(base, i) : 0u;
+        bool cand = ok && (s_empty<F>(w) || (wd && s_tomb<F>(w)));
         uint64_t lanes = t.bits(cand);
         while (lanes) {
             const uint32_t l = (uint32_t)ctz64(lanes);
             bool success = false;
             if (t.rank == l) {
-                const uint32_t expect = (wd && f_tomb<F>(w)) ? F::kmask : 0u;
-                success = cas_word(base + i, expect, word);
+                success = s_claim<F>(base, i, wd && s_tomb<F>(w), word);
                 cand = false;
             }
             if (t.any(success)) {
@@ -233,19 +319,21 @@ __device__ bool gen_claim(typename F::W *base, uint32_t B, uint32_t word, bool w
     return false;
 }
 
-// Lowest slot whose key bits equal fp's (fp & kmask); when none, *fill
-// receives the bucket's fill (same pass).  packed_bucket.cuh:182-219.
+// Lowest matching slot; when none, *fill receives the bucket's fill (same
+// pass).  packed_bucket.cuh:182-219, 12_bit_bucket.cuh:259-300.
 template <int P, class F = Fmt16>
-__device__ bool gen_find(const typename F::W *base, uint32_t B, uint32_t fp, const Tile<P> &t,
+__device__ bool gen_find(const void *base, uint32_t B, uint32_t fp, const Tile<P> &t,
                          uint32_t &val, bool &owner, int &fill) {
     fill = 0;
-    if constexpr (F::kbits == 16) {
+    if constexpr (std::is_same<F, Fmt16>::value) {
         if ((B & 3u) == 0 && B <= 4u * P) {
             // The whole bucket in one row: lane r holds words 4r..4r+3, read
-            // with one 16 B load (B % 4 == 0 keeps every bucket 16 B aligned).
+            // with one 16 B load (B % 4 == 0 keeps every bucket 16 B aligned,
+            // grouped or not).
+            const uint32_t *b32 = (const uint32_t *)base;
             const bool ok = 4u * t.rank < B;
             uint32_t w[4] = {0u, 0u, 0u, 0u};
-            if (ok) load_words<4>(base + 4 * t.rank, w);
+            if (ok) load_words<4>(b32 + 4 * t.rank, w);
             uint32_t m = 0;
 #pragma unroll
             for (int v = 0; v < 4; v++)
@@ -262,40 +350,37 @@ __device__ bool gen_find(const typename F::W *base, uint32_t B, uint32_t fp, con
             return false;
         }
     }
-    const uint32_t fpm = fp & F::kmask;
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
-        const uint32_t w = ok ? (uint32_t)base[i] : 0u;
-        const uint64_t lanes = t.bits(ok && (w & F::kmask) == fpm);
+        const uint32_t w = ok ? s_get<F>(base, i) : 0u;
+        const uint64_t lanes = t.bits(ok && s_match<F>(w, fp, i));
         if (lanes) {
             const uint32_t l = (uint32_t)ctz64(lanes);
             owner = (t.rank == l);
-            if (owner) val = (w >> F::kbits) & F::vmask;
+            if (owner) val = s_value<F>(w);
             return true;
         }
-        fill += t.popc(ok && f_filled<F>(w));
+        fill += t.popc(ok && s_filled<F>(w));
     }
     return false;
 }
 
-// Tombstone the lowest matching slot: CAS from the word as read to the
-// tombstone word.  packed_bucket.cuh:227-271, grouped_storage_sub_bits.cuh:217-234.
+// Tombstone the lowest matching slot.  packed_bucket.cuh:227-271.
 template <int P, class F = Fmt16>
-__device__ bool gen_remove(typename F::W *base, uint32_t B, uint32_t fp, const Tile<P> &t, int &fill) {
+__device__ bool gen_remove(void *base, uint32_t B, uint32_t fp, const Tile<P> &t, int &fill) {
     fill = 0;
-    const uint32_t fpm = fp & F::kmask;
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
-        const uint32_t w = ok ? (uint32_t)base[i] : 0u;
-        bool m = ok && (w & F::kmask) == fpm;
+        const uint32_t w = ok ? s_get<F>(base, i) : 0u;
+        bool m = ok && s_match<F>(w, fp, i);
         uint64_t lanes = t.bits(m);
         while (lanes) {
             const uint32_t l = (uint32_t)ctz64(lanes);
             bool success = false;
             if (t.rank == l) {
-                success = cas_word(base + i, w, F::kmask);
+                success = s_bury<F>(base, i, w);
                 m = false;
             }
             if (t.any(success)) {
@@ -304,7 +389,7 @@ __device__ bool gen_remove(typename F::W *base, uint32_t B, uint32_t fp, const T
             }
             lanes = t.bits(m);
         }
-        fill += t.popc(ok && f_filled<F>(w));
+        fill += t.popc(ok && s_filled<F>(w));
     }
     return false;
 }
@@ -320,7 +405,6 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
                                     bool wd, const Tile<P> &t, int ai) {
     Probe pr;
     pr.begin(L, key, fp);
-    typename F::W *slots = (typename F::W *)L.slots;
     const int B = (int)L.B;
     uint64_t bidx[kMaxShortcutProbes];
     int fill[kMaxShortcutProbes];
@@ -332,9 +416,9 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
         const uint64_t b = fastmod(p, L.nb);
         bidx[j] = b;
         t.acc[ai]++;
-        int f = gen_fill<P, F>(slots + b * L.B, L.B, t);
+        int f = gen_fill<P, F>(bucket_ptr(L, b), L.B, t);
         if (4 * f < 3 * B) {
-            if (gen_claim<P, F>(slots + b * L.B, L.B, word, wd, t)) return true;
+            if (gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t)) return true;
             f = B;
         }
         fill[j] = f;
@@ -347,7 +431,7 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
 #pragma unroll
         for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
             if (j < np) {
-                if (fill[j] == lvl && gen_claim<P, F>(slots + bidx[j] * L.B, L.B, word, wd, t)) return true;
+                if (fill[j] == lvl && gen_claim<P, F>(bucket_ptr(L, bidx[j]), L.B, word, wd, t)) return true;
                 if (fill[j] > lvl && fill[j] < next) next = fill[j];
             }
         }
@@ -363,20 +447,19 @@ __device__ bool gen_linear_insert(const LevelDev &L, uint64_t key, uint32_t fp, 
     Probe pr;
     pr.begin(L, key, fp);
     pr.max_probes = pr.useful_probes(L.nb.d);
-    typename F::W *slots = (typename F::W *)L.slots;
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
         if (p == kEnd) return false;
         const uint64_t b = fastmod(p, L.nb);
         t.acc[ai]++;
-        if (gen_claim<P, F>(slots + b * L.B, L.B, word, wd, t)) return true;
+        if (gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t)) return true;
     }
 }
 
 template <int P, class F = Fmt16>
 __device__ bool gen_level_insert(const LevelDev &L, uint64_t key, uint32_t fp, uint32_t word, bool wd,
                                  const Tile<P> &t, int ai) {
-    if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT) return gen_shortcut_insert<P, F>(L, key, fp, word, wd, t, ai);
+    if (L.scheme != PG_SCHEME_LINEAR_BUCKETS) return gen_shortcut_insert<P, F>(L, key, fp, word, wd, t, ai);
     return gen_linear_insert<P, F>(L, key, fp, word, wd, t, ai);
 }
 
@@ -388,8 +471,7 @@ __device__ bool gen_level_query(const LevelDev &L, uint64_t key, uint32_t fp, co
                                 uint32_t &val, bool &owner, int ai) {
     Probe pr;
     pr.begin(L, key, fp);
-    const typename F::W *slots = (const typename F::W *)L.slots;
-    const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
+    const bool linear = L.scheme == PG_SCHEME_LINEAR_BUCKETS;
     if (linear) pr.max_probes = pr.useful_probes(L.nb.d);
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
@@ -397,7 +479,7 @@ __device__ bool gen_level_query(const LevelDev &L, uint64_t key, uint32_t fp, co
         const uint64_t b = fastmod(p, L.nb);
         int f;
         t.acc[ai]++;
-        if (gen_find<P, F>(slots + b * L.B, L.B, fp, t, val, owner, f)) return true;
+        if (gen_find<P, F>(bucket_ptr(L, b), L.B, fp, t, val, owner, f)) return true;
         if (linear && f != (int)L.B) return false;
     }
 }
@@ -408,8 +490,7 @@ template <int P, class F = Fmt16>
 __device__ bool gen_level_remove(const LevelDev &L, uint64_t key, uint32_t fp, const Tile<P> &t, int ai) {
     Probe pr;
     pr.begin(L, key, fp);
-    typename F::W *slots = (typename F::W *)L.slots;
-    const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
+    const bool linear = L.scheme == PG_SCHEME_LINEAR_BUCKETS;
     if (linear) pr.max_probes = pr.useful_probes(L.nb.d);
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
@@ -417,7 +498,7 @@ __device__ bool gen_level_remove(const LevelDev &L, uint64_t key, uint32_t fp, c
         const uint64_t b = fastmod(p, L.nb);
         int f;
         t.acc[ai]++;
-        if (gen_remove<P, F>(slots + b * L.B, L.B, fp, t, f)) return true;
+        if (gen_remove<P, F>(bucket_ptr(L, b), L.B, fp, t, f)) return true;
         if (linear && f != (int)L.B) return false;
     }
 }
@@ -562,13 +643,12 @@ __device__ bool gen_level_upsert(const LevelDev &L, uint64_t key, uint32_t fp, u
                                  const Tile<P> &t, int ai, uint32_t &val, bool &owner, bool &found) {
     Probe pr;
     pr.begin(L, key, fp);
-    typename F::W *slots = (typename F::W *)L.slots;
-    const bool linear = L.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT;
+    const bool linear = L.scheme == PG_SCHEME_LINEAR_BUCKETS;
     if (linear) pr.max_probes = pr.useful_probes(L.nb.d);
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
         if (p == kEnd) break;
-        typename F::W *base = slots + fastmod(p, L.nb) * L.B;
+        void *base = bucket_ptr(L, fastmod(p, L.nb));
         int f;
         t.acc[ai]++;
         if (gen_find<P, F>(base, L.B, fp, t, val, owner, f)) {
@@ -901,6 +981,7 @@ int launch_tcf_p(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
     const LevelDev &L = T.lv[0];
     if (T.rep == PG_REP_FP8_V8) return launch_tcf_pb<P, 0, Fmt8>(T, op, a, s);
     if (T.rep == PG_REP_FP10_V6) return launch_tcf_pb<P, 0, Fmt10>(T, op, a, s);
+    if (T.rep == PG_REP_FP12) return launch_tcf_pb<P, 0, Fmt12>(T, op, a, s);
     if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT) {
         switch (L.B) {
             case 8: return launch_tcf_pb<P, 8>(T, op, a, s);

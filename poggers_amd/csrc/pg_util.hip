@@ -39,6 +39,28 @@ __global__ __launch_bounds__(kBlock) void k_fill_packed16(const uint16_t *w, uin
     if (__lane_id() == 0 && c) atomicAdd((unsigned long long *)counter, (unsigned long long)c);
 }
 
+// 12-bit buckets: a slot is filled when its composite is nonzero
+// (12_bit_bucket.cuh:51-68 get_fill); one thread per bucket.
+__global__ __launch_bounds__(kBlock) void k_fill_twelve(LevelDev L, uint64_t *counter) {
+    uint64_t c = 0;
+    for (uint64_t b = uint64_t(blockIdx.x) * kBlock + threadIdx.x; b < L.nb.d; b += uint64_t(gridDim.x) * kBlock) {
+        const uint16_t *w = (const uint16_t *)bucket_ptr(L, b);
+        for (uint32_t i = 0; i < L.B; i++) {
+            const uint16_t *g = w + 3 * (i >> 2);
+            uint32_t v;
+            switch (i & 3u) {
+                case 0: v = g[0] & 0x0FFFu; break;
+                case 1: v = (g[0] & 0xF000u) | (g[1] & 0x00FFu); break;
+                case 2: v = (g[1] & 0xFF00u) | (g[2] & 0x000Fu); break;
+                default: v = g[2] & 0xFFF0u; break;
+            }
+            c += (v != 0u);
+        }
+    }
+    c = wave_sum(c);
+    if (__lane_id() == 0 && c) atomicAdd((unsigned long long *)counter, (unsigned long long)c);
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_kv(const ulonglong2 *s, uint64_t n, uint64_t *counter) {
     uint64_t c = 0;
     for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
@@ -61,16 +83,21 @@ int launch_fill(const TableDev &T, uint64_t *d_counter, hipStream_t s) {
     for (uint32_t l = 0; l < T.nlevels; l++) {
         const LevelDev &L = T.lv[l];
         const uint64_t nslots = L.nb.d * (uint64_t)L.B;
-        if (T.rep == PG_REP_KV64)
+        // packed words of the whole array: a grouped level's padding is zero,
+        // i.e. empty, so it is scanned like slots
+        const uint64_t bytes = L.group ? ((L.nb.d - 1) / L.group + 1) * 128 : L.nb.d * (uint64_t)L.bucket_bytes;
+        if (T.rep == PG_REP_FP12)
+            hipLaunchKernelGGL(k_fill_twelve, dim3(stride_grid(L.nb.d)), dim3(kBlock), 0, s, L, d_counter);
+        else if (T.rep == PG_REP_KV64)
             hipLaunchKernelGGL(k_fill_kv, dim3(stride_grid(nslots)), dim3(kBlock), 0, s,
                                (const ulonglong2 *)L.slots, nslots, d_counter);
         else if (T.rep == PG_REP_FP8_V8 || T.rep == PG_REP_FP10_V6)
-            hipLaunchKernelGGL(k_fill_packed16, dim3(stride_grid(nslots)), dim3(kBlock), 0, s,
-                               (const uint16_t *)L.slots, nslots, T.rep == PG_REP_FP8_V8 ? 0xFFu : 0x3FFu,
+            hipLaunchKernelGGL(k_fill_packed16, dim3(stride_grid(bytes / 2)), dim3(kBlock), 0, s,
+                               (const uint16_t *)L.slots, bytes / 2, T.rep == PG_REP_FP8_V8 ? 0xFFu : 0x3FFu,
                                d_counter);
         else
-            hipLaunchKernelGGL(k_fill_packed, dim3(stride_grid(nslots)), dim3(kBlock), 0, s,
-                               (const uint32_t *)L.slots, nslots, d_counter);
+            hipLaunchKernelGGL(k_fill_packed, dim3(stride_grid(bytes / 4)), dim3(kBlock), 0, s,
+                               (const uint32_t *)L.slots, bytes / 4, d_counter);
     }
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
 }

@@ -48,6 +48,13 @@ static int cpu_checks() {
     CHECK(d.level[1].nslots == 10 && d.level[1].scheme == PG_SCHEME_LINEAR_BUCKETS && d.level[1].max_probes == 4000);
     static_assert(tcf::levels == 2 && hash_table_64::levels == 1, "levels");
     static_assert(small_tcf::rep == PG_REP_FP8_V8 && tcf_10_6::rep == PG_REP_FP10_V6, "packed formats");
+    static_assert(tcqf_twelve::rep == PG_REP_FP12 && tcqf_twelve_16::rep == PG_REP_FP12, "12-bit bucket");
+    size_in_num_slots<1> one_12(4096);
+    tcqf_twelve_16::describe(d, 0, &one_12);
+    CHECK(d.level[0].scheme == PG_SCHEME_GROUPED_SHORTCUT && d.level[0].group == 5 && d.level[0].tile == 4);
+    size_in_num_slots<1> one_b(4096);
+    tcqf_twelve::describe(d, 0, &one_b);
+    CHECK(d.level[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT && d.level[0].group == 0);
     size_in_num_slots<2> two_b(1000, 10);  // a sizing is consumed as levels take their sizes
     tcf_10_6::describe(d, 0, &two_b);
     CHECK(d.level[0].tile == 1 && d.level[0].bucket_size == 8);
@@ -93,9 +100,14 @@ static int roundtrip(const char *name, uint64_t nslots, uint64_t backing, uint64
     }
     CHECK(fn == 0);
     if (Table::rep == PG_REP_KV64) CHECK(wrong == 0);
-    const uint64_t slot_bytes = Table::rep == PG_REP_KV64 ? 16 : (Table::rep == PG_REP_FP16_V16 ? 4 : 2);
-    CHECK(t->host_bytes_in_use() == t->get_num_slots() * slot_bytes);
-    if (Table::rep == PG_REP_FP8_V8) {
+    if constexpr (Table::rep == PG_REP_FP12) {
+        // 24-byte buckets of 16 slots, five to a 128-byte line; key-only, no remove
+        CHECK(t->host_bytes_in_use() == ((t->get_num_slots() / 16 - 1) / 5 + 1) * 128);
+    } else {
+        const uint64_t slot_bytes = Table::rep == PG_REP_KV64 ? 16 : (Table::rep == PG_REP_FP16_V16 ? 4 : 2);
+        CHECK(t->host_bytes_in_use() == t->get_num_slots() * slot_bytes);
+    }
+    if (Table::rep == PG_REP_FP8_V8 || Table::rep == PG_REP_FP12) {
         // 8-bit fingerprints collide often: a remove may take another key's
         // word, so the exact post-remove counts below do not apply.
         for (void *p : {(void *)keys, (void *)absent, (void *)vals, (void *)out, (void *)found, (void *)fails})
@@ -127,6 +139,7 @@ static int gpu_checks() {
     if (roundtrip<fast_tcf>("fast_tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<hash_table_64>("hash_table_64", 1 << 20, 0, 900000)) return 1;
     if (roundtrip<small_tcf>("small_tcf", 1 << 20, 1 << 14, 900000)) return 1;
+    if (roundtrip<tcqf_twelve_16>("tcqf_twelve_16", 1 << 20, 0, 700000)) return 1;
     return 0;
 }
 

@@ -49,6 +49,15 @@ def test_create_rejects_bad_descriptors_without_gpu_work():
         pg.TableDesc(pg.REP_FP16_V16, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16, tile=4),
                                        pg.Level(64, pg.SCHEME_LINEAR_BUCKETS, 8, tile=8,
                                                 prober=pg.PROBE_LINEAR, max_probes=100)]),
+        # grouped levels: K buckets must fit one 128-byte line, K only with GROUPED
+        pg.TableDesc(pg.REP_FP16_V16, [pg.Level(1024, pg.SCHEME_GROUPED, 16, group=3)]),   # 3 * 64 B
+        pg.TableDesc(pg.REP_FP12, [pg.Level(1024, pg.SCHEME_GROUPED, 16, group=6)]),      # 6 * 24 B
+        pg.TableDesc(pg.REP_FP16_V16, [pg.Level(1024, pg.SCHEME_GROUPED, 16, group=0)]),
+        pg.TableDesc(pg.REP_FP16_V16, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16, group=2)]),
+        pg.TableDesc(pg.REP_KV64, [pg.Level(1024, pg.SCHEME_GROUPED, 4, group=2)]),
+        # twelve_bucket::tag() returns nothing: no tag-driven probing for it
+        pg.TableDesc(pg.REP_FP12, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16, prober=pg.PROBE_XOR_P2)]),
+        pg.TableDesc(5, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16)]),
     ]
     for d in bad:
         h = ctypes.c_void_p()

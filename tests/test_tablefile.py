@@ -17,7 +17,7 @@ def test_file_header_is_read_on_the_host(tmp_path):
     desc = pg.tcf_desc(1 << 20, 1 << 10, tile=2)
     h = FileHeader()
     h.magic = b"PGAMDTB1"
-    h.version = 1
+    h.version = 2
     h.header_bytes = ctypes.sizeof(FileHeader)
     h.desc = desc.to_c()
     h.seed = 4242
