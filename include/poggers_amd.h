@@ -73,9 +73,17 @@ enum pg_status {
  *             reference's remove and insert_delete for this bucket do not
  *             compile, so remove, insert_with_delete and the upserts return
  *             PG_ERR_UNSUPPORTED; queries report value 0 (uint16 buffers).
+ *   KEY64     dynamic_container<key_container, uint64_t>: one 8 B key per
+ *             slot, the key-only table the CMS allocator keeps its free
+ *             list in (allocators/cms.cuh:203; representations/key_only.cuh).
+ *             The stored key is get_smallKey(key): 0 -> 1, ~0 -> ~0 - 1
+ *             (dynamic_container.cuh:35-50), so every key is accepted and
+ *             0/1 (and ~0/~0-1) share a slot; probes hash the key as
+ *             given.  Schemes BUCKET and SINGLE_SLOT, as KV64; queries
+ *             report value 0 (uint16 buffers, the CMS table's Val).
  *   Value buffers hold the reference's Val type: uint64 (KV64), uint16
- *   (FP16_V16, FP12), uint8 (FP8_V8 and FP10_V6, which keep the low 8 / 6
- *   bits).  The 8/8, 10/6 and 12-bit formats and grouped levels run the
+ *   (FP16_V16, FP12, KEY64), uint8 (FP8_V8 and FP10_V6, which keep the low
+ *   8 / 6 bits).  The 8/8, 10/6 and 12-bit formats and grouped levels run the
  *   generic (runtime bucket size) kernels; batches and the server take KV64
  *   and FP16_V16 tables only.                                             */
 enum pg_rep {
@@ -83,7 +91,8 @@ enum pg_rep {
     PG_REP_FP16_V16 = 1,
     PG_REP_FP8_V8 = 2,
     PG_REP_FP10_V6 = 3,
-    PG_REP_FP12 = 4
+    PG_REP_FP12 = 4,
+    PG_REP_KEY64 = 5
 };
 
 /* Insert scheme of a level.

@@ -45,6 +45,13 @@ struct wrapper_half_bucket {
     static constexpr uint32_t rep = PG_REP_FP12;
 };
 
+// dynamic_container<key_container, uint64_t> (representations/key_only.cuh
+// under dynamic_container.cuh): key-only 8-byte slots, the CMS allocator's
+// free table (allocators/cms.cuh:203).
+struct key_container {
+    static constexpr uint32_t rep = PG_REP_KEY64;
+};
+
 // ---- insert schemes (insert_schemes/*.cuh) ----
 struct bucket_insert { static constexpr uint32_t scheme = PG_SCHEME_BUCKET; };
 struct single_slot_insert { static constexpr uint32_t scheme = PG_SCHEME_SINGLE_SLOT; };
@@ -142,6 +149,8 @@ class bucketed_table {
 
     static_assert(std::is_same<Key, uint64_t>::value, "keys are uint64_t");
     static_assert(rep != PG_REP_KV64 || std::is_same<Val, uint64_t>::value, "key_val_pair values are uint64_t");
+    static_assert(rep != PG_REP_KEY64 || std::is_same<Val, uint16_t>::value,
+                  "key-only tables take uint16_t value buffers (queries report 0)");
     static_assert(rep != PG_REP_FP12 || std::is_same<Val, uint16_t>::value, "12-bit bucket values are uint16_t");
     static_assert(rep != PG_REP_FP12 || !std::is_same<Probing_Scheme, XORPowerOfTwoHasher>::value,
                   "twelve_bucket::tag() returns nothing (12_bit_bucket.cuh:313): no XOR probing");
@@ -158,7 +167,7 @@ class bucketed_table {
                     std::is_same<Probing_Scheme, split64Hasher>::value) ||
                       Max_Probes == 2,
                   "two-choice probing schemes take exactly two probes (power_of_two.cuh:32)");
-    static_assert(rep == PG_REP_KV64
+    static_assert((rep == PG_REP_KV64 || rep == PG_REP_KEY64)
                       ? (Insert_Scheme::scheme == PG_SCHEME_BUCKET || Insert_Scheme::scheme == PG_SCHEME_SINGLE_SLOT)
                       : (Insert_Scheme::scheme == PG_SCHEME_POWER_OF_N_SHORTCUT ||
                          Insert_Scheme::scheme == PG_SCHEME_LINEAR_BUCKETS ||
@@ -306,6 +315,11 @@ using fast_tcf_backing = bucketed_table<uint64_t, uint16_t, bit_grouped_containe
 using fast_tcf = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16, 16>, 2, 16,
                                 power_of_n_insert_shortcut_bucket_scheme, 2, split64Hasher, fmix64Hasher, true,
                                 fast_tcf_backing>;
+
+// The CMS allocator's free table (allocators/cms.cuh:203): key-only,
+// tile 1, 4-slot buckets, double hashing with 20 probes.
+using cms_free_table = static_table<uint64_t, uint16_t, key_container, 1, 4, bucket_insert, 20, doubleHasher,
+                                    murmurHasher>;
 
 // hash_table_64 (data_structs/uint64_hash_table.cuh:35): u64 -> u64, tile 4,
 // 16-slot buckets, double hashing with 20 probes.

@@ -19,7 +19,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12 = 0, 1, 2, 3, 4
+REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12, REP_KEY64 = 0, 1, 2, 3, 4, 5
 SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS, SCHEME_GROUPED = 0, 1, 2, 3, 4
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
@@ -175,7 +175,7 @@ class Table:
         if not self._h:
             raise ValueError("oracle table creation failed")
         self.val_bytes = 8 if rep == REP_KV64 else (1 if rep in (REP_FP8_V8, REP_FP10_V6) else 2)
-        # REP_FP12 stores no values: queries return 0
+        # REP_FP12 and REP_KEY64 store no values: queries return 0
         self.val_dtype = {8: np.uint64, 2: np.uint16, 1: np.uint8}[self.val_bytes]
 
     def __del__(self):
@@ -250,7 +250,7 @@ class Table:
 
     def level_array(self, lvl: int) -> np.ndarray:
         dt = {REP_KV64: np.uint64, REP_FP16_V16: np.uint32, REP_FP8_V8: np.uint16,
-              REP_FP10_V6: np.uint16, REP_FP12: np.uint16}[self.rep]
+              REP_FP10_V6: np.uint16, REP_FP12: np.uint16, REP_KEY64: np.uint64}[self.rep]
         return np.frombuffer(self.level_bytes(lvl), dtype=dt).copy()
 
     def num_buckets(self, lvl: int) -> int:

@@ -36,7 +36,8 @@
 
 /* ---- enums (values mirror include/poggers_amd.h so tests can share one
  *      descriptor layout; the oracle does not include that header) ---- */
-enum { OR_REP_KV64 = 0, OR_REP_FP16_V16 = 1, OR_REP_FP8_V8 = 2, OR_REP_FP10_V6 = 3, OR_REP_FP12 = 4 };
+enum { OR_REP_KV64 = 0, OR_REP_FP16_V16 = 1, OR_REP_FP8_V8 = 2, OR_REP_FP10_V6 = 3, OR_REP_FP12 = 4,
+       OR_REP_KEY64 = 5 };
 enum { OR_SCHEME_BUCKET = 0, OR_SCHEME_SINGLE_SLOT = 1,
        OR_SCHEME_POWER_OF_N_SHORTCUT = 2, OR_SCHEME_LINEAR_BUCKETS = 3,
        OR_SCHEME_GROUPED = 4 };
@@ -296,6 +297,7 @@ typedef struct {
     uint64_t buckets_read[2];  /* distinct bucket examinations per level */
 } or_table;
 
+static inline int is_kv_rep(uint32_t rep) { return rep == OR_REP_KV64 || rep == OR_REP_KEY64; }
 static inline int narrow(const or_table *t)
 {
     return t->rep == OR_REP_FP8_V8 || t->rep == OR_REP_FP10_V6 || t->rep == OR_REP_FP12;
@@ -380,6 +382,7 @@ static inline uint32_t vbits_of(const or_table *t)
 static inline uint32_t kmask_of(const or_table *t) { return (1u << kbits_of(t)) - 1u; }
 static inline uint32_t slot_bytes(const or_table *t)
 {
+    if (t->rep == OR_REP_KEY64) return 8u;
     return t->rep == OR_REP_KV64 ? 16u : (narrow(t) ? 2u : 4u);
 }
 /* Bytes of one bucket: B slots, or (12*B - 1)/16 + 1 uint16 words for the
@@ -388,6 +391,7 @@ static inline uint64_t bucket_bytes_of(uint32_t rep, uint32_t B)
 {
     if (rep == OR_REP_FP12) return 2ull * ((12ull * B - 1) / 16 + 1);
     if (rep == OR_REP_KV64) return 16ull * B;
+    if (rep == OR_REP_KEY64) return 8ull * B;
     return (uint64_t)B * ((rep == OR_REP_FP16_V16) ? 4u : 2u);
 }
 
@@ -647,9 +651,27 @@ static int linear_remove(or_table *t, or_level *L, int lvl, uint64_t key)
 }
 
 /* ------------------------------------------------------------------ */
-/* KV (key_val_pair<uint64,uint64>) schemes                            */
+/* KV (key_val_pair<uint64,uint64>) schemes, and the key-only variant    */
+/* dynamic_container<key_container, uint64_t> (KEY64) the CMS allocator   */
+/* keeps its free list in (allocators/cms.cuh:203).                     */
 /* ------------------------------------------------------------------ */
-static inline uint64_t *kv_slot(or_level *L, uint64_t s) { return (uint64_t *)L->slots + 2 * s; }
+static inline int kv_keyonly(const or_table *t) { return t->rep == OR_REP_KEY64; }
+static inline uint64_t *kv_slot_t(const or_table *t, or_level *L, uint64_t s)
+{
+    return (uint64_t *)L->slots + (kv_keyonly(t) ? 1 : 2) * s;
+}
+#define kv_slot(L, s) kv_slot_t(t, (L), (s))
+/* The stored key: key_val_pair keeps the key itself; the key-only container
+ * stores get_smallKey(key) with SmallKey = uint64_t, i.e. 0 -> 1 and
+ * ~0 -> ~0 - 1 (representations/dynamic_container.cuh:35-50 over
+ * key_only.cuh:41-47,70-73).  Probing always hashes the key as given. */
+static inline uint64_t kv_skey(const or_table *t, uint64_t key)
+{
+    if (!kv_keyonly(t)) return key;
+    if (key == KV_EMPTY) key += 1;
+    if (key == KV_TOMB) key -= 1;
+    return key;
+}
 
 /* Sector accounting for KV probes: count distinct 64 B sectors in the
  * examined slot range [s0, s1) of the slot array, skipping the sector the
@@ -657,7 +679,8 @@ static inline uint64_t *kv_slot(or_level *L, uint64_t s) { return (uint64_t *)L-
 static void account_kv_range(or_table *t, int lvl, uint64_t s0, uint64_t s1, uint64_t *last_sector)
 {
     t->buckets_read[lvl] += 1;
-    uint64_t a = (s0 * 16) / 64, z = (s1 * 16 - 1) / 64;
+    uint64_t sb = slot_bytes(t);
+    uint64_t a = (s0 * sb) / 64, z = (s1 * sb - 1) / 64;
     for (uint64_t s = a; s <= z; s++) {
         if (s != *last_sector) t->sectors_read += 1;
         *last_sector = s;
@@ -681,8 +704,8 @@ static int kv_insert(or_table *t, or_level *L, int lvl, uint64_t key, uint64_t v
             uint64_t *s = kv_slot(L, base + i);
             if (s[0] == KV_EMPTY) {
                 account_kv_range(t, lvl, base, base + i + 1, &last);
-                s[0] = key;
-                s[1] = val;
+                s[0] = kv_skey(t, key);
+                if (!kv_keyonly(t)) s[1] = val;
                 t->sectors_written += 1;
                 return 1;
             }
@@ -713,7 +736,7 @@ static int kv_query(or_table *t, or_level *L, int lvl, uint64_t key, uint64_t *v
             account_kv_range(t, lvl, base + s0, base + s1, &last);
             for (uint32_t i = s0; i < s1; i++) {
                 uint64_t *s = kv_slot(L, base + i);
-                if (s[0] == key) { *val = s[1]; return 1; }
+                if (s[0] == kv_skey(t, key)) { *val = kv_keyonly(t) ? 0 : s[1]; return 1; }
             }
             for (uint32_t i = s0; i < s1; i++)
                 if (kv_slot(L, base + i)[0] == KV_EMPTY) return 0;
@@ -740,7 +763,7 @@ static int kv_remove(or_table *t, or_level *L, int lvl, uint64_t key)
             account_kv_range(t, lvl, base + s0, base + s1, &last);
             for (uint32_t i = s0; i < s1; i++) {
                 uint64_t *s = kv_slot(L, base + i);
-                if (s[0] == key) {
+                if (s[0] == kv_skey(t, key)) {
                     s[0] = KV_TOMB;
                     t->sectors_written += 1;
                     return 1;
@@ -765,16 +788,16 @@ static int kv_remove(or_table *t, or_level *L, int lvl, uint64_t key)
  * slot scheme allocates nslots slots (single_slot_insert.cuh:89-96). */
 or_table *or_create(uint32_t rep, uint32_t nlevels, const or_level_desc *lv, uint64_t seed)
 {
-    if (nlevels < 1 || nlevels > 2 || rep > OR_REP_FP12) return NULL;
+    if (nlevels < 1 || nlevels > 2 || rep > OR_REP_KEY64) return NULL;
     for (uint32_t l = 0; l < nlevels; l++) {
         /* GROUPED levels hold `group` packed buckets per 128-byte line
          * (grouped_power_buckets.cuh:58: sizeof(bucket) * K <= 128). */
         int grouped = lv[l].scheme == OR_SCHEME_GROUPED;
         if (grouped != (lv[l].group != 0)) return NULL;
-        if (grouped && (rep == OR_REP_KV64 || lv[l].bucket_size == 0 ||
+        if (grouped && (is_kv_rep(rep) || lv[l].bucket_size == 0 ||
                         lv[l].group * bucket_bytes_of(rep, lv[l].bucket_size) > 128))
             return NULL;
-        if ((rep == OR_REP_KV64) != (lv[l].scheme <= OR_SCHEME_SINGLE_SLOT)) return NULL;
+        if (is_kv_rep(rep) != (lv[l].scheme <= OR_SCHEME_SINGLE_SLOT)) return NULL;
     }
     or_table *t = (or_table *)calloc(1, sizeof(or_table));
     if (!t) return NULL;
@@ -842,7 +865,7 @@ static int lvl_remove(or_table *t, int l, uint64_t key)
 /* Insert one key; returns 0 on failure, else 1 + level index. */
 int or_insert(or_table *t, uint64_t key, uint64_t val, int with_delete)
 {
-    if (t->rep == OR_REP_KV64) {
+    if (is_kv_rep(t->rep)) {
         for (uint32_t l = 0; l < t->nlevels; l++)
             if (kv_insert(t, &t->lv[l], (int)l, key, val)) return 1 + (int)l;
         return 0;
@@ -859,7 +882,7 @@ int or_insert(or_table *t, uint64_t key, uint64_t val, int with_delete)
 
 int or_query(or_table *t, uint64_t key, uint64_t *val)
 {
-    if (t->rep == OR_REP_KV64) {
+    if (is_kv_rep(t->rep)) {
         for (uint32_t l = 0; l < t->nlevels; l++)
             if (kv_query(t, &t->lv[l], (int)l, key, val)) return 1;
         return 0;
@@ -873,7 +896,7 @@ int or_query(or_table *t, uint64_t key, uint64_t *val)
 
 int or_remove(or_table *t, uint64_t key)
 {
-    if (t->rep == OR_REP_KV64) {
+    if (is_kv_rep(t->rep)) {
         for (uint32_t l = 0; l < t->nlevels; l++) {
             if (t->lv[l].d.scheme == OR_SCHEME_SINGLE_SLOT) return -1; /* no remove in the reference */
             if (kv_remove(t, &t->lv[l], (int)l, key)) return 1;
@@ -941,7 +964,7 @@ int or_insert_if_not_exists(or_table *t, uint64_t key, uint64_t val, int del, in
 {
     *found = 0;
     *val_out = 0;
-    if (t->rep == OR_REP_KV64) return -1; /* bucket_insert has no upsert */
+    if (is_kv_rep(t->rep)) return -1; /* bucket_insert has no upsert */
     if (t->rep == OR_REP_FP12) return -1; /* not offered for the 12-bit bucket */
     uint32_t word = or_pack(kbits_of(t), vbits_of(t), fp_of(t->rep, key), (uint32_t)val);
     for (uint32_t l = 0; l < t->nlevels; l++) {
@@ -1019,9 +1042,9 @@ uint64_t or_fill(const or_table *t)
     uint64_t f = 0;
     for (uint32_t l = 0; l < t->nlevels; l++) {
         const or_level *L = &t->lv[l];
-        if (t->rep == OR_REP_KV64) {
+        if (is_kv_rep(t->rep)) {
             for (uint64_t s = 0; s < L->nslot_alloc; s++) {
-                uint64_t k = ((const uint64_t *)L->slots)[2 * s];
+                uint64_t k = ((const uint64_t *)L->slots)[(kv_keyonly(t) ? 1 : 2) * s];
                 if (k != KV_EMPTY && k != KV_TOMB) f++;
             }
             continue;

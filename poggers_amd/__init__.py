@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(_HERE, "libpoggers_amd.so")
 
 # enum values of include/poggers_amd.h
 PG_OK = 0
-REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12 = 0, 1, 2, 3, 4
+REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12, REP_KEY64 = 0, 1, 2, 3, 4, 5
 SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS, SCHEME_GROUPED = 0, 1, 2, 3, 4
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
@@ -216,6 +216,14 @@ def kv_desc(nslots: int, bucket_size: int = 1, tile: int = 4, prober: int = PROB
     if backing is not None:
         levels.append(backing)
     return TableDesc(REP_KV64, levels)
+
+
+def key_only_desc(nslots: int, bucket_size: int = 4, tile: int = 1, prober: int = PROBE_DOUBLE,
+                  hasher: int = HASH_MURMUR64A, max_probes: int = 20) -> TableDesc:
+    """Key-only table of 8-byte keys (dynamic_container<key_container, uint64_t>);
+    the defaults are the CMS allocator's free table (allocators/cms.cuh:203:
+    tile 1, 4-slot buckets, bucket_insert, double hashing, 20 probes)."""
+    return TableDesc(REP_KEY64, [Level(nslots, SCHEME_BUCKET, bucket_size, tile, prober, hasher, max_probes)])
 
 
 class Table:

@@ -32,10 +32,10 @@ int check_level(uint32_t rep, const pg_level_desc &l, uint32_t index) {
     if ((l.prober == PG_PROBE_P2 || l.prober == PG_PROBE_XOR_P2 || l.prober == PG_PROBE_SPLIT64) &&
         l.max_probes != 2)
         return PG_ERR_INVALID;
-    if (rep == PG_REP_KV64) {
+    if (rep == PG_REP_KV64 || rep == PG_REP_KEY64) {
         if (l.scheme != PG_SCHEME_BUCKET && l.scheme != PG_SCHEME_SINGLE_SLOT) return PG_ERR_UNSUPPORTED;
         if (l.scheme == PG_SCHEME_BUCKET && (l.bucket_size == 0 || l.bucket_size > 64)) return PG_ERR_INVALID;
-    } else if (rep <= PG_REP_FP12) {
+    } else if (rep <= PG_REP_FP12) {  // the packed filter slots
         if (l.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT && l.scheme != PG_SCHEME_LINEAR_BUCKETS &&
             l.scheme != PG_SCHEME_GROUPED_SHORTCUT)
             return PG_ERR_UNSUPPORTED;
@@ -186,7 +186,8 @@ int pg_set_mode(pg_handle h, uint32_t mode) {
 }
 
 static int run_op(pg_handle h, pg::Op op, const pg::OpArgs &a, hipStream_t s) {
-    if (h->dev.rep == PG_REP_KV64) return pg::launch_kv(h->dev, h->desc.level[0].tile, op, a, s);
+    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_KEY64)
+        return pg::launch_kv(h->dev, h->desc.level[0].tile, op, a, s);
     return pg::launch_tcf(h->dev, h->desc.level[0].tile, op, a, s);
 }
 
@@ -216,7 +217,8 @@ int pg_bulk_insert_with_delete(pg_handle h, const uint64_t *d_keys, const void *
     if (!h) return PG_ERR_INVALID;
     // bucket_insert / single_slot_insert have no insert_with_delete, and
     // twelve_bucket's does not compile (12_bit_bucket.cuh:148-203).
-    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_FP12) return PG_ERR_UNSUPPORTED;
+    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_KEY64 || h->dev.rep == PG_REP_FP12)
+        return PG_ERR_UNSUPPORTED;
     return do_insert(h, d_keys, d_vals, n, d_status, d_fail_count, stream, 1);
 }
 
@@ -225,7 +227,8 @@ static int do_upsert(pg_handle h, const uint64_t *d_keys, const void *d_vals, ui
     if (!h || (n && !d_keys)) return PG_ERR_INVALID;
     // bucket_insert / single_slot_insert have no insert_if_not_exists; not
     // offered for the 12-bit bucket (its _delete form needs remove).
-    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_FP12) return PG_ERR_UNSUPPORTED;
+    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_KEY64 || h->dev.rep == PG_REP_FP12)
+        return PG_ERR_UNSUPPORTED;
     pg::OpArgs a{};
     a.keys = d_keys;
     a.vals = d_vals;

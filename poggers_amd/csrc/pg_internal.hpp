@@ -122,6 +122,7 @@ struct LevelDev {
 __host__ __device__ inline uint64_t bucket_bytes(uint32_t rep, uint32_t B) {
     if (rep == PG_REP_FP12) return 2ull * ((12ull * B - 1) / 16 + 1);
     if (rep == PG_REP_KV64) return 16ull * B;
+    if (rep == PG_REP_KEY64) return 8ull * B;
     return (uint64_t)B * (rep == PG_REP_FP16_V16 ? 4u : 2u);
 }
 

@@ -1,61 +1,61 @@
-// pg_kv.hip -- bulk kernels for key_val_pair<u64,u64> tables (device ops in
-// pg_kv_ops.hpp).
+// pg_kv.hip -- bulk kernels for key_val_pair<u64,u64> tables and key-only
+// tables (device ops in pg_kv_ops.hpp).
 #include "pg_kv_ops.hpp"
 
 namespace pg {
 
 constexpr int kBlock = 256;
 
-template <int P, bool ACC>
+template <int P, bool ACC, class SL>
 __global__ __launch_bounds__(kBlock) void k_kv_insert(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
     const bool act = tid < a.n;
     int st = 1;
     if (act) {
-        const uint64_t val = a.vals ? ((const uint64_t *)a.vals)[tid] : 0ull;
-        st = kv_insert_key<P>(T, a.keys[tid], val, t);
+        const uint64_t val = a.vals ? (uint64_t)((const typename SL::V *)a.vals)[tid] : 0ull;
+        st = kv_insert_key<P, SL>(T, a.keys[tid], val, t);
         if (a.flags && t.leader()) a.flags[tid] = (uint8_t)st;
     }
     wave_count(a.fail_count, act && t.leader() && st == 0);
     if constexpr (ACC) flush_acc(a.stats, t, act);
 }
 
-template <int P, bool ACC>
+template <int P, bool ACC, class SL>
 __global__ __launch_bounds__(kBlock) void k_kv_query(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
     if (tid < a.n) {
         uint64_t val = 0;
         bool own = false;
-        const bool found = kv_query_key<P>(T, a.keys[tid], t, val, own);
+        const bool found = kv_query_key<P, SL>(T, a.keys[tid], t, val, own);
         const bool writer = found ? own : t.leader();
         if (writer) {
             if (a.flags) a.flags[tid] = found ? 1 : 0;
-            if (a.vals_out) ((uint64_t *)a.vals_out)[tid] = found ? val : 0ull;
+            if (a.vals_out) ((typename SL::V *)a.vals_out)[tid] = found ? (typename SL::V)val : 0;
         }
     }
     if constexpr (ACC) flush_acc(a.stats, t, tid < a.n);
 }
 
-template <int P, bool ACC>
+template <int P, bool ACC, class SL>
 __global__ __launch_bounds__(kBlock) void k_kv_remove(TableDev T, OpArgs a) {
     Tile<P> t;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
     if (tid < a.n) {
-        const bool ok = kv_remove_key<P>(T, a.keys[tid], t);
+        const bool ok = kv_remove_key<P, SL>(T, a.keys[tid], t);
         if (a.flags && t.leader()) a.flags[tid] = ok ? 1 : 0;
     }
     if constexpr (ACC) flush_acc(a.stats, t, tid < a.n);
 }
 
-template <int P>
+template <int P, class SL>
 __global__ __launch_bounds__(64) void k_kv_insert_seq(TableDev T, OpArgs a) {
     Tile<P> t;
     uint64_t fails = 0;
     for (uint64_t i = 0; i < a.n; i++) {
-        const uint64_t val = a.vals ? ((const uint64_t *)a.vals)[i] : 0ull;
-        const int st = kv_insert_key<P>(T, a.keys[i], val, t);
+        const uint64_t val = a.vals ? (uint64_t)((const typename SL::V *)a.vals)[i] : 0ull;
+        const int st = kv_insert_key<P, SL>(T, a.keys[i], val, t);
         if (a.flags && t.leader()) a.flags[i] = (uint8_t)st;
         fails += (st == 0);
         fence_agent();
@@ -64,37 +64,43 @@ __global__ __launch_bounds__(64) void k_kv_insert_seq(TableDev T, OpArgs a) {
     flush_acc(a.stats, t, true);
 }
 
-template <int P>
+template <int P, class SL>
 __global__ __launch_bounds__(64) void k_kv_remove_seq(TableDev T, OpArgs a) {
     Tile<P> t;
     for (uint64_t i = 0; i < a.n; i++) {
-        const bool ok = kv_remove_key<P>(T, a.keys[i], t);
+        const bool ok = kv_remove_key<P, SL>(T, a.keys[i], t);
         if (a.flags && t.leader()) a.flags[i] = ok ? 1 : 0;
         fence_agent();
     }
     flush_acc(a.stats, t, true);
 }
 
-template <int P, bool ACC>
+template <int P, bool ACC, class SL>
 static void launch_kv_conc(Op op, unsigned grid, const TableDev &T, const OpArgs &a, hipStream_t s) {
-    if (op == Op::Query) hipLaunchKernelGGL((k_kv_query<P, ACC>), dim3(grid), dim3(kBlock), 0, s, T, a);
-    else if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert<P, ACC>), dim3(grid), dim3(kBlock), 0, s, T, a);
-    else hipLaunchKernelGGL((k_kv_remove<P, ACC>), dim3(grid), dim3(kBlock), 0, s, T, a);
+    if (op == Op::Query) hipLaunchKernelGGL((k_kv_query<P, ACC, SL>), dim3(grid), dim3(kBlock), 0, s, T, a);
+    else if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert<P, ACC, SL>), dim3(grid), dim3(kBlock), 0, s, T, a);
+    else hipLaunchKernelGGL((k_kv_remove<P, ACC, SL>), dim3(grid), dim3(kBlock), 0, s, T, a);
+}
+
+template <int P, class SL>
+static int launch_kv_ps(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
+    if (a.n == 0) return PG_OK;
+    const unsigned grid = (unsigned)((a.n * (uint64_t)P + kBlock - 1) / kBlock);
+    if (op != Op::Query && a.deterministic) {
+        if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert_seq<P, SL>), dim3(1), dim3(P), 0, s, T, a);
+        else hipLaunchKernelGGL((k_kv_remove_seq<P, SL>), dim3(1), dim3(P), 0, s, T, a);
+    } else if (a.stats) {
+        launch_kv_conc<P, true, SL>(op, grid, T, a, s);
+    } else {
+        launch_kv_conc<P, false, SL>(op, grid, T, a, s);
+    }
+    return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
 }
 
 template <int P>
 static int launch_kv_p(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
-    if (a.n == 0) return PG_OK;
-    const unsigned grid = (unsigned)((a.n * (uint64_t)P + kBlock - 1) / kBlock);
-    if (op != Op::Query && a.deterministic) {
-        if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert_seq<P>), dim3(1), dim3(P), 0, s, T, a);
-        else hipLaunchKernelGGL((k_kv_remove_seq<P>), dim3(1), dim3(P), 0, s, T, a);
-    } else if (a.stats) {
-        launch_kv_conc<P, true>(op, grid, T, a, s);
-    } else {
-        launch_kv_conc<P, false>(op, grid, T, a, s);
-    }
-    return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
+    if (T.rep == PG_REP_KEY64) return launch_kv_ps<P, SlotKey>(T, op, a, s);
+    return launch_kv_ps<P, SlotKV>(T, op, a, s);
 }
 
 int launch_kv(const TableDev &T, uint32_t tile, Op op, const OpArgs &a, hipStream_t s) {

@@ -1,8 +1,11 @@
 // pg_kv_ops.hpp -- per-key device ops for key_val_pair<u64,u64>
-// tables (representations/key_val_pair.cuh:63-171), gfx950 wave64.
+// tables (representations/key_val_pair.cuh:63-171) and for the key-only
+// dynamic_container<key_container, uint64_t> tables (the CMS allocator's
+// free table, allocators/cms.cuh:203), gfx950 wave64.
 //
-// Layout in HBM: 16 B slots {key, val}, B contiguous slots per bucket, bucket
-// b at slot b*B.  Empty key 0, tombstone key ~0.
+// Layout in HBM: 16 B slots {key, val} (SlotKV) or 8 B keys (SlotKey), B
+// contiguous slots per bucket, bucket b at slot b*B.  Empty key 0,
+// tombstone key ~0.
 //
 // Two tile shapes, chosen per table:
 //   * stripe mode (B > 1): lane r of a P-lane tile owns slot stripe*P + r of
@@ -24,7 +27,37 @@ namespace pg {
 constexpr uint64_t kKvEmpty = 0ULL;
 constexpr uint64_t kKvTomb = ~0ULL;
 
-__device__ inline ulonglong2 load_slot(const ulonglong2 *p) { return *p; }
+// Slot layouts.  Probing always hashes the key as given; the stored key is
+// stored(key): the key itself for key_val_pair, get_smallKey(key) for the
+// key-only container (0 -> 1, ~0 -> ~0 - 1, dynamic_container.cuh:35-50).
+struct SlotKV {
+    using S = ulonglong2;
+    using V = uint64_t;  // value buffer type
+    __device__ static uint64_t stored(uint64_t key) { return key; }
+    __device__ static S load(const S *p) { return *p; }
+    __device__ static S none() { return make_ulonglong2(kKvTomb, 0); }
+    __device__ static uint64_t key(const S &s) { return s.x; }
+    __device__ static uint64_t val(const S &s) { return s.y; }
+    __device__ static unsigned long long *key_ptr(S *p) { return (unsigned long long *)&p->x; }
+    // key_val_pair::atomic_swap: CAS key 0 -> key, then store the value
+    // (key_val_pair.cuh:84-95)
+    __device__ static void store_val(S *p, uint64_t v) { p->y = v; }
+};
+struct SlotKey {
+    using S = uint64_t;
+    using V = uint16_t;  // the CMS table's Val; queries report 0
+    __device__ static uint64_t stored(uint64_t key) {
+        key += (key == kKvEmpty);
+        key -= (key == kKvTomb);
+        return key;
+    }
+    __device__ static S load(const S *p) { return *p; }
+    __device__ static S none() { return kKvTomb; }
+    __device__ static uint64_t key(const S &s) { return s; }
+    __device__ static uint64_t val(const S &) { return 0; }
+    __device__ static unsigned long long *key_ptr(S *p) { return (unsigned long long *)p; }
+    __device__ static void store_val(S *, uint64_t) {}
+};
 
 // Lanes strictly below the lowest set bit of `endm` (all lanes when 0).
 __device__ inline uint64_t below_first(uint64_t endm) {
@@ -35,32 +68,34 @@ __device__ inline uint64_t below_first(uint64_t endm) {
 // insert (single_slot_insert.cuh:125-166): lowest empty slot in probe order;
 // key_val_pair::atomic_swap = CAS key 0 -> key, then store the value
 // (key_val_pair.cuh:84-95).
-template <int P>
+template <int P, class SL = SlotKV>
 __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
+    using S = typename SL::S;
     Probe pr;
     pr.begin(L, key, key);
     pr.max_probes = pr.useful_probes(L.nb.d);
-    ulonglong2 *slots = (ulonglong2 *)L.slots;
+    const uint64_t sk = SL::stored(key);
+    S *slots = (S *)L.slots;
     if (P > 1 && L.B == 1) {
         for (uint32_t j0 = 0;; j0 += P) {
             const uint64_t p = pr.at(j0 + t.rank);
             const bool end = (p == kEnd);
             uint64_t si = 0;
-            ulonglong2 s = make_ulonglong2(kKvTomb, 0);
+            S s = SL::none();
             if (!end) {
                 si = fastmod(p, L.nb);
-                s = load_slot(slots + si);
+                s = SL::load(slots + si);
             }
             const uint64_t endm = t.bits(end);
-            bool cand = !end && s.x == kKvEmpty;
+            bool cand = !end && SL::key(s) == kKvEmpty;
             uint64_t cm = t.bits(cand) & below_first(endm);
             t.acc[0] += cm ? (uint32_t)ctz64(cm) + 1 : (endm ? (uint32_t)ctz64(endm) : P);
             while (cm) {
                 const uint32_t l = (uint32_t)ctz64(cm);
                 bool success = false;
                 if (t.rank == l) {
-                    success = atomicCAS((unsigned long long *)&slots[si].x, kKvEmpty, key) == kKvEmpty;
-                    if (success) slots[si].y = val;
+                    success = atomicCAS(SL::key_ptr(slots + si), kKvEmpty, sk) == kKvEmpty;
+                    if (success) SL::store_val(slots + si, val);
                 }
                 if (t.any(success)) {
                     t.acc[2]++;
@@ -78,15 +113,15 @@ __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, c
         for (uint32_t s0 = 0; s0 < L.B; s0 += P) {
             const uint32_t i = s0 + t.rank;
             const bool ok = i < L.B;
-            const ulonglong2 s = ok ? load_slot(slots + base + i) : make_ulonglong2(kKvTomb, 0);
+            const S s = ok ? SL::load(slots + base + i) : SL::none();
             t.acc[0] += (L.B - s0 < P) ? L.B - s0 : P;
-            uint64_t cm = t.bits(ok && s.x == kKvEmpty);
+            uint64_t cm = t.bits(ok && SL::key(s) == kKvEmpty);
             while (cm) {
                 const uint32_t l = (uint32_t)ctz64(cm);
                 bool success = false;
                 if (t.rank == l) {
-                    success = atomicCAS((unsigned long long *)&slots[base + i].x, kKvEmpty, key) == kKvEmpty;
-                    if (success) slots[base + i].y = val;
+                    success = atomicCAS(SL::key_ptr(slots + base + i), kKvEmpty, sk) == kKvEmpty;
+                    if (success) SL::store_val(slots + base + i, val);
                 }
                 if (t.any(success)) {
                     t.acc[2]++;
@@ -100,30 +135,32 @@ __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, c
 
 // bucket_insert::query (bucket_insert.cuh:295-357), single_slot_insert::query
 // (single_slot_insert.cuh:168-229).
-template <int P>
+template <int P, class SL = SlotKV>
 __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t, uint64_t &val,
                                bool &owner) {
+    using S = typename SL::S;
     Probe pr;
     pr.begin(L, key, key);
     pr.max_probes = pr.useful_probes(L.nb.d);
-    const ulonglong2 *slots = (const ulonglong2 *)L.slots;
+    const uint64_t sk = SL::stored(key);
+    const S *slots = (const S *)L.slots;
     if (P > 1 && L.B == 1) {
         for (uint32_t j0 = 0;; j0 += P) {
             const uint64_t p = pr.at(j0 + t.rank);
             const bool end = (p == kEnd);
-            ulonglong2 s = make_ulonglong2(kKvTomb, 0);
-            if (!end) s = load_slot(slots + fastmod(p, L.nb));
+            S s = SL::none();
+            if (!end) s = SL::load(slots + fastmod(p, L.nb));
             const uint64_t endm = t.bits(end);
             const uint64_t lim = below_first(endm);
-            const uint64_t mm = t.bits(!end && s.x == key) & lim;
-            const uint64_t em = t.bits(!end && s.x == kKvEmpty) & lim;
+            const uint64_t mm = t.bits(!end && SL::key(s) == sk) & lim;
+            const uint64_t em = t.bits(!end && SL::key(s) == kKvEmpty) & lim;
             const uint64_t dec = mm | em;
             t.acc[0] += dec ? (uint32_t)ctz64(dec) + 1 : (endm ? (uint32_t)ctz64(endm) : P);
             if (dec) {
                 const uint32_t l = (uint32_t)ctz64(dec);
                 if ((mm >> l) & 1ull) {
                     owner = (t.rank == l);
-                    if (owner) val = s.y;
+                    if (owner) val = SL::val(s);
                     return true;
                 }
                 return false;
@@ -138,16 +175,16 @@ __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t
         for (uint32_t s0 = 0; s0 < L.B; s0 += P) {
             const uint32_t i = s0 + t.rank;
             const bool ok = i < L.B;
-            const ulonglong2 s = ok ? load_slot(slots + base + i) : make_ulonglong2(kKvTomb, 0);
+            const S s = ok ? SL::load(slots + base + i) : SL::none();
             t.acc[0] += (L.B - s0 < P) ? L.B - s0 : P;
-            const uint64_t mm = t.bits(ok && s.x == key);
+            const uint64_t mm = t.bits(ok && SL::key(s) == sk);
             if (mm) {
                 const uint32_t l = (uint32_t)ctz64(mm);
                 owner = (t.rank == l);
-                if (owner) val = s.y;
+                if (owner) val = SL::val(s);
                 return true;
             }
-            if (t.any(ok && s.x == kKvEmpty)) return false;
+            if (t.any(ok && SL::key(s) == kKvEmpty)) return false;
         }
     }
 }
@@ -155,26 +192,28 @@ __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t
 // bucket_insert::remove (bucket_insert.cuh:205-293): CAS the lowest match to
 // the tombstone key (key_val_pair.cuh:143-151); a stripe with an empty slot
 // and no successful removal ends the search.
-template <int P>
+template <int P, class SL = SlotKV>
 __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
+    using S = typename SL::S;
     Probe pr;
     pr.begin(L, key, key);
     pr.max_probes = pr.useful_probes(L.nb.d);
-    ulonglong2 *slots = (ulonglong2 *)L.slots;
+    const uint64_t sk = SL::stored(key);
+    S *slots = (S *)L.slots;
     if (P > 1 && L.B == 1) {
         for (uint32_t j0 = 0;; j0 += P) {
             const uint64_t p = pr.at(j0 + t.rank);
             const bool end = (p == kEnd);
             uint64_t si = 0;
-            ulonglong2 s = make_ulonglong2(kKvTomb, 0);
+            S s = SL::none();
             if (!end) {
                 si = fastmod(p, L.nb);
-                s = load_slot(slots + si);
+                s = SL::load(slots + si);
             }
             const uint64_t endm = t.bits(end);
             const uint64_t lim = below_first(endm);
-            uint64_t mm = t.bits(!end && s.x == key) & lim;
-            const uint64_t em = t.bits(!end && s.x == kKvEmpty) & lim;
+            uint64_t mm = t.bits(!end && SL::key(s) == sk) & lim;
+            const uint64_t em = t.bits(!end && SL::key(s) == kKvEmpty) & lim;
             // Walk decisive lanes in probe order: matches are tried, the first
             // empty ends the search.
             uint64_t dec = mm | em;
@@ -184,7 +223,7 @@ __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &
                 if (!((mm >> l) & 1ull)) return false;
                 bool success = false;
                 if (t.rank == l)
-                    success = atomicCAS((unsigned long long *)&slots[si].x, key, kKvTomb) == key;
+                    success = atomicCAS(SL::key_ptr(slots + si), sk, kKvTomb) == sk;
                 if (t.any(success)) {
                     t.acc[2]++;
                     return true;
@@ -201,44 +240,44 @@ __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &
         for (uint32_t s0 = 0; s0 < L.B; s0 += P) {
             const uint32_t i = s0 + t.rank;
             const bool ok = i < L.B;
-            const ulonglong2 s = ok ? load_slot(slots + base + i) : make_ulonglong2(kKvTomb, 0);
-            const bool match = ok && s.x == key;
+            const S s = ok ? SL::load(slots + base + i) : SL::none();
+            const bool match = ok && SL::key(s) == sk;
             t.acc[0] += (L.B - s0 < P) ? L.B - s0 : P;
             uint64_t mm = t.bits(match);
             while (mm) {
                 const uint32_t l = (uint32_t)ctz64(mm);
                 bool success = false;
                 if (t.rank == l)
-                    success = atomicCAS((unsigned long long *)&slots[base + i].x, key, kKvTomb) == key;
+                    success = atomicCAS(SL::key_ptr(slots + base + i), sk, kKvTomb) == sk;
                 if (t.any(success)) {
                     t.acc[2]++;
                     return true;
                 }
                 mm &= mm - 1ull;
             }
-            if (t.any(ok && !match && s.x == kKvEmpty)) return false;
+            if (t.any(ok && !match && SL::key(s) == kKvEmpty)) return false;
         }
     }
 }
 
 // Whole-table ops: level 0, then the backing level on false
 // (tables/base_table.cuh:167-308).
-template <int P>
+template <int P, class SL = SlotKV>
 __device__ int kv_insert_key(const TableDev &T, uint64_t key, uint64_t val, const Tile<P> &t) {
     for (uint32_t l = 0; l < T.nlevels; l++)
-        if (kv_level_insert<P>(T.lv[l], key, val, t)) return 1 + (int)l;
+        if (kv_level_insert<P, SL>(T.lv[l], key, val, t)) return 1 + (int)l;
     return 0;
 }
-template <int P>
+template <int P, class SL = SlotKV>
 __device__ bool kv_query_key(const TableDev &T, uint64_t key, const Tile<P> &t, uint64_t &val, bool &owner) {
     for (uint32_t l = 0; l < T.nlevels; l++)
-        if (kv_level_query<P>(T.lv[l], key, t, val, owner)) return true;
+        if (kv_level_query<P, SL>(T.lv[l], key, t, val, owner)) return true;
     return false;
 }
-template <int P>
+template <int P, class SL = SlotKV>
 __device__ bool kv_remove_key(const TableDev &T, uint64_t key, const Tile<P> &t) {
     for (uint32_t l = 0; l < T.nlevels; l++)
-        if (kv_level_remove<P>(T.lv[l], key, t)) return true;
+        if (kv_level_remove<P, SL>(T.lv[l], key, t)) return true;
     return false;
 }
 

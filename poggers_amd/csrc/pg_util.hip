@@ -61,6 +61,16 @@ __global__ __launch_bounds__(kBlock) void k_fill_twelve(LevelDev L, uint64_t *co
     if (__lane_id() == 0 && c) atomicAdd((unsigned long long *)counter, (unsigned long long)c);
 }
 
+__global__ __launch_bounds__(kBlock) void k_fill_key64(const uint64_t *s, uint64_t n, uint64_t *counter) {
+    uint64_t c = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
+        const uint64_t k = s[i];
+        c += (k != 0ull && k != ~0ull);
+    }
+    c = wave_sum(c);
+    if (__lane_id() == 0 && c) atomicAdd((unsigned long long *)counter, (unsigned long long)c);
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_kv(const ulonglong2 *s, uint64_t n, uint64_t *counter) {
     uint64_t c = 0;
     for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
@@ -88,6 +98,9 @@ int launch_fill(const TableDev &T, uint64_t *d_counter, hipStream_t s) {
         const uint64_t bytes = L.group ? ((L.nb.d - 1) / L.group + 1) * 128 : L.nb.d * (uint64_t)L.bucket_bytes;
         if (T.rep == PG_REP_FP12)
             hipLaunchKernelGGL(k_fill_twelve, dim3(stride_grid(L.nb.d)), dim3(kBlock), 0, s, L, d_counter);
+        else if (T.rep == PG_REP_KEY64)
+            hipLaunchKernelGGL(k_fill_key64, dim3(stride_grid(nslots)), dim3(kBlock), 0, s,
+                               (const uint64_t *)L.slots, nslots, d_counter);
         else if (T.rep == PG_REP_KV64)
             hipLaunchKernelGGL(k_fill_kv, dim3(stride_grid(nslots)), dim3(kBlock), 0, s,
                                (const ulonglong2 *)L.slots, nslots, d_counter);
