@@ -207,12 +207,19 @@ def run_serve(args):
             reqs.append((t, op, kt, vals, out, flags, n))
         return reqs
 
+    def packed_of(reqs):
+        return pg.Server.pack([(t, op, kt, vals, out, flags) for (t, op, kt, vals, out, flags, n) in reqs])
+
     def saturated(srv, reqs):
+        """All requests offered at once, handed over 64 at a time while the
+        engine runs (submission overlaps execution)."""
+        packed = packed_of(reqs)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for (t, op, kt, vals, out, flags, n) in reqs:
-            srv.submit(t, op, kt, vals, out, flags)
         done = 0
+        for s0 in range(0, len(reqs), 64):
+            srv.submit_many(packed, s0, min(64, len(reqs) - s0))
+            done += len(srv.progress())
         while done < len(reqs):
             done += len(srv.progress())
         t1 = time.perf_counter()
@@ -225,18 +232,22 @@ def run_serve(args):
         return time.perf_counter() - t0, sum(r[6] for r in reqs)
 
     def poisson(srv, reqs, rate, seed):
+        """Open-loop arrivals; each engine step first hands over every request
+        that has arrived."""
         r = np.random.default_rng(seed)
         arrive = np.cumsum(r.exponential(1.0 / rate, len(reqs)))
         lat = np.zeros(len(reqs))
+        packed = packed_of(reqs)
         tick2i = {}
         i = done = 0
         t0 = time.perf_counter()
         while done < len(reqs):
             now = time.perf_counter() - t0
-            while i < len(reqs) and arrive[i] <= now:
-                (t, op, kt, vals, out, flags, n) = reqs[i]
-                tick2i[srv.submit(t, op, kt, vals, out, flags)] = i
-                i += 1
+            j = int(np.searchsorted(arrive, now, side="right"))
+            if j > i:
+                for k, tk in enumerate(srv.submit_many(packed, i, j - i)):
+                    tick2i[tk] = i + k
+                i = j
             fin = srv.progress()
             if fin:
                 now = time.perf_counter() - t0

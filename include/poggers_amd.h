@@ -293,12 +293,16 @@ int pg_batch_bytes(pg_batch b, uint64_t *out);
 
 /* ---- serving: continuous batching over resident tables ----
  * No reference counterpart (the reference has no host request path).
- * Requests carry HOST buffers (pinned with hipHostMalloc for asynchronous
- * copies).  pg_server_progress never blocks: it reports finished requests,
- * admits queued requests in FIFO order while their staging bytes fit the
- * device budget and a stream is free, and launches each admitted group as
- * one batch (copies in, one persistent kernel over all its requests, copies
- * out).  Up to `nstreams` batches overlap on the device.  Every served table
+ * Requests carry host or device buffers.  Buffers the GPU can address
+ * (pinned with hipHostMalloc, or device memory) are used in place: the
+ * kernel reads keys and writes results through them, with no copy calls and
+ * no staging bytes.  Pageable buffers are staged through the device budget
+ * with copies in and out.  pg_server_progress never blocks: it reports
+ * finished requests, admits queued requests in FIFO order while their
+ * staging bytes fit the budget and a stream is free, and launches each
+ * admitted group as one batch (one copy of the request table, one kernel
+ * with a tile per key over all its requests, plus the staged requests'
+ * copies).  Up to `nstreams` batches overlap on the device.  Every served table
  * must use the same tile width (at most 16).  Requests that touch the same
  * keys of one table from different in-flight batches are not ordered with
  * each other (as concurrent bulk calls on two streams would not be). */
@@ -310,15 +314,33 @@ int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_
 int pg_server_destroy(pg_server s);
 /* Queue a request; PG_ERR_NOMEM when it could never fit the budget.
  * h_vals: insert values (uint64 KV / uint16 filter) or NULL; h_vals_out:
- * query values or NULL; h_flags: per-key status / found or NULL. */
+ * query values or NULL; h_flags: per-key status / found or NULL.  In-place
+ * buffers must stay valid and untouched until the request is reported. */
 int pg_server_submit(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,
                      uint64_t n, void *h_vals_out, uint8_t *h_flags, uint64_t *ticket);
+/* One request of pg_server_submit_many (the arguments of pg_server_submit). */
+typedef struct pg_request {
+    uint32_t table;
+    uint32_t op;            /* enum pg_op */
+    const uint64_t *keys;
+    const void *vals;
+    uint64_t n;
+    void *vals_out;
+    uint8_t *flags;
+} pg_request;
+/* Queue `count` requests in order, as that many pg_server_submit calls:
+ * tickets[i] (if not NULL) receives request i's ticket.  On an invalid
+ * request it stops and returns that request's error; *nqueued tells how
+ * many were queued before it. */
+int pg_server_submit_many(pg_server s, const pg_request *reqs, uint32_t count, uint64_t *tickets,
+                          uint32_t *nqueued);
 /* Engine step: writes up to `cap` finished tickets to done[]. */
 int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndone);
 int pg_server_pending(pg_server s, uint64_t *queued, uint64_t *inflight);
-/* Batch formation: at most max_batch_keys keys per batch (a larger single
- * request still forms its own batch) and `chunk` keys per work item. */
-int pg_server_set_batch(pg_server s, uint64_t max_batch_keys, uint32_t chunk);
+/* Batch formation: at most max_batch_keys keys (a larger single request
+ * still forms its own batch) and max_batch_requests (1..4096) requests per
+ * batch. */
+int pg_server_set_batch(pg_server s, uint64_t max_batch_keys, uint32_t max_batch_requests);
 /* One request alone: copies in, the table's bulk kernel, copies out, wait.
  * The per-request baseline; only when nothing is queued or in flight. */
 int pg_server_run_plain(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,

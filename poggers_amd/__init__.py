@@ -39,7 +39,7 @@ EXPORTS = [
     "pg_batch_insert", "pg_batch_query", "pg_batch_capture", "pg_batch_replay", "pg_batch_fill",
     "pg_batch_level_ptr", "pg_batch_bytes",
     "pg_server_create", "pg_server_destroy", "pg_server_submit", "pg_server_progress", "pg_server_pending",
-    "pg_server_run_plain", "pg_server_set_batch", "pg_save", "pg_load", "pg_file_info",
+    "pg_server_run_plain", "pg_server_set_batch", "pg_server_submit_many", "pg_save", "pg_load", "pg_file_info",
 ]
 OP_INSERT, OP_QUERY, OP_REMOVE = 0, 1, 2
 
@@ -59,6 +59,19 @@ class LevelDesc(ctypes.Structure):
         ("max_probes", ctypes.c_uint32),
         ("group", ctypes.c_uint32),
         ("reserved", ctypes.c_uint32),
+    ]
+
+
+class Request(ctypes.Structure):
+    """pg_request (include/poggers_amd.h)."""
+    _fields_ = [
+        ("table", ctypes.c_uint32),
+        ("op", ctypes.c_uint32),
+        ("keys", ctypes.c_void_p),
+        ("vals", ctypes.c_void_p),
+        ("n", ctypes.c_uint64),
+        ("vals_out", ctypes.c_void_p),
+        ("flags", ctypes.c_void_p),
     ]
 
 
@@ -119,6 +132,7 @@ def lib():
     L.pg_server_pending.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     L.pg_server_run_plain.argtypes = [vp, u32, u32, vp, vp, u64, vp, vp]
     L.pg_server_set_batch.argtypes = [vp, u64, u32]
+    L.pg_server_submit_many.argtypes = [vp, vp, u32, vp, vp]
     L.pg_save.argtypes = [vp, ctypes.c_char_p]
     L.pg_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
     L.pg_file_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(Desc), ctypes.POINTER(u64)]
@@ -459,14 +473,41 @@ class Server:
                                      _ptr(flags), ctypes.byref(t)), "pg_server_submit")
         return t.value
 
+    @staticmethod
+    def pack(requests):
+        """A pg_request array for submit_many from (table, op, keys, vals,
+        vals_out, flags) tuples of tensors (None where absent)."""
+        arr = (Request * len(requests))()
+        for i, (table, op, keys, vals, out, flags) in enumerate(requests):
+            r = arr[i]
+            r.table, r.op, r.n = table, op, keys.numel()
+            r.keys = keys.data_ptr()
+            r.vals = vals.data_ptr() if vals is not None else None
+            r.vals_out = out.data_ptr() if out is not None else None
+            r.flags = flags.data_ptr() if flags is not None else None
+        return arr
+
+    def submit_many(self, packed, start: int = 0, count: int | None = None):
+        """Queue packed[start:start+count] in order; returns their tickets."""
+        if count is None:
+            count = len(packed) - start
+        if count <= 0:
+            return []
+        tickets = (ctypes.c_uint64 * count)()
+        nq = ctypes.c_uint32()
+        ptr = ctypes.cast(ctypes.byref(packed, start * ctypes.sizeof(Request)), ctypes.c_void_p)
+        check(lib().pg_server_submit_many(self._h, ptr, count, tickets, ctypes.byref(nq)),
+              "pg_server_submit_many")
+        return list(tickets)
+
     def progress(self):
         """One engine step; returns the tickets that finished."""
         nd = ctypes.c_uint32()
         check(lib().pg_server_progress(self._h, self._done, 65536, ctypes.byref(nd)), "pg_server_progress")
         return [self._done[i] for i in range(nd.value)]
 
-    def set_batch(self, max_batch_keys: int, chunk: int = 2048):
-        check(lib().pg_server_set_batch(self._h, max_batch_keys, chunk), "pg_server_set_batch")
+    def set_batch(self, max_batch_keys: int, max_batch_requests: int = 4096):
+        check(lib().pg_server_set_batch(self._h, max_batch_keys, max_batch_requests), "pg_server_set_batch")
 
     def pending(self):
         q, f = ctypes.c_uint64(), ctypes.c_uint64()

@@ -1,17 +1,22 @@
 // pg_serve.hip -- continuous-batching request loop over resident tables.
 //
-// Requests (table, op, n keys in pinned host memory) are queued by
+// Requests (table, op, n keys in host or device memory) are queued by
 // pg_server_submit.  pg_server_progress is the engine: it retires finished
 // batches (event query, no blocking), admits queued requests in FIFO order
 // while their staging bytes fit the device arena (the HBM budget) and a
-// stream slot is free, and launches each admitted group as one batch:
-// host-to-device copies of the keys, one persistent kernel that applies every
-// request of the batch to its own table (mixed tables, mixed ops), and
-// device-to-host copies of the results, all on one stream; up to `nstreams`
-// batches are in flight, so requests that arrive while others run overlap
-// with them on the device.  pg_server_run_plain processes one request with
-// its own copies and kernel launch and waits for it (the per-request
-// baseline).
+// stream slot is free, and launches each admitted group as one batch: one
+// copy of the batch's request table, one persistent kernel that applies
+// every request of the batch to its own table (mixed tables, mixed ops), all
+// on one stream; up to `nstreams` batches are in flight, so requests that
+// arrive while others run overlap with them on the device.
+//
+// Buffers the GPU can address (pinned host memory from hipHostMalloc, or
+// device memory) are used in place: the kernel reads the keys and writes the
+// results through them, so a request costs no copy calls and no staging
+// bytes.  Other (pageable) buffers are staged through the arena with
+// host-to-device / device-to-host copies.  pg_server_run_plain processes one
+// request with its own kernel launch (and copies, if staged) and waits for
+// it: the per-request baseline.
 //
 // No reference counterpart: the reference has no host request path.
 #include <deque>
@@ -34,14 +39,15 @@ struct ReqDev {
     uint32_t op;
 };
 
+constexpr uint32_t kIdxShift = 8;  // one index entry per 256 keys of a batch
+
 struct ServeArgs {
     const TableDev *tables;
     const ReqDev *reqs;
     const uint64_t *off;   // nreq + 1 key offsets
+    const uint32_t *idx;   // idx[c]: the request holding key c << kIdxShift (NULL: search)
     uint32_t nreq;
     uint64_t total;
-    uint64_t *counter;
-    uint32_t chunk;
 };
 
 __device__ inline uint32_t find_req(const uint64_t *off, uint32_t n, uint64_t i) {
@@ -52,10 +58,6 @@ __device__ inline uint32_t find_req(const uint64_t *off, uint32_t n, uint64_t i)
         else hi = mid;
     }
     return lo;
-}
-
-__global__ __launch_bounds__(64) void k_serve_reset(uint64_t *counter) {
-    if (threadIdx.x == 0) counter[0] = 0;
 }
 
 template <int P>
@@ -114,33 +116,30 @@ __device__ inline void serve_one(const TableDev &T, const ReqDev &r, uint64_t j,
     }
 }
 
+// One tile per key of the batch, like the single-table bulk kernels, so every
+// key's loads (its key, its buckets) are in flight at once.  A tile finds its
+// request from the batch's coarse index (one load, then a short walk over the
+// key offsets) rather than a binary search, whose dependent loads would sit
+// in front of every key.
 template <int P>
 __global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
     Tile<P> t;
-    constexpr uint32_t kTilesPerWave = 64 / P;
-    const uint32_t tile_in_wave = (threadIdx.x & 63) / P;
-    while (true) {
-        uint64_t base = 0;
-        uint32_t r0 = 0;
-        if ((threadIdx.x & 63) == 0) {
-            base = atomicAdd((unsigned long long *)a.counter, (unsigned long long)a.chunk);
-            if (base < a.total) r0 = find_req(a.off, a.nreq, base);
-        }
-        base = __shfl(base, 0, 64);
-        r0 = __shfl(r0, 0, 64);
-        if (base >= a.total) break;
-        const uint64_t end = (base + a.chunk < a.total) ? base + a.chunk : a.total;
-        uint32_t r = r0;
-        for (uint64_t i = base + tile_in_wave; i < end; i += kTilesPerWave) {
+    const uint64_t i = (uint64_t(blockIdx.x) * 256 + threadIdx.x) / P;
+    if (i < a.total) {
+        uint32_t r;
+        if (a.idx) {
+            r = a.idx[i >> kIdxShift];
             while (a.off[r + 1] <= i) r++;
-            const ReqDev &rq = a.reqs[r];
-            serve_one<P>(a.tables[rq.table], rq, i - a.off[r], t);
+        } else {
+            r = find_req(a.off, a.nreq, i);
         }
+        const ReqDev &rq = a.reqs[r];
+        serve_one<P>(a.tables[rq.table], rq, i - a.off[r], t);
     }
 }
 
-static int launch_serve(uint32_t P, const ServeArgs &a, unsigned blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_serve_reset, dim3(1), dim3(64), 0, s, a.counter);
+static int launch_serve(uint32_t P, const ServeArgs &a, hipStream_t s) {
+    const unsigned blocks = (unsigned)((a.total * P + 255) / 256);
     switch (P) {
         case 1: hipLaunchKernelGGL((k_serve<1>), dim3(blocks), dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL((k_serve<2>), dim3(blocks), dim3(256), 0, s, a); break;
@@ -161,6 +160,7 @@ namespace {
 
 constexpr uint64_t kAlign = 256;
 constexpr uint32_t kMaxBatchReqs = 4096;
+constexpr uint64_t kMaxIdx = (1ull << 24) >> pg::kIdxShift;  // index capacity: batches up to 16 M keys
 
 inline uint64_t al(uint64_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
@@ -173,8 +173,46 @@ struct ServeReq {
     void *h_vals_out;
     uint8_t *h_flags;
     uint32_t vbytes;   // value width of the table (8 KV, 2 filter)
-    uint64_t need;     // staging bytes
+    uint64_t need;     // staging bytes (0 when used in place)
+    bool in_place;     // every buffer is GPU-addressable: no staging
+    pg::ReqDev dev;    // in place: the buffers as the kernel sees them
 };
+
+// The address the GPU uses for p: device memory as is, pinned host memory
+// through its mapping; false for memory the GPU cannot address (pageable).
+bool gpu_view(const void *p, void **out) {
+    *out = nullptr;
+    if (!p) return true;
+    hipPointerAttribute_t a{};
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    if (a.type == hipMemoryTypeDevice) {
+        *out = const_cast<void *>(p);
+        return true;
+    }
+    if (a.type == hipMemoryTypeHost && a.devicePointer) {
+        const char *hp = a.hostPointer ? (const char *)a.hostPointer : (const char *)p;
+        *out = (char *)a.devicePointer + ((const char *)p - hp);
+        return true;
+    }
+    return false;
+}
+
+// Classify a request's buffers: in place when all of them are GPU-addressable.
+void classify(ServeReq &r) {
+    void *k = nullptr, *v = nullptr, *vo = nullptr, *f = nullptr;
+    r.in_place = gpu_view(r.h_keys, &k) && (r.op != PG_OP_INSERT || gpu_view(r.h_vals, &v)) &&
+                 (r.op != PG_OP_QUERY || gpu_view(r.h_vals_out, &vo)) && gpu_view(r.h_flags, &f);
+    r.dev = pg::ReqDev{};
+    if (r.in_place) {
+        r.dev.keys = (const uint64_t *)k;
+        r.dev.vals = r.op == PG_OP_INSERT ? v : nullptr;
+        r.dev.vals_out = r.op == PG_OP_QUERY ? vo : nullptr;
+        r.dev.flags = (uint8_t *)f;
+    }
+}
 
 struct ServeBatch {
     std::vector<uint64_t> tickets;
@@ -203,13 +241,11 @@ struct pg_server_s {
     pg::Ring ring;
     std::vector<hipStream_t> streams;
     std::vector<int> busy;
-    std::vector<pg::ReqDev *> h_desc;   // pinned, per stream
-    std::vector<uint64_t *> h_off;      // pinned, per stream
+    std::vector<char *> h_meta;         // pinned request table per stream: offsets, descriptors, index
     std::deque<ServeReq> queue;
     std::deque<ServeBatch> inflight;
     uint64_t next_ticket = 1;
-    unsigned blocks = 0;
-    uint32_t chunk = 2048;
+    uint32_t max_batch_reqs = kMaxBatchReqs;
     uint64_t max_batch_keys = 1ull << 22;  // bounds a batch's time on the device
 };
 
@@ -247,22 +283,17 @@ int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_
     s->ring.cap = budget_bytes;
     for (uint32_t i = 0; i < nstreams && rc == PG_OK; i++) {
         hipStream_t st;
-        pg::ReqDev *d = nullptr;
-        uint64_t *o = nullptr;
+        char *m = nullptr;
         if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
-            hipHostMalloc((void **)&d, sizeof(pg::ReqDev) * kMaxBatchReqs) != hipSuccess ||
-            hipHostMalloc((void **)&o, sizeof(uint64_t) * (kMaxBatchReqs + 1)) != hipSuccess) {
+            hipHostMalloc((void **)&m, al(8 * (kMaxBatchReqs + 1)) + al(sizeof(pg::ReqDev) * kMaxBatchReqs) +
+                                           al(4 * (kMaxIdx + 1))) != hipSuccess) {
             rc = PG_ERR_HIP;
             break;
         }
         s->streams.push_back(st);
         s->busy.push_back(0);
-        s->h_desc.push_back(d);
-        s->h_off.push_back(o);
+        s->h_meta.push_back(m);
     }
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    s->blocks = (unsigned)cus * 4u;
     if (rc != PG_OK) {
         pg_server_destroy(s);
         return rc;
@@ -277,8 +308,7 @@ int pg_server_destroy(pg_server s) {
     for (auto &b : s->inflight)
         if (b.done) hipEventDestroy(b.done);
     for (hipStream_t st : s->streams) hipStreamDestroy(st);
-    for (auto *d : s->h_desc) hipHostFree(d);
-    for (auto *o : s->h_off) hipHostFree(o);
+    for (auto *m : s->h_meta) hipHostFree(m);
     if (s->arena) hipFree(s->arena);
     if (s->d_tables) hipFree(s->d_tables);
     delete s;
@@ -302,7 +332,8 @@ int pg_server_submit(pg_server s, uint32_t table, uint32_t op, const uint64_t *h
     r.h_vals_out = h_vals_out;
     r.h_flags = h_flags;
     r.vbytes = T.rep == PG_REP_KV64 ? 8u : 2u;
-    r.need = req_bytes(r);
+    classify(r);
+    r.need = r.in_place ? 0 : req_bytes(r);
     // a request that can never fit the budget is refused at submission
     const uint64_t overhead = al(sizeof(pg::ReqDev)) + al(16) + kAlign;
     if (r.need + overhead > s->ring.cap) return PG_ERR_NOMEM;
@@ -311,83 +342,120 @@ int pg_server_submit(pg_server s, uint32_t table, uint32_t op, const uint64_t *h
     return PG_OK;
 }
 
-// Launch queued requests that fit into one batch on a free stream.
+int pg_server_submit_many(pg_server s, const pg_request *reqs, uint32_t count, uint64_t *tickets,
+                          uint32_t *nqueued) {
+    if (nqueued) *nqueued = 0;
+    if (!s || (count && !reqs)) return PG_ERR_INVALID;
+    for (uint32_t i = 0; i < count; i++) {
+        const pg_request &r = reqs[i];
+        uint64_t t = 0;
+        const int rc = pg_server_submit(s, r.table, r.op, r.keys, r.vals, r.n, r.vals_out, r.flags, &t);
+        if (rc != PG_OK) return rc;
+        if (tickets) tickets[i] = t;
+        if (nqueued) *nqueued = i + 1;
+    }
+    return PG_OK;
+}
+
+// Launch queued requests that fit into one batch on a free stream.  The
+// batch's request table (key offsets, then descriptors) is built in pinned
+// memory and copied with one call; staged requests add their own copies.
 static int launch_batch(pg_server s, uint32_t si) {
     const uint64_t free_bytes = s->ring.largest_free();
     uint64_t need = 0, keys = 0;
     size_t take = 0;
-    const uint64_t fixed = kAlign /* counter */;
-    while (take < s->queue.size() && take < kMaxBatchReqs) {
-        const uint64_t meta = al(sizeof(pg::ReqDev) * (take + 1)) + al(8 * (take + 2));
-        if (fixed + meta + need + s->queue[take].need > free_bytes) break;
+    const uint64_t fixed = 0;
+    // offsets, descriptors and the coarse index of k requests over nk keys
+    auto meta_bytes = [](size_t k, uint64_t nk) {
+        const uint64_t nidx = (nk >> pg::kIdxShift) + 2;
+        return al(8 * (k + 1)) + al(sizeof(pg::ReqDev) * k) + al(4 * (nidx < kMaxIdx + 1 ? nidx : kMaxIdx + 1));
+    };
+    while (take < s->queue.size() && take < s->max_batch_reqs) {
+        if (fixed + meta_bytes(take + 1, keys + s->queue[take].n) + need + s->queue[take].need > free_bytes) break;
         if (take > 0 && keys + s->queue[take].n > s->max_batch_keys) break;
         need += s->queue[take].need;
         keys += s->queue[take].n;
         take++;
     }
     if (take == 0) return PG_OK;
-    const uint64_t meta = al(sizeof(pg::ReqDev) * take) + al(8 * (take + 1));
+    const uint64_t meta = meta_bytes(take, keys);
     uint64_t at = 0;
     if (!s->ring.alloc(fixed + meta + need, at)) return PG_OK;
     hipStream_t st = s->streams[si];
     char *base = s->arena + at;
-    uint64_t *d_counter = (uint64_t *)base;
-    pg::ReqDev *d_desc = (pg::ReqDev *)(base + fixed);
-    uint64_t *d_off = (uint64_t *)(base + fixed + al(sizeof(pg::ReqDev) * take));
+    uint64_t *d_off = (uint64_t *)(base + fixed);
+    pg::ReqDev *d_desc = (pg::ReqDev *)(base + fixed + al(8 * (take + 1)));
     char *p = base + fixed + meta;
-    pg::ReqDev *hd = s->h_desc[si];
-    uint64_t *ho = s->h_off[si];
+    char *hm = s->h_meta[si];
+    uint64_t *ho = (uint64_t *)hm;
+    pg::ReqDev *hd = (pg::ReqDev *)(hm + al(8 * (take + 1)));
     ServeBatch b;
     b.stream = si;
     uint64_t total = 0;
     for (size_t i = 0; i < take; i++) {
         const ServeReq &r = s->queue[i];
         pg::ReqDev &d = hd[i];
-        d = pg::ReqDev{};
+        if (r.in_place) {
+            d = r.dev;
+        } else {
+            d = pg::ReqDev{};
+            d.keys = (const uint64_t *)p;
+            if (r.n && hipMemcpyAsync(p, r.h_keys, r.n * 8, hipMemcpyHostToDevice, st) != hipSuccess)
+                return PG_ERR_HIP;
+            p += al(r.n * 8);
+            if (r.op == PG_OP_INSERT && r.h_vals) {
+                d.vals = p;
+                if (r.n && hipMemcpyAsync(p, r.h_vals, r.n * r.vbytes, hipMemcpyHostToDevice, st) != hipSuccess)
+                    return PG_ERR_HIP;
+                p += al(r.n * r.vbytes);
+            }
+            if (r.h_flags) {
+                d.flags = (uint8_t *)p;
+                p += al(r.n);
+            }
+            if (r.op == PG_OP_QUERY && r.h_vals_out) {
+                d.vals_out = p;
+                p += al(r.n * r.vbytes);
+            }
+        }
         d.n = r.n;
         d.table = r.table;
         d.op = r.op;
-        d.keys = (const uint64_t *)p;
-        if (r.n && hipMemcpyAsync(p, r.h_keys, r.n * 8, hipMemcpyHostToDevice, st) != hipSuccess) return PG_ERR_HIP;
-        p += al(r.n * 8);
-        if (r.op == PG_OP_INSERT && r.h_vals) {
-            d.vals = p;
-            if (r.n && hipMemcpyAsync(p, r.h_vals, r.n * r.vbytes, hipMemcpyHostToDevice, st) != hipSuccess)
-                return PG_ERR_HIP;
-            p += al(r.n * r.vbytes);
-        }
-        if (r.h_flags) {
-            d.flags = (uint8_t *)p;
-            p += al(r.n);
-        }
-        if (r.op == PG_OP_QUERY && r.h_vals_out) {
-            d.vals_out = p;
-            p += al(r.n * r.vbytes);
-        }
         ho[i] = total;
         total += r.n;
         b.tickets.push_back(r.ticket);
     }
     ho[take] = total;
-    if (hipMemcpyAsync(d_desc, hd, sizeof(pg::ReqDev) * take, hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(d_off, ho, 8 * (take + 1), hipMemcpyHostToDevice, st) != hipSuccess)
-        return PG_ERR_HIP;
+    // coarse index: the request holding key c << kIdxShift, for every c
+    const uint64_t nidx = total ? ((total - 1) >> pg::kIdxShift) + 1 : 0;
+    const bool use_idx = take > 1 && nidx <= kMaxIdx;
+    const uint64_t idx_at = al(8 * (take + 1)) + al(sizeof(pg::ReqDev) * take);
+    uint64_t meta_copy = al(8 * (take + 1)) + sizeof(pg::ReqDev) * take;
+    if (use_idx) {
+        uint32_t *hi = (uint32_t *)(hm + idx_at);
+        uint32_t r = 0;
+        for (uint64_t c = 0; c < nidx; c++) {
+            const uint64_t k = c << pg::kIdxShift;
+            while (ho[r + 1] <= k) r++;
+            hi[c] = r;
+        }
+        meta_copy = idx_at + 4 * nidx;
+    }
+    if (hipMemcpyAsync(d_off, hm, meta_copy, hipMemcpyHostToDevice, st) != hipSuccess) return PG_ERR_HIP;
     if (total) {
         pg::ServeArgs a{};
         a.tables = s->d_tables;
         a.reqs = d_desc;
         a.off = d_off;
+        a.idx = use_idx ? (const uint32_t *)((char *)d_off + idx_at) : nullptr;
         a.nreq = (uint32_t)take;
         a.total = total;
-        a.counter = d_counter;
-        a.chunk = s->chunk;
-        const uint64_t want = (total + s->chunk - 1) / s->chunk;
-        const unsigned blocks = (unsigned)(want < s->blocks * 4ull ? (want + 3) / 4 : s->blocks);
-        const int rc = pg::launch_serve(s->P, a, blocks ? blocks : 1, st);
+        const int rc = pg::launch_serve(s->P, a, st);
         if (rc != PG_OK) return rc;
     }
     for (size_t i = 0; i < take; i++) {
         const ServeReq &r = s->queue[i];
+        if (r.in_place) continue;
         const pg::ReqDev &d = hd[i];
         if (r.h_flags && r.n &&
             hipMemcpyAsync(r.h_flags, d.flags, r.n, hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -440,10 +508,11 @@ int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndon
     return PG_OK;
 }
 
-int pg_server_set_batch(pg_server s, uint64_t max_batch_keys, uint32_t chunk) {
-    if (!s || max_batch_keys == 0 || chunk == 0) return PG_ERR_INVALID;
+int pg_server_set_batch(pg_server s, uint64_t max_batch_keys, uint32_t max_batch_requests) {
+    if (!s || max_batch_keys == 0 || max_batch_requests == 0 || max_batch_requests > kMaxBatchReqs)
+        return PG_ERR_INVALID;
     s->max_batch_keys = max_batch_keys;
-    s->chunk = chunk;
+    s->max_batch_reqs = max_batch_requests;
     return PG_OK;
 }
 
@@ -470,10 +539,21 @@ int pg_server_run_plain(pg_server s, uint32_t table, uint32_t op, const uint64_t
     r.h_vals = h_vals;
     r.h_flags = h_flags;
     r.h_vals_out = h_vals_out;
+    r.h_keys = h_keys;
     r.vbytes = vb;
+    hipStream_t st = s->streams[0];
+    classify(r);
+    if (r.in_place) {  // one launch on the caller's buffers, then wait
+        pg_handle h = s->handles[table];
+        int rc;
+        if (op == PG_OP_INSERT) rc = pg_bulk_insert(h, r.dev.keys, r.dev.vals, n, r.dev.flags, nullptr, st);
+        else if (op == PG_OP_QUERY) rc = pg_bulk_query(h, r.dev.keys, n, r.dev.vals_out, r.dev.flags, st);
+        else rc = pg_bulk_remove(h, r.dev.keys, n, r.dev.flags, st);
+        if (hipStreamSynchronize(st) != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
+        return rc;
+    }
     uint64_t at = 0;
     if (!s->ring.alloc(req_bytes(r), at)) return PG_ERR_NOMEM;
-    hipStream_t st = s->streams[0];
     char *p = s->arena + at;
     uint64_t *d_keys = (uint64_t *)p;
     p += al(n * 8);

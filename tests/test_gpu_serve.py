@@ -1,7 +1,8 @@
 """GPU tests of the continuous-batching server (pg_server_*): every request's
 result is the same whether it runs alone (per-request plain launch) or in
-the served mix; admission respects the HBM budget; requests submitted while
-others are in flight complete."""
+the served mix, with buffers used in place (pinned host memory) or staged
+(pageable memory); admission of staged requests respects the HBM budget;
+requests submitted while others are in flight complete."""
 import numpy as np
 import pytest
 import torch
@@ -14,14 +15,14 @@ pytestmark = pytest.mark.gpu
 GOLD = np.uint64(0x9E3779B97F4A7C15)
 
 
-def pinned_u64(a):
-    t = torch.empty(len(a), dtype=torch.int64, pin_memory=True)
+def host_u64(a, pinned=True):
+    t = torch.empty(len(a), dtype=torch.int64, pin_memory=pinned)
     t.numpy()[:] = np.ascontiguousarray(a, dtype=np.uint64).view(np.int64)
     return t
 
 
-def pinned_u16(a):
-    t = torch.empty(len(a), dtype=torch.int16, pin_memory=True)
+def host_u16(a, pinned=True):
+    t = torch.empty(len(a), dtype=torch.int16, pin_memory=pinned)
     t.numpy()[:] = np.ascontiguousarray(a, dtype=np.uint16).view(np.int16)
     return t
 
@@ -67,19 +68,21 @@ def make_requests(rng, base, ntables, nreq):
     return reqs
 
 
-def buffers(tab, op, keys):
+def buffers(tab, op, keys, pinned=True):
     n = len(keys)
-    k = pinned_u64(keys)
+    k = host_u64(keys, pinned)
     vals = out = None
     if op == pg.OP_INSERT:
-        vals = pinned_u64(keys ^ GOLD) if tab.is_kv else pinned_u16((keys & np.uint64(0xFFFF)).astype(np.uint16))
+        vals = (host_u64(keys ^ GOLD, pinned) if tab.is_kv
+                else host_u16((keys & np.uint64(0xFFFF)).astype(np.uint16), pinned))
     if op == pg.OP_QUERY:
-        out = torch.zeros(n, dtype=torch.int64 if tab.is_kv else torch.int16, pin_memory=True)
-    flags = torch.zeros(n, dtype=torch.uint8, pin_memory=True)
+        out = torch.zeros(n, dtype=torch.int64 if tab.is_kv else torch.int16, pin_memory=pinned)
+    flags = torch.zeros(n, dtype=torch.uint8, pin_memory=pinned)
     return k, vals, out, flags
 
 
-def test_results_alone_equal_results_in_mix():
+@pytest.mark.parametrize("pinned", [True, False], ids=["in-place", "staged"])
+def test_results_alone_equal_results_in_mix(pinned):
     rng = np.random.default_rng(4)
     tabs_mix, tabs_alone = build_tables(), build_tables()
     base = [oracle.gen_keys(50 + i, 0, int(0.5 * t.get_num_slots())) for i, t in enumerate(tabs_mix)]
@@ -98,7 +101,7 @@ def test_results_alone_equal_results_in_mix():
         clean.append((t, op, k))
     reqs = clean
     srv_mix = pg.Server(tabs_mix, budget_bytes=64 << 20, nstreams=4)
-    bufs_mix = [buffers(tabs_mix[t], op, k) for t, op, k in reqs]
+    bufs_mix = [buffers(tabs_mix[t], op, k, pinned) for t, op, k in reqs]
     tickets = {}
     done = set()
     for i, ((t, op, k), (kk, vv, oo, ff)) in enumerate(zip(reqs, bufs_mix)):
@@ -111,7 +114,7 @@ def test_results_alone_equal_results_in_mix():
             break
     assert len(done) == len(reqs)
     srv_alone = pg.Server(tabs_alone, budget_bytes=64 << 20, nstreams=1)
-    bufs_alone = [buffers(tabs_alone[t], op, k) for t, op, k in reqs]
+    bufs_alone = [buffers(tabs_alone[t], op, k, pinned) for t, op, k in reqs]
     for (t, op, k), (kk, vv, oo, ff) in zip(reqs, bufs_alone):
         srv_alone.run_plain(t, op, kk, vv, oo, ff)
     for i, ((t, op, k), bm, ba) in enumerate(zip(reqs, bufs_mix, bufs_alone)):
@@ -133,14 +136,16 @@ def test_results_alone_equal_results_in_mix():
 
 
 def test_admission_respects_budget_and_drains_queue():
+    """Pageable buffers are staged through the arena: the budget bounds what
+    is in flight, and a request that can never fit is refused."""
     tabs = build_tables()[:1]
     srv = pg.Server(tabs, budget_bytes=1 << 20, nstreams=2)
     big = oracle.gen_keys(1, 0, 1 << 17)       # 1 MiB of keys alone: can never fit
-    kk, _, _, ff = buffers(tabs[0], pg.OP_INSERT, big)
+    kk, _, _, ff = buffers(tabs[0], pg.OP_INSERT, big, pinned=False)
     with pytest.raises(pg.PgError):
         srv.submit(0, pg.OP_INSERT, kk, None, None, ff)
     reqs = [oracle.gen_keys(2, i * 20000, 20000) for i in range(20)]  # 4.4 MB staged in total, 76 % load
-    bufs = [buffers(tabs[0], pg.OP_INSERT, k) for k in reqs]
+    bufs = [buffers(tabs[0], pg.OP_INSERT, k, pinned=False) for k in reqs]
     for (kk, vv, oo, ff) in bufs:
         srv.submit(0, pg.OP_INSERT, kk, vv, oo, ff)
     q, f = srv.pending()
@@ -157,4 +162,31 @@ def test_admission_respects_budget_and_drains_queue():
     for (kk, vv, oo, ff) in bufs:
         assert ff.numpy().all()
     assert tabs[0].get_fill() == 20 * 20000
+    srv.free()
+
+
+def test_in_place_requests_take_no_staging():
+    """Pinned buffers are used in place: a request far larger than the
+    staging budget is admitted, and its results land in the caller's
+    buffers."""
+    tabs = build_tables()[:1]
+    srv = pg.Server(tabs, budget_bytes=1 << 20, nstreams=2)
+    keys = oracle.gen_keys(3, 0, 1 << 18)      # 2 MiB of keys > the 1 MiB budget
+    kk, vv, _, ff = buffers(tabs[0], pg.OP_INSERT, keys)
+    srv.submit(0, pg.OP_INSERT, kk, vv, None, ff)
+    done = []
+    for _ in range(1_000_000):
+        done += srv.progress()
+        if done:
+            break
+    assert len(done) == 1 and ff.numpy().all()
+    qk, _, qo, qf = buffers(tabs[0], pg.OP_QUERY, keys)
+    srv.submit(0, pg.OP_QUERY, qk, None, qo, qf)
+    done = []
+    for _ in range(1_000_000):
+        done += srv.progress()
+        if done:
+            break
+    assert qf.numpy().all()
+    assert (qo.numpy().view(np.uint16) == (keys & np.uint64(0xFFFF)).astype(np.uint16)).mean() > 0.99
     srv.free()
