@@ -34,6 +34,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--log2-slots", type=int, default=28)
     ap.add_argument("--tile", type=int, default=2)
+    ap.add_argument("--query-items", type=int, default=0,
+                    help="query launch shape: keys per tile (1, 2, 4; 0 = library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-roofline", action="store_true", help="skip the random-sector microbenchmark")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -608,6 +610,7 @@ def main():
     desc = pg.tcf_desc(nslots, backing, bucket_size=16, tile=args.tile, prober=pg.PROBE_SPLIT64,
                        hasher=pg.HASH_FMIX64)
     table = pg.Table(desc, seed=42)
+    table.set_items(0, args.query_items)
     n_ins = int(0.9 * nslots)
     batch = 1 << 24 if args.log2_slots >= 26 else max(1, n_ins // 8)
     nq = 1 << (args.log2_slots - 2)

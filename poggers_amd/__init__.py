@@ -307,7 +307,8 @@ class Table:
         check(lib().pg_set_stats(self._h, _ptr(stats)), "pg_set_stats")
 
     def set_items(self, insert_items: int = 0, query_items: int = 0):
-        """Keys per tile for concurrent inserts / queries (1, 2, 4; 0 = default)."""
+        """Keys per tile for concurrent inserts / queries (1, 2, 4; 0 = the
+        default, 1).  Results do not depend on it."""
         check(lib().pg_set_items(self._h, insert_items, query_items), "pg_set_items")
 
     def get_fill(self, stream=None) -> int:

@@ -87,6 +87,29 @@ def test_query_on_oracle_image_bit_exact(tile, B, prober, hasher):
     assert (v_g == v_o).all()
 
 
+@pytest.mark.parametrize("tile,B", [(2, 16), (4, 8), (8, 32)])
+def test_query_kernels_agree(tile, B):
+    """Every query launch shape (pg_set_items query_items = 0, 1, 2, 4)
+    returns identical flags and values, for batch sizes that leave partial
+    waves and partial key groups."""
+    desc = pg.tcf_desc(1 << 20, bucket_size=B, tile=tile)
+    o, g = pair(desc)
+    ops = Ops(g)
+    keys = oracle.gen_keys(31, 0, int(0.9 * (1 << 20)))
+    ops.insert(keys, (keys >> np.uint64(9)).astype(np.uint16))
+    pool = np.concatenate([keys, oracle.gen_keys(31, 1 << 43, 1 << 20)])
+    np.random.default_rng(5).shuffle(pool)
+    for n in (1, 63, 64, 65, 100_003, len(pool)):
+        q = pool[:n]
+        g.set_items(0, 0)
+        f0, v0 = ops.query(q)
+        for items in (1, 2, 4):
+            g.set_items(0, items)
+            f1, v1 = ops.query(q)
+            assert (f0 == f1).all() and (v0 == v1).all(), (n, items)
+    g.set_items(0, 0)
+
+
 @pytest.mark.parametrize("tile", [4, 16])
 def test_concurrent_zero_false_negatives_and_fp_rate(tile):
     nslots = 1 << 22
