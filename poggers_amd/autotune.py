@@ -30,13 +30,15 @@ def _time_replays(batch: Batch, stream, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def tune_batch(sizes, ins_keys, ins_off, q_keys, q_off, tiles=(4, 8, 16, 32, 64), buckets=(8, 16, 32),
+def tune_batch(sizes, ins_keys, ins_off, q_keys, q_off, tiles=(1, 2, 4, 8, 16, 32), buckets=(8, 16, 32),
                chunks=(256, 1024, 4096), reps=3, seed=42, log=None):
     """Sweep tile width x bucket size x chunk on the given workload (device
     tensors).  Returns {"best": {...}, "results": [...]} with times in ms for
-    one clear + insert + query replay."""
+    one clear + insert + query replay and the inserts each replay failed;
+    the fastest shape with no failed insert wins."""
     stream = torch.cuda.Stream()
     found = torch.empty(q_keys.numel(), dtype=torch.uint8, device=q_keys.device)
+    fails = torch.zeros(1, dtype=torch.int64, device=q_keys.device)
     results = []
     for B in buckets:
         for P in tiles:
@@ -44,12 +46,18 @@ def tune_batch(sizes, ins_keys, ins_off, q_keys, q_off, tiles=(4, 8, 16, 32, 64)
             batch = Batch(desc, sizes, seed=seed)
             for chunk in chunks:
                 batch.set_launch(chunk)
-                batch.capture(ins_keys, ins_off, q_keys, q_off, None, found, None, True, stream)
+                fails.zero_()
+                torch.cuda.synchronize()
+                batch.capture(ins_keys, ins_off, q_keys, q_off, None, found, fails, True, stream)
                 ms = _time_replays(batch, stream, reps)
-                results.append({"tile": P, "bucket": B, "chunk": chunk, "ms": round(ms, 4)})
+                dropped = int(fails.item()) // (reps + 1)
+                results.append({"tile": P, "bucket": B, "chunk": chunk, "ms": round(ms, 4),
+                                "insert_failures": dropped})
                 if log:
-                    log(f"tune tile={P} bucket={B} chunk={chunk}: {ms:.3f} ms")
+                    log(f"tune tile={P} bucket={B} chunk={chunk}: {ms:.3f} ms, {dropped} failed inserts")
             batch.free()
             del batch
-    best = min(results, key=lambda r: r["ms"])
+    # a shape that drops keys is only chosen when every shape does
+    complete = [r for r in results if r["insert_failures"] == 0] or results
+    best = min(complete, key=lambda r: r["ms"])
     return {"best": best, "results": results}
