@@ -30,7 +30,7 @@ def _time_replays(batch: Batch, stream, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
-def tune_batch(sizes, ins_keys, ins_off, q_keys, q_off, tiles=(1, 2, 4, 8, 16, 32), buckets=(8, 16, 32),
+def tune_batch(sizes, ins_keys, ins_off, q_keys, q_off, tiles=(1, 2, 4, 8, 16, 32, 64), buckets=(8, 16, 32),
                chunks=(256, 1024, 4096), reps=3, seed=42, log=None):
     """Sweep tile width x bucket size x chunk on the given workload (device
     tensors).  Returns {"best": {...}, "results": [...]} with times in ms for
