@@ -49,6 +49,7 @@ def parse():
                     help="tcf: the headline 2^28-slot filter; batched: 4096 uneven filters in one graph; "
                          "kv: 2^30-slot double-hashing key-value table")
     ap.add_argument("--tables", type=int, default=4096, help="batched: number of filters")
+    ap.add_argument("--kv-tile", type=int, default=4, help="kv: tile width (4 is hash_table_64's)")
     return ap.parse_args()
 
 
@@ -67,8 +68,8 @@ def run_kv(args):
     m = 1 << log2
     n = int(0.9 * m)
     nq = 1 << (log2 - 2)
-    table = pg.Table(pg.kv_desc(m, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, hasher=pg.HASH_FMIX64,
-                                max_probes=20))
+    table = pg.Table(pg.kv_desc(m, bucket_size=16, tile=args.kv_tile, prober=pg.PROBE_DOUBLE,
+                                hasher=pg.HASH_FMIX64, max_probes=20))
     keys = torch.empty(n, dtype=torch.int64, device=dev)
     pg.gen_keys(99, 0, n, keys)
     vals = keys ^ torch.tensor(0x9E3779B97F4A7C15 - (1 << 64), dtype=torch.int64, device=dev)
@@ -124,7 +125,7 @@ def run_kv(args):
         "unit": "Mops/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(sum(res.values()), 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic (splitmix64 keys, Zipf(0.99) hits)",
-        "config": {"workload": f"KV 2^{log2} slots x 16 B ({m * 16 >> 30} GiB), double hashing, B=16, tile 4, "
+        "config": {"workload": f"KV 2^{log2} slots x 16 B ({m * 16 >> 30} GiB), double hashing, B=16, tile {args.kv_tile}, "
                                f"20 probes; {n} inserts, {nq} queries (50% Zipf(0.99) hits, 50% absent), "
                                f"delete {ndel}, re-query"},
         "insert_mops": round(n / res["insert"] / 1e3, 1), "query_mops": round(nq / res["query"] / 1e3, 1),

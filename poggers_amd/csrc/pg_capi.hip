@@ -185,10 +185,38 @@ int pg_set_mode(pg_handle h, uint32_t mode) {
     return PG_OK;
 }
 
+// Bytes per element of the value buffers (vals, vals_out) of a representation.
+static uint64_t value_bytes(uint32_t rep) {
+    if (rep == PG_REP_KV64) return 8;
+    if (rep == PG_REP_FP8_V8 || rep == PG_REP_FP10_V6) return 1;
+    return 2;
+}
+
+// One tile per key: a launch covers n * tile work-items, and HIP caps the
+// work-items of one grid dimension below 2^32.  Bulk calls are cut into
+// launches of at most 2^31 work-items, issued in order on the stream, each
+// with its key, value and output pointers advanced.
 static int run_op(pg_handle h, pg::Op op, const pg::OpArgs &a, hipStream_t s) {
-    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_KEY64)
-        return pg::launch_kv(h->dev, h->desc.level[0].tile, op, a, s);
-    return pg::launch_tcf(h->dev, h->desc.level[0].tile, op, a, s);
+    const uint32_t tile = h->desc.level[0].tile;
+    const bool kv = h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_KEY64;
+    const uint64_t max_keys = (1ull << 31) / tile;
+    const uint64_t vb = value_bytes(h->dev.rep);
+    uint64_t done = 0;
+    do {
+        pg::OpArgs c = a;
+        c.n = a.n - done < max_keys ? a.n - done : max_keys;
+        if (done) {
+            c.keys = a.keys + done;
+            if (a.vals) c.vals = (const uint8_t *)a.vals + done * vb;
+            if (a.vals_out) c.vals_out = (uint8_t *)a.vals_out + done * vb;
+            if (a.flags) c.flags = a.flags + done;
+            if (a.found) c.found = a.found + done;
+        }
+        const int rc = kv ? pg::launch_kv(h->dev, tile, op, c, s) : pg::launch_tcf(h->dev, tile, op, c, s);
+        if (rc != PG_OK) return rc;
+        done += c.n;
+    } while (done < a.n);
+    return PG_OK;
 }
 
 static int do_insert(pg_handle h, const uint64_t *d_keys, const void *d_vals, uint64_t n, uint8_t *d_status,

@@ -120,12 +120,14 @@ __device__ inline void serve_one(const TableDev &T, const ReqDev &r, uint64_t j,
 // key's loads (its key, its buckets) are in flight at once.  A tile finds its
 // request from the batch's coarse index (one load, then a short walk over the
 // key offsets) rather than a binary search, whose dependent loads would sit
-// in front of every key.
+// in front of every key.  The grid is capped at 2^31 work-items (HIP's limit
+// for one dimension is below 2^32), and a tile takes key i + k * tiles until
+// the batch is done.
 template <int P>
 __global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
     Tile<P> t;
-    const uint64_t i = (uint64_t(blockIdx.x) * 256 + threadIdx.x) / P;
-    if (i < a.total) {
+    const uint64_t tiles = (uint64_t(gridDim.x) * 256) / P;
+    for (uint64_t i = (uint64_t(blockIdx.x) * 256 + threadIdx.x) / P; i < a.total; i += tiles) {
         uint32_t r;
         if (a.idx) {
             r = a.idx[i >> kIdxShift];
@@ -139,7 +141,8 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
 }
 
 static int launch_serve(uint32_t P, const ServeArgs &a, hipStream_t s) {
-    const unsigned blocks = (unsigned)((a.total * P + 255) / 256);
+    const uint64_t want = (a.total * P + 255) / 256;
+    const unsigned blocks = (unsigned)(want < (1ull << 23) ? want : (1ull << 23));
     switch (P) {
         case 1: hipLaunchKernelGGL((k_serve<1>), dim3(blocks), dim3(256), 0, s, a); break;
         case 2: hipLaunchKernelGGL((k_serve<2>), dim3(blocks), dim3(256), 0, s, a); break;
