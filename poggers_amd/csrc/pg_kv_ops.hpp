@@ -12,6 +12,10 @@
 //     the probe bucket and loads it with one 16 B global_load_dwordx4; this is
 //     the reference's stripe scan (insert_schemes/bucket_insert.cuh:127-357)
 //     with the same early exit (a stripe holding an empty slot ends a query).
+//     When B is 2P or 4P the whole bucket is loaded at once (KvBucket: each
+//     lane issues its 2 or 4 slot loads together) and the stripes are then
+//     evaluated in the same order, so a walk over several stripes costs one
+//     round trip instead of one per stripe.
 //   * probe-parallel mode (B == 1, P > 1): lane r evaluates probe j0 + r, so
 //     one tile step examines P consecutive probes (for linear probing, P
 //     adjacent slots in one or two 64 B sectors).  The lowest decisive lane
@@ -64,6 +68,119 @@ __device__ inline uint64_t below_first(uint64_t endm) {
     return endm ? ((1ull << ctz64(endm)) - 1ull) : ~0ull;
 }
 
+// A probe bucket of NS stripes of P slots held in registers: lane r holds
+// slot k*P + r of stripe k.  The NS loads are independent, so they are all in
+// flight together.
+template <int P, int NS, class SL>
+struct KvBucket {
+    typename SL::S s[NS];
+    __device__ inline void load(const typename SL::S *slots, uint64_t base, const Tile<P> &t) {
+#pragma unroll
+        for (int k = 0; k < NS; k++) s[k] = SL::load(slots + base + k * P + t.rank);
+    }
+};
+
+// The stripe scans below over a register bucket: the same decisions in the
+// same order as the stripe-by-stripe walks of kv_level_insert / _query /
+// _remove (bucket_insert.cuh:127-357), on a bucket loaded in one round trip.
+template <int P, int NS, class SL>
+__device__ bool kv_reg_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
+    using S = typename SL::S;
+    Probe pr;
+    pr.begin(L, key, key);
+    pr.max_probes = pr.useful_probes(L.nb.d);
+    const uint64_t sk = SL::stored(key);
+    S *slots = (S *)L.slots;
+    for (uint32_t j = 0;; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) return false;
+        const uint64_t base = fastmod(p, L.nb) * (uint64_t)(NS * P);
+        KvBucket<P, NS, SL> bk;
+        bk.load(slots, base, t);
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            t.acc[0] += P;
+            uint64_t cm = t.bits(SL::key(bk.s[k]) == kKvEmpty);
+            while (cm) {
+                const uint32_t l = (uint32_t)ctz64(cm);
+                bool success = false;
+                if (t.rank == l) {
+                    S *sp = slots + base + k * P + t.rank;
+                    success = atomicCAS(SL::key_ptr(sp), kKvEmpty, sk) == kKvEmpty;
+                    if (success) SL::store_val(sp, val);
+                }
+                if (t.any(success)) {
+                    t.acc[2]++;
+                    return true;
+                }
+                cm &= cm - 1ull;
+            }
+        }
+    }
+}
+
+template <int P, int NS, class SL>
+__device__ bool kv_reg_query(const LevelDev &L, uint64_t key, const Tile<P> &t, uint64_t &val, bool &owner) {
+    using S = typename SL::S;
+    Probe pr;
+    pr.begin(L, key, key);
+    pr.max_probes = pr.useful_probes(L.nb.d);
+    const uint64_t sk = SL::stored(key);
+    const S *slots = (const S *)L.slots;
+    for (uint32_t j = 0;; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) return false;
+        KvBucket<P, NS, SL> bk;
+        bk.load(slots, fastmod(p, L.nb) * (uint64_t)(NS * P), t);
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            t.acc[0] += P;
+            const uint64_t mm = t.bits(SL::key(bk.s[k]) == sk);
+            if (mm) {
+                const uint32_t l = (uint32_t)ctz64(mm);
+                owner = (t.rank == l);
+                if (owner) val = SL::val(bk.s[k]);
+                return true;
+            }
+            if (t.any(SL::key(bk.s[k]) == kKvEmpty)) return false;
+        }
+    }
+}
+
+template <int P, int NS, class SL>
+__device__ bool kv_reg_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
+    using S = typename SL::S;
+    Probe pr;
+    pr.begin(L, key, key);
+    pr.max_probes = pr.useful_probes(L.nb.d);
+    const uint64_t sk = SL::stored(key);
+    S *slots = (S *)L.slots;
+    for (uint32_t j = 0;; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) return false;
+        const uint64_t base = fastmod(p, L.nb) * (uint64_t)(NS * P);
+        KvBucket<P, NS, SL> bk;
+        bk.load(slots, base, t);
+#pragma unroll
+        for (int k = 0; k < NS; k++) {
+            const bool match = SL::key(bk.s[k]) == sk;
+            t.acc[0] += P;
+            uint64_t mm = t.bits(match);
+            while (mm) {
+                const uint32_t l = (uint32_t)ctz64(mm);
+                bool success = false;
+                if (t.rank == l) success = atomicCAS(SL::key_ptr(slots + base + k * P + t.rank), sk, kKvTomb) == sk;
+                if (t.any(success)) {
+                    t.acc[2]++;
+                    return true;
+                }
+                mm &= mm - 1ull;
+            }
+            if (t.any(!match && SL::key(bk.s[k]) == kKvEmpty)) return false;
+        }
+    }
+}
+
 // bucket_insert::insert (bucket_insert.cuh:127-203) / single_slot_insert::
 // insert (single_slot_insert.cuh:125-166): lowest empty slot in probe order;
 // key_val_pair::atomic_swap = CAS key 0 -> key, then store the value
@@ -71,6 +188,10 @@ __device__ inline uint64_t below_first(uint64_t endm) {
 template <int P, class SL = SlotKV>
 __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
     using S = typename SL::S;
+    if constexpr (P <= 16) {
+        if (L.B == 4 * P) return kv_reg_insert<P, 4, SL>(L, key, val, t);
+        if (L.B == 2 * P) return kv_reg_insert<P, 2, SL>(L, key, val, t);
+    }
     Probe pr;
     pr.begin(L, key, key);
     pr.max_probes = pr.useful_probes(L.nb.d);
@@ -139,6 +260,10 @@ template <int P, class SL = SlotKV>
 __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t, uint64_t &val,
                                bool &owner) {
     using S = typename SL::S;
+    if constexpr (P <= 16) {
+        if (L.B == 4 * P) return kv_reg_query<P, 4, SL>(L, key, t, val, owner);
+        if (L.B == 2 * P) return kv_reg_query<P, 2, SL>(L, key, t, val, owner);
+    }
     Probe pr;
     pr.begin(L, key, key);
     pr.max_probes = pr.useful_probes(L.nb.d);
@@ -195,6 +320,10 @@ __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t
 template <int P, class SL = SlotKV>
 __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
     using S = typename SL::S;
+    if constexpr (P <= 16) {
+        if (L.B == 4 * P) return kv_reg_remove<P, 4, SL>(L, key, t);
+        if (L.B == 2 * P) return kv_reg_remove<P, 2, SL>(L, key, t);
+    }
     Probe pr;
     pr.begin(L, key, key);
     pr.max_probes = pr.useful_probes(L.nb.d);

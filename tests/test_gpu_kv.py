@@ -43,6 +43,12 @@ def test_linear_2p20_config(tile):
     (8, 8, pg.PROBE_DOUBLE, pg.HASH_MURMUR64A),
     (2, 4, pg.PROBE_XOR_P2, pg.HASH_FMIX64),
     (64, 1, pg.PROBE_DOUBLE, pg.HASH_FMIX64),
+    # whole-bucket register loads: B = 4P and B = 2P
+    (1, 4, pg.PROBE_DOUBLE, pg.HASH_MURMUR64A),
+    (4, 8, pg.PROBE_DOUBLE, pg.HASH_FMIX64),
+    (8, 32, pg.PROBE_LINEAR, pg.HASH_MURMUR64A),
+    (16, 32, pg.PROBE_DOUBLE, pg.HASH_FMIX64),
+    (2, 8, pg.PROBE_LINEAR, pg.HASH_FMIX64),
 ])
 def test_deterministic_layout_bit_exact(tile, B, prober, hasher):
     m = 1 << 12
