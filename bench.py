@@ -576,6 +576,18 @@ def cpu_baseline(log2_slots=26):
     t.bulk_query(absent)
     t2 = time.perf_counter()
     ops = n + 2 * nq
+    # the same algorithm on every core this process may use (atomic slot
+    # claims), capped by OMP_NUM_THREADS when the host sets a CPU share
+    threads = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        threads = max(1, min(threads, int(os.environ["OMP_NUM_THREADS"])))
+    tm = oracle.Table(oracle.REP_FP16_V16, levels, seed=42)
+    m0 = time.perf_counter()
+    st, used = tm.bulk_insert_mt(keys, threads=threads)
+    m1 = time.perf_counter()
+    tm.bulk_query_mt(keys[::max(1, n // nq)][:nq], threads=threads)
+    tm.bulk_query_mt(absent, threads=threads)
+    m2 = time.perf_counter()
     return {
         "value": round(ops / (t2 - t0) / 1e6, 3),
         "unit": "Mops/s",
@@ -586,6 +598,12 @@ def cpu_baseline(log2_slots=26):
         "insert_mops": round(n / (t1 - t0) / 1e6, 3),
         "query_mops": round(2 * nq / (t2 - t1) / 1e6, 3),
         "seconds": round(t2 - t0, 2),
+        "multi_thread": {
+            "value": round(ops / (m2 - m0) / 1e6, 3), "unit": "Mops/s", "cores": used,
+            "insert_mops": round(n / (m1 - m0) / 1e6, 3), "query_mops": round(2 * nq / (m2 - m1) / 1e6, 3),
+            "failed_inserts": int((st == 0).sum()), "seconds": round(m2 - m0, 2),
+            "host_threads_online": oracle.hw_threads(),
+        },
     }
 
 

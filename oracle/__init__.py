@@ -89,6 +89,12 @@ def lib():
         L.or_bulk_query.argtypes = [vp, vp, u64, vp, u32, vp]
         L.or_bulk_remove.restype = None
         L.or_bulk_remove.argtypes = [vp, vp, u64, vp]
+        L.or_bulk_insert_mt.restype = i32
+        L.or_bulk_insert_mt.argtypes = [vp, vp, vp, u64, vp, u32]
+        L.or_bulk_query_mt.restype = i32
+        L.or_bulk_query_mt.argtypes = [vp, vp, u64, vp, vp, u32]
+        L.or_hw_threads.restype = u32
+        L.or_hw_threads.argtypes = []
         L.or_insert_if_not_exists.restype = i32
         L.or_insert_if_not_exists.argtypes = [vp, u64, u64, i32, ctypes.POINTER(ctypes.c_int),
                                               ctypes.POINTER(ctypes.c_uint64)]
@@ -146,6 +152,11 @@ def probes(prober: int, hasher: int, max_probes: int, seed: int, key: int, tag: 
     out = (ctypes.c_uint64 * cap)()
     n = lib().or_probes(prober, hasher, max_probes, seed, key, tag, out, cap)
     return [out[i] for i in range(n)]
+
+
+def hw_threads() -> int:
+    """Host threads online (sysconf)."""
+    return int(lib().or_hw_threads())
 
 
 def gen_keys(seed: int, start: int, n: int) -> np.ndarray:
@@ -213,6 +224,31 @@ class Table:
         vals = np.zeros(len(keys), dtype=self.val_dtype)
         lib().or_bulk_query(self._h, _ptr(keys), len(keys), _ptr(vals), self.val_bytes, _ptr(found))
         return found, vals
+
+    def bulk_insert_mt(self, keys: np.ndarray, vals: np.ndarray | None = None, threads: int = 0):
+        """The same filter insert on `threads` host threads (0 = every core)
+        with atomic slot claims; 16/16 slots, shortcut level 0 and optional
+        linear backing only.  Returns (status, threads used)."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        st = np.zeros(len(keys), dtype=np.uint8)
+        v = None if vals is None else np.ascontiguousarray(vals, dtype=np.uint16)
+        used = lib().or_bulk_insert_mt(self._h, _ptr(keys), None if v is None else _ptr(v), len(keys),
+                                       _ptr(st), threads)
+        if used < 0:
+            raise ValueError("the multi-threaded run covers 16/16 slots with a shortcut level 0 "
+                             "and an optional linear backing level")
+        return st, used
+
+    def bulk_query_mt(self, keys: np.ndarray, threads: int = 0):
+        """bulk_query on `threads` host threads; returns (found, vals, threads used)."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        found = np.zeros(len(keys), dtype=np.uint8)
+        vals = np.zeros(len(keys), dtype=np.uint16)
+        used = lib().or_bulk_query_mt(self._h, _ptr(keys), len(keys), _ptr(vals), _ptr(found), threads)
+        if used < 0:
+            raise ValueError("the multi-threaded run covers 16/16 slots with a shortcut level 0 "
+                             "and an optional linear backing level")
+        return found, vals, used
 
     def insert_if_not_exists(self, key, val=0, delete=False):
         f = ctypes.c_int(0)

@@ -1,7 +1,8 @@
 /*
  * pg_oracle.c -- single-threaded CPU restatement of poggers' bulk
  * insert / query / remove path for the modular hash tables and the
- * Two-Choice Filter (TCF).
+ * Two-Choice Filter (TCF), plus a multi-threaded run of the same filter
+ * algorithm (atomic slot claims) for the all-cores CPU baseline.
  *
  * THIS IS TEST INFRASTRUCTURE.  It is the checker that the HIP path is
  * compared against (and the CPU baseline that bench.py times).  Nothing in
@@ -27,10 +28,16 @@
  *   KV insert     -- tests/src/test_components.cu:88-127 and :345-366 known
  *                    answers, linear-probing closed-form probe lengths,
  *                    brute force on tiny tables.
+ *   multi-thread  -- with one thread, byte-identical layouts to the
+ *                    sequential functions; with several, zero false
+ *                    negatives, fill = stored keys, queries identical to
+ *                    the sequential query of the same table.
  */
+#include <pthread.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #define OR_END (~(uint64_t)0)
 
@@ -1106,4 +1113,192 @@ uint64_t or_key_at(uint64_t seed, uint64_t i)
 void or_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *out)
 {
     for (uint64_t i = 0; i < n; i++) out[i] = or_key_at(seed, start + i);
+}
+
+/* ------------------------------------------------------------------ */
+/* Multi-threaded run of the same algorithm (the CPU baseline's         */
+/* all-cores figure).  16/16 slots, a shortcut level 0 and an optional  */
+/* linear backing level: the table shape the filter benchmark uses.    */
+/* nthreads host threads take contiguous slices of the keys; slots are  */
+/* read with relaxed atomic loads and claimed with an atomic compare-   */
+/* and-swap from the empty word (the reference's atomicCAS,             */
+/* grouped_storage_sub_bits.cuh:127-141); a lost CAS moves on to the    */
+/* next empty slot as the reference's leader loop does                   */
+/* (packed_bucket.cuh:64-114).  With one thread the slot layout is the   */
+/* sequential oracle's; with more, placement depends on thread timing   */
+/* (as on the GPU) while membership does not.  No accounting counters.  */
+/* ------------------------------------------------------------------ */
+static inline uint32_t *mt_slots(const or_level *L, uint64_t b) { return (uint32_t *)bucket_at(L, b); }
+
+static int mt_fill(const or_level *L, uint64_t b)
+{
+    const uint32_t *s = mt_slots(L, b);
+    int f = 0;
+    for (uint32_t i = 0; i < L->B; i++) {
+        uint32_t fp = __atomic_load_n(s + i, __ATOMIC_RELAXED) & 0xFFFFu;
+        f += (fp != 0 && fp != 0xFFFFu);
+    }
+    return f;
+}
+
+static int mt_claim(const or_level *L, uint64_t b, uint32_t word)
+{
+    uint32_t *s = mt_slots(L, b);
+    for (uint32_t i = 0; i < L->B; i++) {
+        uint32_t expect = 0;
+        if (__atomic_load_n(s + i, __ATOMIC_RELAXED) == 0 &&
+            __atomic_compare_exchange_n(s + i, &expect, word, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED))
+            return 1;
+    }
+    return 0;
+}
+
+static int mt_find(const or_level *L, uint64_t b, uint32_t fp, uint32_t *val)
+{
+    const uint32_t *s = mt_slots(L, b);
+    for (uint32_t i = 0; i < L->B; i++) {
+        uint32_t w = __atomic_load_n(s + i, __ATOMIC_RELAXED);
+        if ((w & 0xFFFFu) == fp) {
+            *val = w >> 16;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+/* shortcut_insert / linear_insert above, over atomic slots */
+static int mt_insert(const or_table *t, uint64_t key, uint32_t word)
+{
+    const or_level *L = &t->lv[0];
+    uint32_t fp = fp_of(t->rep, key);
+    uint64_t buckets[MAXP];
+    int fill[MAXP];
+    int np = 0, B = (int)L->B, min_fill = B;
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes, L->seed, key, fp);
+         p != OR_END && np < MAXP; p = probe_next(&it)) {
+        uint64_t b = p % L->nb;
+        buckets[np] = b;
+        int f = mt_fill(L, b);
+        if (4 * f < 3 * B) {
+            if (mt_claim(L, b, word)) return 1;
+            f = B;
+        }
+        fill[np] = f;
+        if (f < min_fill) min_fill = f;
+        np++;
+    }
+    for (int lvl = min_fill; lvl < B;) {
+        int next = B;
+        for (int j = 0; j < np; j++) {
+            if (fill[j] == lvl && mt_claim(L, buckets[j], word)) return 1;
+            if (fill[j] > lvl && fill[j] < next) next = fill[j];
+        }
+        lvl = next;
+    }
+    if (t->nlevels > 1) {
+        const or_level *K = &t->lv[1];
+        for (uint64_t p = probe_begin(&it, K->d.prober, K->d.hasher, K->d.max_probes, K->seed, key, fp);
+             p != OR_END; p = probe_next(&it))
+            if (mt_claim(K, p % K->nb, word)) return 2;
+    }
+    return 0;
+}
+
+/* shortcut_query, then linear_query's walk to the first bucket not full */
+static int mt_query(const or_table *t, uint64_t key, uint32_t *val)
+{
+    const or_level *L = &t->lv[0];
+    uint32_t fp = fp_of(t->rep, key);
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes, L->seed, key, fp);
+         p != OR_END; p = probe_next(&it))
+        if (mt_find(L, p % L->nb, fp, val)) return 1;
+    if (t->nlevels > 1) {
+        const or_level *K = &t->lv[1];
+        for (uint64_t p = probe_begin(&it, K->d.prober, K->d.hasher, K->d.max_probes, K->seed, key, fp);
+             p != OR_END; p = probe_next(&it)) {
+            uint64_t b = p % K->nb;
+            if (mt_find(K, b, fp, val)) return 1;
+            if (mt_fill(K, b) != (int)K->B) return 0;
+        }
+    }
+    return 0;
+}
+
+typedef struct {
+    const or_table *t;
+    const uint64_t *keys;
+    const uint16_t *vals;
+    uint64_t lo, hi;
+    int query;
+    uint8_t *out;        /* insert: status; query: found */
+    uint16_t *vals_out;  /* query values */
+} mt_job;
+
+static void *mt_worker(void *arg)
+{
+    const mt_job *j = (const mt_job *)arg;
+    for (uint64_t i = j->lo; i < j->hi; i++) {
+        if (j->query) {
+            uint32_t v = 0;
+            int r = mt_query(j->t, j->keys[i], &v);
+            if (j->out) j->out[i] = (uint8_t)r;
+            if (j->vals_out) j->vals_out[i] = r ? (uint16_t)v : 0;
+        } else {
+            uint32_t word = or_pack(16, 16, fp_of(j->t->rep, j->keys[i]), j->vals ? j->vals[i] : 0u);
+            int r = mt_insert(j->t, j->keys[i], word);
+            if (j->out) j->out[i] = (uint8_t)r;
+        }
+    }
+    return NULL;
+}
+
+/* Host threads the process may run on. */
+uint32_t or_hw_threads(void)
+{
+    long n = sysconf(_SC_NPROCESSORS_ONLN);
+    return n > 0 ? (uint32_t)n : 1u;
+}
+
+static int mt_shape_ok(const or_table *t)
+{
+    if (t->rep != OR_REP_FP16_V16 || t->lv[0].d.scheme != OR_SCHEME_POWER_OF_N_SHORTCUT) return 0;
+    return t->nlevels == 1 || t->lv[1].d.scheme == OR_SCHEME_LINEAR_BUCKETS;
+}
+
+static int mt_run(const or_table *t, const uint64_t *keys, const uint16_t *vals, uint64_t n, int query,
+                  uint8_t *out, uint16_t *vals_out, uint32_t nthreads)
+{
+    if (!mt_shape_ok(t)) return -1;
+    if (nthreads == 0) nthreads = or_hw_threads();
+    if (nthreads > 256) nthreads = 256;
+    if ((uint64_t)nthreads > n) nthreads = n ? (uint32_t)n : 1u;
+    pthread_t th[256];
+    mt_job jobs[256];
+    int created[256];
+    for (uint32_t k = 0; k < nthreads; k++) {
+        jobs[k] = (mt_job){t, keys, vals, n * k / nthreads, n * (k + 1) / nthreads, query, out, vals_out};
+        /* the last slice runs on this thread; a slice whose thread cannot be
+         * started runs here too */
+        created[k] = k + 1 < nthreads && pthread_create(&th[k], NULL, mt_worker, &jobs[k]) == 0;
+        if (!created[k] && k + 1 < nthreads) mt_worker(&jobs[k]);
+    }
+    mt_worker(&jobs[nthreads - 1]);
+    for (uint32_t k = 0; k + 1 < nthreads; k++)
+        if (created[k]) pthread_join(th[k], NULL);
+    return (int)nthreads;
+}
+
+/* Returns the number of threads used, or -1 for an unsupported shape. */
+int or_bulk_insert_mt(or_table *t, const uint64_t *keys, const uint16_t *vals, uint64_t n, uint8_t *status,
+                      uint32_t nthreads)
+{
+    return mt_run(t, keys, vals, n, 0, status, NULL, nthreads);
+}
+
+int or_bulk_query_mt(or_table *t, const uint64_t *keys, uint64_t n, uint16_t *vals_out, uint8_t *found,
+                     uint32_t nthreads)
+{
+    return mt_run(t, keys, NULL, n, 1, found, vals_out, nthreads);
 }

@@ -293,3 +293,61 @@ def test_upsert_does_not_see_a_key_held_only_by_the_backing_level(oracle_lib):
     assert t.insert_if_not_exists(spill, 9, delete=True) == (1, False, 0)
     assert bucket(t, 0, 2, B0)[0] == word(spill, 9)
     assert bucket(t, 1, 0, B1)[0] == word(spill, 4)
+
+
+def _bench_shape(o, nslots):
+    return [o.level(nslots, o.SCHEME_SHORTCUT, 16, prober=o.PROBE_SPLIT64, hasher=o.HASH_FMIX64),
+            o.level(nslots // 100, o.SCHEME_LINEAR_BUCKETS, 8, prober=o.PROBE_LINEAR, max_probes=4000)]
+
+
+def test_multi_thread_run_with_one_thread_is_the_sequential_oracle(oracle_lib):
+    """The all-cores baseline runs the same algorithm with atomic slot
+    claims: on one thread its statuses and both levels' bytes equal the
+    sequential functions', including backing spill and failures."""
+    o = oracle_lib
+    nslots = 1 << 14
+    keys = o.gen_keys(71, 0, int(1.02 * nslots))      # overfill: spill and failures
+    vals = ((keys >> np.uint64(9)) & np.uint64(0xFFFF)).astype(np.uint16)
+    a = o.Table(o.REP_FP16_V16, _bench_shape(o, nslots), seed=SEED)
+    b = o.Table(o.REP_FP16_V16, _bench_shape(o, nslots), seed=SEED)
+    st_a = a.bulk_insert(keys, vals)
+    st_b, used = b.bulk_insert_mt(keys, vals, threads=1)
+    assert used == 1
+    assert (st_a == st_b).all() and (st_a == 2).any() and (st_a == 0).any()
+    for lvl in range(2):
+        assert a.level_bytes(lvl) == b.level_bytes(lvl)
+    q = np.concatenate([keys, o.gen_keys(71, 1 << 40, 4096)])
+    f_a, v_a = a.bulk_query(q)
+    f_b, v_b, _ = b.bulk_query_mt(q, threads=1)
+    assert (f_a == f_b).all() and (v_a == v_b).all()
+
+
+def test_multi_thread_run_membership(oracle_lib):
+    """Eight threads: every stored key is found, fill counts the stored keys,
+    threaded queries equal sequential queries of the same table, and the FP
+    rate is the sequential table's within 20 %."""
+    o = oracle_lib
+    nslots = 1 << 20
+    keys = o.gen_keys(72, 0, int(0.9 * nslots))
+    absent = o.gen_keys(72, 1 << 40, 1 << 20)
+    t = o.Table(o.REP_FP16_V16, _bench_shape(o, nslots), seed=SEED)
+    st, used = t.bulk_insert_mt(keys, threads=8)
+    assert used == 8 and (st > 0).mean() > 0.9999
+    assert t.fill() == int((st > 0).sum())
+    f_seq, v_seq = t.bulk_query(np.concatenate([keys, absent]))
+    f_mt, v_mt, _ = t.bulk_query_mt(np.concatenate([keys, absent]), threads=8)
+    assert (f_seq == f_mt).all() and (v_seq == v_mt).all()
+    assert f_seq[:len(keys)][st > 0].all()
+    ref = o.Table(o.REP_FP16_V16, _bench_shape(o, nslots), seed=SEED)
+    ref.bulk_insert(keys)
+    fp_ref = ref.bulk_query(absent)[0].mean()
+    fp_mt = f_mt[len(keys):].mean()
+    assert abs(fp_mt - fp_ref) <= 0.2 * fp_ref, (fp_mt, fp_ref)
+
+
+def test_multi_thread_run_refuses_other_shapes(oracle_lib):
+    o = oracle_lib
+    t = o.Table(o.REP_FP8_V8, [o.level(1 << 10, o.SCHEME_SHORTCUT, 16)], seed=SEED)
+    with pytest.raises(ValueError):
+        t.bulk_insert_mt(o.gen_keys(1, 0, 10))
+    assert o.hw_threads() >= 1
