@@ -604,7 +604,20 @@ def cpu_baseline(log2_slots=26):
             "failed_inserts": int((st == 0).sum()), "seconds": round(m2 - m0, 2),
             "host_threads_online": oracle.hw_threads(),
         },
+        "cpu_model": _cpu_model(),
     }
+
+
+def _cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or "" where it is not readable."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
 
 
 def main():
