@@ -23,6 +23,9 @@
  *     pg_destroy and pg_fill, which synchronise.
  *   - A handle is not safe for concurrent host calls without external
  *     ordering.  Device-side ops within one call are lock-free per slot.
+ *   - A bulk call takes any n: calls needing more than 2^31 work-items
+ *     (n * tile) are issued as several launches, in key order, on the
+ *     same stream.
  *   - Keys 0 and ~0 are reserved (empty / tombstone of the KV slot,
  *     representations/key_val_pair.cuh:70,115-117); filters remap their
  *     fingerprints so any key is accepted there.
