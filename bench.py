@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--summarize-profiles", nargs=3, metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="write profiles/ summaries from rocprofv3 csv output (kernel stats, "
                          "FETCH_SIZE run, WRITE_SIZE run) and exit")
+    ap.add_argument("--summarize-kernel-stats", nargs=2, metavar=("STATS_DIR", "NAME"),
+                    help="write profiles/NAME_kernel_stats.md from a rocprofv3 --stats csv directory "
+                         "(NAME: kv_2p30, batched_4096, serve_64_tables) and exit")
     ap.add_argument("--requests", type=int, default=400, help="serve: requests per experiment")
     ap.add_argument("--workload", choices=["tcf", "batched", "kv", "serve"], default="tcf",
                     help="tcf: the headline 2^28-slot filter; batched: 4096 uneven filters in one graph; "
@@ -545,6 +548,62 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
     print("\n".join(lines))
 
 
+WORKLOAD_PROFILES = {
+    # name: (title, rocprofv3 command); summarised with --summarize-kernel-stats DIR NAME
+    "kv_2p30": ("KV table 2^30 slots x 16 B, 0.9 load, double hashing, B 16, tile 4",
+                "rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_kv -o kv -- "
+                "python3 bench.py --workload kv --steps 2 --warmup 1"),
+    "batched_4096": ("4096 batched TCFs (one hipGraph per step; the autotuner's sweep runs first, "
+                     "so every swept shape appears; the shape it picked for the bench line is listed "
+                     "after the top kernels)",
+                     "rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_batch -o batch -- "
+                     "python3 bench.py --workload batched --steps 3 --warmup 1 > gpurun_out/prof_batch.log"),
+    "serve_64_tables": ("serving loop over 64 resident tables against per-request plain launches",
+                        "rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_serve -o serve -- "
+                        "python3 bench.py --workload serve --requests 200"),
+}
+
+
+def summarize_kernel_stats(stats_dir, name):
+    """profiles/<name>_kernel_stats.md from a `rocprofv3 --kernel-trace
+    --stats` csv directory of one of the WORKLOAD_PROFILES runs."""
+    title, cmd = WORKLOAD_PROFILES[name]
+    stats = _csv_rows(stats_dir, "_kernel_stats.csv")
+    lines = [f"# {title}: rocprofv3 kernel statistics", "",
+             "Commands (run from the repository root on one MI355X):", "", f"    {cmd}",
+             f"    python3 bench.py --summarize-kernel-stats {stats_dir} {name}", "",
+             "| kernel | calls | average us | total ms | % |", "|---|---|---|---|---|"]
+    ranked = sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))
+    row = lambda r: (f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.2f} |")
+    lines += [row(r) for r in ranked[:14]]
+    if name == "batched_4096":
+        # the shape the autotuner picked: the bench line (in the run's
+        # log, next to the csv directory) names it
+        import re
+        shape = None
+        for path in (os.path.join(stats_dir, "..", "prof_batch.log"),):
+            try:
+                with open(path) as f:
+                    for line in f:
+                        if line.startswith("{"):
+                            c = json.loads(line)["config"]
+                            shape = (c["tile"], c["bucket"])
+            except (OSError, ValueError, KeyError):
+                pass
+        if shape:
+            pat = re.compile(rf"k_batch_persistent<{shape[0]}, {shape[1]}, ")
+            lines += ["", f"The picked shape (tile {shape[0]}, bucket {shape[1]}), sweep replays and "
+                          "timed replays together:", "",
+                      "| kernel | calls | average us | total ms | % |", "|---|---|---|---|---|"]
+            lines += [row(r) for r in ranked if pat.search(r["Name"])]
+    lines.append("")
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    with open(os.path.join(ROOT, "profiles", f"{name}_kernel_stats.md"), "w") as f:
+        f.write("\n".join(lines))
+    print("\n".join(lines))
+
+
 def load_traffic(path):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
     WRITE_SIZE), committed as a small JSON summary; None when absent."""
@@ -624,6 +683,9 @@ def main():
     args = parse()
     if args.summarize_profiles:
         summarize_profiles(*args.summarize_profiles)
+        return
+    if args.summarize_kernel_stats:
+        summarize_kernel_stats(*args.summarize_kernel_stats)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU: run with --gpus 1 (multi-GPU sharding "
