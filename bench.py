@@ -309,21 +309,23 @@ def run_batched(args):
     """4096 independent TCFs, slots log-uniform in [2^12, 2^22], each filled to
     85 % and queried with as many keys (half present, half absent), all in one
     hipGraph replay of a persistent insert kernel and a persistent query
-    kernel; tile width, bucket size and work chunk picked by the autotuner on
-    a 512-table sample.  Compared with plain per-table launches."""
+    kernel; tile width, bucket size, work chunk and keys per tile in flight
+    picked by the autotuner on a 512-table sample.  Compared with plain
+    per-table launches."""
     import numpy as np
     import torch
     import poggers_amd as pg
     from poggers_amd.autotune import tune_batch
-    from poggers_amd.workloads import batched_workload, log_uniform_sizes
+    from poggers_amd.workloads import absent_mask, batched_workload, log_uniform_sizes
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
     sizes = log_uniform_sizes(args.tables, 12, 22, seed=11)
     log = lambda m: print(m, file=sys.stderr, flush=True)
     sample = sizes[:min(512, len(sizes))]
-    s_ins, s_off, s_q, s_qoff, _, _ = batched_workload(sample, 0.85, 77, dev)
-    tune = tune_batch(sample, s_ins, s_off, s_q, s_qoff, chunks=(512, 2048, 8192), reps=2, log=log)
+    s_ins, s_off, s_q, s_qoff, s_hoff, s_npres = batched_workload(sample, 0.85, 77, dev)
+    tune = tune_batch(sample, s_ins, s_off, s_q, s_qoff, chunks=(512, 2048, 8192), reps=2,
+                      absent=absent_mask(s_hoff, s_npres, dev), log=log)
     del s_ins, s_off, s_q, s_qoff
     best = tune["best"]
     ins, off, q, qoff, hoff, npres = batched_workload(sizes, 0.85, 77, dev)
@@ -331,6 +333,7 @@ def run_batched(args):
     desc = pg.tcf_desc(1 << 12, 1 << 6, bucket_size=best["bucket"], tile=best["tile"])
     batch = pg.Batch(desc, sizes)
     batch.set_launch(best["chunk"])
+    batch.set_items(best["items"])
     found = torch.zeros(total, dtype=torch.uint8, device=dev)
     fails = torch.zeros(1, dtype=torch.int64, device=dev)
     stream = torch.cuda.Stream()
@@ -350,10 +353,7 @@ def run_batched(args):
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     gpu_ms = e0.elapsed_time(e1) / args.steps
-    present = torch.zeros(total, dtype=torch.bool, device=dev)
-    for t in range(len(sizes)):
-        if npres[t]:
-            present[int(hoff[t]):int(hoff[t]) + int(npres[t])] = True
+    present = ~absent_mask(hoff, npres, dev)
     false_neg = int((found[present] == 0).sum().item())
     fp_rate = float(found[~present].float().mean().item())
     insert_fails = int(fails.item()) // max(1, args.steps)
@@ -397,7 +397,8 @@ def run_batched(args):
         "config": {"workload": f"{len(sizes)} TCFs, slots log-uniform in [2^12, 2^22] "
                                f"({int(sizes.sum())} slots), insert to 85% ({total} keys), query {total} keys "
                                f"(half present, half absent)",
-                   "tile": best["tile"], "bucket": best["bucket"], "chunk": best["chunk"]},
+                   "tile": best["tile"], "bucket": best["bucket"], "chunk": best["chunk"],
+                   "items": best["items"]},
         "false_negatives": false_neg, "fp_rate": fp_rate, "insert_failures": insert_fails,
         "plain_launches_ms": round(plain_ms, 3),
         "plain_launch_shape": {"tile": PLAIN_TILE, "bucket": PLAIN_BUCKET},

@@ -278,6 +278,11 @@ int pg_batch_destroy(pg_batch b);
 int pg_batch_set_mode(pg_batch b, uint32_t mode);
 /* Persistent launch shape: keys per work chunk and blocks (0 = fill the GPU). */
 int pg_batch_set_launch(pg_batch b, uint32_t chunk, uint32_t blocks);
+/* Keys each tile has in flight in the persistent kernels (1, 2 or 4; 0 = 1):
+ * the first-choice bucket loads of that many keys are issued before any is
+ * examined.  The reference has one key per tile (test_mhm_tcf.cu:318-487);
+ * like the tile width, a launch-shape choice that leaves results unchanged. */
+int pg_batch_set_items(pg_batch b, uint32_t items);
 int pg_batch_clear(pg_batch b, void *stream);
 int pg_batch_insert(pg_batch b, const uint64_t *d_keys, const uint64_t *d_offsets, uint64_t total,
                     const uint16_t *d_vals, uint8_t *d_status, uint64_t *d_fail_count, void *stream);

@@ -35,7 +35,8 @@ EXPORTS = [
     "pg_bytes", "pg_num_slots", "pg_level_ptr", "pg_error_string", "pg_gen_keys",
     "pg_random_sector_bench", "pg_fastmod_host", "pg_set_stats", "pg_set_items",
     "pg_bulk_insert_if_not_exists", "pg_bulk_insert_if_not_exists_delete",
-    "pg_batch_create", "pg_batch_destroy", "pg_batch_set_mode", "pg_batch_set_launch", "pg_batch_clear",
+    "pg_batch_create", "pg_batch_destroy", "pg_batch_set_mode", "pg_batch_set_launch", "pg_batch_set_items",
+    "pg_batch_clear",
     "pg_batch_insert", "pg_batch_query", "pg_batch_capture", "pg_batch_replay", "pg_batch_fill",
     "pg_batch_level_ptr", "pg_batch_bytes",
     "pg_server_create", "pg_server_destroy", "pg_server_submit", "pg_server_progress", "pg_server_pending",
@@ -117,6 +118,7 @@ def lib():
     L.pg_batch_destroy.argtypes = [vp]
     L.pg_batch_set_mode.argtypes = [vp, u32]
     L.pg_batch_set_launch.argtypes = [vp, u32, u32]
+    L.pg_batch_set_items.argtypes = [vp, u32]
     L.pg_batch_clear.argtypes = [vp, vp]
     L.pg_batch_insert.argtypes = [vp, vp, vp, u64, vp, vp, vp, vp]
     L.pg_batch_query.argtypes = [vp, vp, vp, u64, vp, vp, vp]
@@ -402,6 +404,10 @@ class Batch:
 
     def set_launch(self, chunk: int, blocks: int = 0):
         check(lib().pg_batch_set_launch(self._h, chunk, blocks), "pg_batch_set_launch")
+
+    def set_items(self, items: int):
+        """Keys each tile keeps in flight in the persistent kernels (1, 2, 4)."""
+        check(lib().pg_batch_set_items(self._h, items), "pg_batch_set_items")
 
     def clear(self, stream=None):
         check(lib().pg_batch_clear(self._h, _stream(stream)), "pg_batch_clear")

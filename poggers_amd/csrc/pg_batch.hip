@@ -52,7 +52,108 @@ __global__ __launch_bounds__(64) void k_batch_reset(uint64_t *counter) {
     if (threadIdx.x == 0) counter[0] = 0;
 }
 
+// One key of a batch: the single-table per-key code on the key's table.
 template <int P, int B, bool QUERY>
+__device__ inline void batch_one(const BatchArgs &a, uint64_t i, uint32_t &tb, const Tile<P> &t, uint64_t &fails) {
+    while (a.off[tb + 1] <= i) tb++;
+    const TableDev &T = a.tables[tb];
+    const uint64_t key = a.keys[i];
+    if constexpr (QUERY) {
+        const uint32_t fp = fp16_of(key);
+        uint32_t val = 0;
+        bool own = false, found = false;
+        if constexpr (B != 0) {
+            const LevelDev &L = T.lv[0];
+            Probe pr;
+            pr.begin(L, key, fp);
+            for (uint32_t j = 0; j < L.max_probes && !found; j++) {
+                const uint64_t p = pr.at(j);
+                if (p == kEnd) break;
+                Bucket<P, B> bk;
+                bk.load((const uint32_t *)L.slots, fastmod(p, L.nb), t);
+                found = bk.find(fp, t, val, own);
+            }
+        } else {
+            found = gen_level_query<P>(T.lv[0], key, fp, t, val, own, 0);
+        }
+        for (uint32_t l = 1; l < T.nlevels && !found; l++)
+            found = gen_level_query<P>(T.lv[l], key, fp, t, val, own, 1);
+        const bool writer = found ? own : t.leader();
+        if (writer) {
+            if (a.flags) a.flags[i] = found ? 1 : 0;
+            if (a.vals_out) a.vals_out[i] = found ? (uint16_t)val : 0;
+        }
+    } else {
+        const uint32_t v = a.vals ? a.vals[i] : 0u;
+        const int st = tcf_insert_key<P, B>(T, key, v, false, t);
+        if (a.flags && t.leader()) a.flags[i] = (uint8_t)st;
+        fails += (st == 0 && t.leader());
+    }
+}
+
+// I keys of one tile at once (register-bucket shapes, B != 0): every key's
+// table is found and its first probe bucket loaded before any is examined,
+// so a tile has I first-choice reads in flight; then each key runs the
+// single-table code from that loaded bucket.
+template <int P, int B, bool QUERY, int I>
+__device__ inline void batch_many(const BatchArgs &a, uint64_t i0, uint64_t end, uint32_t stride, uint32_t &tb,
+                                  const Tile<P> &t, uint64_t &fails) {
+    const TableDev *T[I];
+    uint64_t idx[I], key[I], b0[I];
+    uint32_t fp[I];
+    bool live[I];
+    Probe pr[I];
+    Bucket<P, B> bk[I];
+#pragma unroll
+    for (int k = 0; k < I; k++) {
+        idx[k] = i0 + (uint64_t)k * stride;
+        live[k] = idx[k] < end;
+        if (live[k])
+            while (a.off[tb + 1] <= idx[k]) tb++;
+        T[k] = &a.tables[tb];
+        key[k] = live[k] ? a.keys[idx[k]] : 0;
+        fp[k] = fp16_of(key[k]);
+        const LevelDev &L = T[k]->lv[0];
+        pr[k].begin(L, key[k], fp[k]);
+        const uint64_t p0 = pr[k].at(0);
+        b0[k] = (p0 == kEnd) ? 0 : fastmod(p0, L.nb);
+        if (live[k] && p0 != kEnd) bk[k].load((const uint32_t *)L.slots, b0[k], t);
+    }
+#pragma unroll
+    for (int k = 0; k < I; k++) {
+        if (!live[k]) continue;
+        const TableDev &Tk = *T[k];
+        const uint64_t i = idx[k];
+        if constexpr (QUERY) {
+            const LevelDev &L = Tk.lv[0];
+            uint32_t val = 0;
+            bool own = false;
+            bool found = pr[k].at(0) != kEnd && bk[k].find(fp[k], t, val, own);
+            for (uint32_t j = 1; j < L.max_probes && !found; j++) {
+                const uint64_t p = pr[k].at(j);
+                if (p == kEnd) break;
+                Bucket<P, B> bj;
+                bj.load((const uint32_t *)L.slots, fastmod(p, L.nb), t);
+                found = bj.find(fp[k], t, val, own);
+            }
+            for (uint32_t l = 1; l < Tk.nlevels && !found; l++)
+                found = gen_level_query<P>(Tk.lv[l], key[k], fp[k], t, val, own, 1);
+            const bool writer = found ? own : t.leader();
+            if (writer) {
+                if (a.flags) a.flags[i] = found ? 1 : 0;
+                if (a.vals_out) a.vals_out[i] = found ? (uint16_t)val : 0;
+            }
+        } else {
+            const uint32_t v = a.vals ? a.vals[i] : 0u;
+            const int st = tcf_insert_prepared<P, B>(Tk, key[k], fp[k], (v << 16) | fp[k], false, t, pr[k],
+                                                     bk[k], b0[k]);
+            if (a.flags && t.leader()) a.flags[i] = (uint8_t)st;
+            fails += (st == 0 && t.leader());
+        }
+    }
+}
+
+template <int P, int B, bool QUERY, int I>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(BatchArgs a) {
     const uint64_t *off = a.off;
     Tile<P> t;
@@ -71,41 +172,12 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(BatchArgs a) {
         if (base >= a.total) break;
         const uint64_t end = (base + a.chunk < a.total) ? base + a.chunk : a.total;
         uint32_t tb = tb0;
-        for (uint64_t i = base + tile_in_wave; i < end; i += kTilesPerWave) {
-            while (off[tb + 1] <= i) tb++;
-            const TableDev &T = a.tables[tb];
-            const uint64_t key = a.keys[i];
-            if constexpr (QUERY) {
-                const uint32_t fp = fp16_of(key);
-                uint32_t val = 0;
-                bool own = false, found = false;
-                if constexpr (B != 0) {
-                    const LevelDev &L = T.lv[0];
-                    Probe pr;
-                    pr.begin(L, key, fp);
-                    for (uint32_t j = 0; j < L.max_probes && !found; j++) {
-                        const uint64_t p = pr.at(j);
-                        if (p == kEnd) break;
-                        Bucket<P, B> bk;
-                        bk.load((const uint32_t *)L.slots, fastmod(p, L.nb), t);
-                        found = bk.find(fp, t, val, own);
-                    }
-                } else {
-                    found = gen_level_query<P>(T.lv[0], key, fp, t, val, own, 0);
-                }
-                for (uint32_t l = 1; l < T.nlevels && !found; l++)
-                    found = gen_level_query<P>(T.lv[l], key, fp, t, val, own, 1);
-                const bool writer = found ? own : t.leader();
-                if (writer) {
-                    if (a.flags) a.flags[i] = found ? 1 : 0;
-                    if (a.vals_out) a.vals_out[i] = found ? (uint16_t)val : 0;
-                }
-            } else {
-                const uint32_t v = a.vals ? a.vals[i] : 0u;
-                const int st = tcf_insert_key<P, B>(T, key, v, false, t);
-                if (a.flags && t.leader()) a.flags[i] = (uint8_t)st;
-                fails += (st == 0 && t.leader());
-            }
+        if constexpr (I == 1 || B == 0) {
+            for (uint64_t i = base + tile_in_wave; i < end; i += kTilesPerWave)
+                batch_one<P, B, QUERY>(a, i, tb, t, fails);
+        } else {
+            for (uint64_t i0 = base + tile_in_wave; i0 < end; i0 += (uint64_t)kTilesPerWave * I)
+                batch_many<P, B, QUERY, I>(a, i0, end, kTilesPerWave, tb, t, fails);
         }
     }
     if (!QUERY && a.fail_count && fails) atomicAdd((unsigned long long *)a.fail_count, fails);
@@ -131,42 +203,57 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_insert_ordered(BatchArgs 
     if (a.fail_count && t.leader() && fails) atomicAdd((unsigned long long *)a.fail_count, fails);
 }
 
+template <int P, int B, bool QUERY>
+static void launch_persistent(const BatchArgs &a, unsigned blocks, uint32_t items, hipStream_t s) {
+    if constexpr (B != 0) {
+        if (items == 4) {
+            hipLaunchKernelGGL((k_batch_persistent<P, B, QUERY, 4>), dim3(blocks), dim3(kBatchBlock), 0, s, a);
+            return;
+        }
+        if (items == 2) {
+            hipLaunchKernelGGL((k_batch_persistent<P, B, QUERY, 2>), dim3(blocks), dim3(kBatchBlock), 0, s, a);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_batch_persistent<P, B, QUERY, 1>), dim3(blocks), dim3(kBatchBlock), 0, s, a);
+}
+
 template <int P, int B>
 static int launch_batch_pb(const BatchArgs &a, bool query, bool ordered, unsigned persistent_blocks,
-                           hipStream_t s) {
+                           uint32_t items, hipStream_t s) {
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, a.counter);
     if (ordered && !query) {
         const unsigned grid = (unsigned)(((uint64_t)a.ntables * P + kBatchBlock - 1) / kBatchBlock);
         hipLaunchKernelGGL((k_batch_insert_ordered<P, B>), dim3(grid), dim3(kBatchBlock), 0, s, a);
     } else if (query) {
-        hipLaunchKernelGGL((k_batch_persistent<P, B, true>), dim3(persistent_blocks), dim3(kBatchBlock), 0, s, a);
+        launch_persistent<P, B, true>(a, persistent_blocks, items, s);
     } else {
-        hipLaunchKernelGGL((k_batch_persistent<P, B, false>), dim3(persistent_blocks), dim3(kBatchBlock), 0, s, a);
+        launch_persistent<P, B, false>(a, persistent_blocks, items, s);
     }
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
 }
 
 template <int P>
 static int launch_batch_p(uint32_t B, const BatchArgs &a, bool query, bool ordered, unsigned blocks,
-                          hipStream_t s) {
+                          uint32_t items, hipStream_t s) {
     switch (B) {
-        case 8: return launch_batch_pb<P, 8>(a, query, ordered, blocks, s);
-        case 16: return launch_batch_pb<P, 16>(a, query, ordered, blocks, s);
-        case 32: return launch_batch_pb<P, 32>(a, query, ordered, blocks, s);
-        default: return launch_batch_pb<P, 0>(a, query, ordered, blocks, s);
+        case 8: return launch_batch_pb<P, 8>(a, query, ordered, blocks, items, s);
+        case 16: return launch_batch_pb<P, 16>(a, query, ordered, blocks, items, s);
+        case 32: return launch_batch_pb<P, 32>(a, query, ordered, blocks, items, s);
+        default: return launch_batch_pb<P, 0>(a, query, ordered, blocks, items, s);
     }
 }
 
 static int launch_batch(uint32_t P, uint32_t B, const BatchArgs &a, bool query, bool ordered, unsigned blocks,
-                        hipStream_t s) {
+                        uint32_t items, hipStream_t s) {
     switch (P) {
-        case 4: return launch_batch_p<4>(B, a, query, ordered, blocks, s);
-        case 8: return launch_batch_p<8>(B, a, query, ordered, blocks, s);
-        case 16: return launch_batch_p<16>(B, a, query, ordered, blocks, s);
-        case 32: return launch_batch_p<32>(B, a, query, ordered, blocks, s);
-        case 64: return launch_batch_p<64>(B, a, query, ordered, blocks, s);
-        case 2: return launch_batch_p<2>(B, a, query, ordered, blocks, s);
-        case 1: return launch_batch_p<1>(B, a, query, ordered, blocks, s);
+        case 4: return launch_batch_p<4>(B, a, query, ordered, blocks, items, s);
+        case 8: return launch_batch_p<8>(B, a, query, ordered, blocks, items, s);
+        case 16: return launch_batch_p<16>(B, a, query, ordered, blocks, items, s);
+        case 32: return launch_batch_p<32>(B, a, query, ordered, blocks, items, s);
+        case 64: return launch_batch_p<64>(B, a, query, ordered, blocks, items, s);
+        case 2: return launch_batch_p<2>(B, a, query, ordered, blocks, items, s);
+        case 1: return launch_batch_p<1>(B, a, query, ordered, blocks, items, s);
         default: return PG_ERR_INVALID;
     }
 }
@@ -181,6 +268,7 @@ struct pg_batch_s {
     uint32_t ntables;
     uint32_t mode;
     uint32_t chunk;
+    uint32_t items;            // keys per tile in flight (1, 2, 4)
     unsigned blocks;           // persistent grid
     void *arena;               // every table's slot arrays
     uint64_t arena_bytes;
@@ -238,6 +326,7 @@ int pg_batch_create(const pg_desc *proto, uint32_t ntables, const uint64_t *nslo
     b->ntables = ntables;
     b->mode = PG_CONCURRENT;
     b->chunk = 1024;
+    b->items = 1;
     b->blocks = persistent_grid();
     b->h_tables.resize(ntables);
     b->level_bytes.resize((size_t)ntables * PG_MAX_LEVELS, 0);
@@ -327,6 +416,12 @@ int pg_batch_set_launch(pg_batch b, uint32_t chunk, uint32_t blocks) {
     return PG_OK;
 }
 
+int pg_batch_set_items(pg_batch b, uint32_t items) {
+    if (!b || !(items == 0 || items == 1 || items == 2 || items == 4)) return PG_ERR_INVALID;
+    b->items = items ? items : 1;
+    return PG_OK;
+}
+
 int pg_batch_clear(pg_batch b, void *stream) {
     if (!b) return PG_ERR_INVALID;
     return pg::launch_zero(b->arena, b->arena_bytes, (hipStream_t)stream);
@@ -349,7 +444,7 @@ static int batch_op(pg_batch b, bool query, const uint64_t *d_keys, const uint64
     a.counter = b->d_counter;
     a.chunk = b->chunk;
     return pg::launch_batch(b->proto.level[0].tile, b->proto.level[0].bucket_size, a, query,
-                            b->mode == PG_DETERMINISTIC, b->blocks, (hipStream_t)stream);
+                            b->mode == PG_DETERMINISTIC, b->blocks, b->items, (hipStream_t)stream);
 }
 
 int pg_batch_insert(pg_batch b, const uint64_t *d_keys, const uint64_t *d_offsets, uint64_t total,

@@ -53,3 +53,12 @@ def batched_workload(sizes: np.ndarray, load: float, seed: int, device="cuda"):
         a += m
     d_off = torch.from_numpy(off.astype(np.int64)).to(device)
     return ins, d_off, q, d_off.clone(), off, npres
+
+
+def absent_mask(off: np.ndarray, npres: np.ndarray, device="cuda"):
+    """Bool tensor over a batched_workload query array: True for the absent
+    keys (each table's queries after its first npres present ones)."""
+    m = np.ones(int(off[-1]), dtype=bool)
+    for t in range(len(npres)):
+        m[int(off[t]):int(off[t]) + int(npres[t])] = False
+    return torch.from_numpy(m).to(device)

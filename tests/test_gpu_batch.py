@@ -164,13 +164,76 @@ def test_persistent_narrow_tiles(tile, B):
     assert found[absent].mean() < 2 * B * 2.0 ** -16 * 1.5
 
 
+@pytest.mark.parametrize("tile,B,items", [(1, 8, 2), (4, 16, 2), (4, 16, 4), (2, 32, 4), (16, 16, 4)])
+def test_persistent_items_per_tile(tile, B, items):
+    """Several keys per tile in flight: the same answers as one key per tile
+    (membership and values are placement-independent for stored keys; the
+    absent-key answers agree up to placement-dependent false positives).
+    4096-key chunks keep few keys of one small table in flight at once; with
+    small chunks, extra keys per tile raise the spill into the backing level
+    and with it the false-positive rate, which is why the autotuner screens
+    shapes by their measured rate."""
+    desc = proto(tile, B)
+    sizes, ins, ins_off, q, q_off = make_workload(256, 10, 15, 0.85, 15)
+    vals = ((ins >> np.uint64(5)) & np.uint64(0xFFFF)).astype(np.uint16)
+    res = []
+    for it in (1, items):
+        b = pg.Batch(desc, sizes)
+        b.set_launch(4096)
+        b.set_items(it)
+        st = torch.zeros(len(ins), dtype=torch.uint8, device="cuda")
+        fails = torch.zeros(1, dtype=torch.int64, device="cuda")
+        b.insert(dev_u64(ins), dev_u64(ins_off), torch.from_numpy(vals.view(np.int16)).cuda(), st, fails)
+        found = torch.zeros(len(q), dtype=torch.uint8, device="cuda")
+        out = torch.zeros(len(q), dtype=torch.int16, device="cuda")
+        b.query(dev_u64(q), dev_u64(q_off), out, found)
+        torch.cuda.synchronize()
+        st = st.cpu().numpy()
+        fill_ok = all(b.fill(t) == int((st[int(ins_off[t]):int(ins_off[t + 1])] > 0).sum())
+                      for t in range(0, len(sizes), 37))
+        res.append((int(fails.item()), st, found.cpu().numpy(), host(out, np.uint16), fill_ok))
+        b.free()
+    (f1, st1, fd1, v1, ok1), (fn, stn, fdn, vn, okn) = res
+    # 1-lane tiles over 8-slot buckets overflow a few small backing levels
+    # under concurrency (see test_persistent_narrow_tiles)
+    assert f1 == int((st1 == 0).sum()) <= len(ins) // 10_000
+    assert fn == int((stn == 0).sum()) <= len(ins) // 10_000
+    assert ok1 and okn
+    pres = np.zeros(len(q), dtype=bool)
+    qv = np.zeros(len(q), dtype=np.uint16)
+    stored1 = np.zeros(len(q), dtype=bool)
+    storedn = np.zeros(len(q), dtype=bool)
+    for t in range(len(sizes)):
+        a, qa = int(ins_off[t]), int(q_off[t])
+        npres = (int(ins_off[t + 1]) - a) // 2
+        pres[qa:qa + npres] = True
+        qv[qa:qa + npres] = vals[a:a + npres]
+        stored1[qa:qa + npres] = st1[a:a + npres] > 0
+        storedn[qa:qa + npres] = stn[a:a + npres] > 0
+    assert fd1[stored1].all() and fdn[storedn].all()
+    # a stored key's value is its own unless another key with its tag shares a bucket
+    assert (v1[pres] == qv[pres]).mean() > 0.999 and (vn[pres] == qv[pres]).mean() > 0.999
+    assert abs(int(fdn[~pres].sum()) - int(fd1[~pres].sum())) <= 0.15 * max(100, int(fd1[~pres].sum()))
+
+
 def test_autotune_picks_a_lossless_shape():
+    """The tuner's pick: the fastest shape that failed no insert and whose
+    false-positive rate is within 5 % of the lowest measured for its bucket
+    size."""
     from poggers_amd.autotune import tune_batch
     sizes, ins, ins_off, q, q_off = make_workload(64, 10, 14, 0.85, 13)
+    absent = np.ones(len(q), dtype=bool)
+    for t in range(len(sizes)):
+        absent[int(q_off[t]):int(q_off[t]) + (int(ins_off[t + 1]) - int(ins_off[t])) // 2] = False
     tune = tune_batch(sizes, dev_u64(ins), dev_u64(ins_off), dev_u64(q), dev_u64(q_off), tiles=(1, 4),
-                      buckets=(8, 16), chunks=(256, 2048), reps=1)
-    assert len(tune["results"]) == 8
+                      buckets=(8, 16), chunks=(256, 2048), items=(2, 4), refine=2, reps=1,
+                      absent=torch.from_numpy(absent).cuda())
+    res = tune["results"]
+    assert len(res) == 8 + 2 * 2
+    assert {r["items"] for r in res} == {1, 2, 4}
+    lossless = [r for r in res if r["insert_failures"] == 0]
+    floor = {b: min(r["fp_rate"] for r in lossless if r["bucket"] == b) for b in {r["bucket"] for r in lossless}}
+    qualified = [r for r in lossless if r["fp_rate"] <= 1.05 * floor[r["bucket"]]]
     best = tune["best"]
-    assert best["insert_failures"] == 0
-    assert best["ms"] == min(r["ms"] for r in tune["results"] if r["insert_failures"] == 0)
+    assert best in qualified and best["ms"] == min(r["ms"] for r in qualified)
     assert (best["tile"], best["bucket"]) in {(1, 8), (1, 16), (4, 8), (4, 16)} and best["chunk"] in (256, 2048)
