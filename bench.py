@@ -41,9 +41,9 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB streaming read+write kernel and 2^24 random 64 B reads "
                          "of a 1 GiB buffer (known byte counts that scale the PMC byte counters)")
-    ap.add_argument("--summarize-profiles", nargs=3, metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR"),
+    ap.add_argument("--summarize-profiles", nargs="+", metavar="DIR",
                     help="write profiles/ summaries from rocprofv3 csv output (kernel stats, "
-                         "FETCH_SIZE run, WRITE_SIZE run) and exit")
+                         "FETCH_SIZE run, WRITE_SIZE run, and optionally the TCC counter run) and exit")
     ap.add_argument("--summarize-kernel-stats", nargs=2, metavar=("STATS_DIR", "NAME"),
                     help="write profiles/NAME_kernel_stats.md from a rocprofv3 --stats csv directory "
                          "(NAME: kv_2p30, batched_4096, serve_64_tables) and exit")
@@ -417,8 +417,10 @@ PROFILE_CMDS = {
              "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-roofline --calibrate-copy",
     "write": "rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/pmc_write -o tcf -- "
              "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-roofline --calibrate-copy",
+    "tcc": "rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum TCC_ATOMIC_sum TCC_EA0_ATOMIC_sum -f csv "
+           "-d gpurun_out/pmc_tcc -o tcf -- python3 bench.py --steps 1 --warmup 0 --no-cpu --no-roofline",
     "summary": "python3 bench.py --summarize-profiles gpurun_out/prof_stats gpurun_out/pmc_fetch "
-               "gpurun_out/pmc_write",
+               "gpurun_out/pmc_write gpurun_out/pmc_tcc",
 }
 
 
@@ -436,7 +438,7 @@ def _short(name, n=110):
     return name if len(name) <= n else name[:n - 3] + "..."
 
 
-def summarize_profiles(stats_dir, fetch_dir, write_dir):
+def summarize_profiles(stats_dir, fetch_dir, write_dir, tcc_dir=None):
     """Condense rocprofv3 csv output into the committed text summaries:
     profiles/tcf_2p28_kernel_stats.md (per-kernel launch statistics, and the
     per-dispatch HBM counters with their calibration) and
@@ -503,6 +505,34 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
     copy_scaled = fetch_b * fetch_factor + write_b * write_factor
     mixed = ((fetch_b - key_bytes / fetch_factor) * rand_factor + key_bytes + write_b * write_factor
              if rand_factor else copy_scaled)
+    # L2 (TCC) hit rate and atomics per dispatch, from the four-counter run
+    tcc = {}
+    if tcc_dir:
+        rows_t = _csv_rows(tcc_dir, "_counter_collection.csv")
+        avg_ns = {r["Name"]: float(r["AverageNs"]) for r in stats}
+        for key, label in (("k_tcf_insert", "insert"), ("k_tcf_query", "query")):
+            per = {}
+            names = set()
+            for r in rows_t:
+                n = r["Kernel_Name"]
+                if key in n and ", true" not in n:
+                    names.add(n)
+                    per.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+            if not names:
+                continue
+            m = {c: sum(v) / len(v) for c, v in per.items()}
+            hit, miss = m.get("TCC_HIT_sum", 0.0), m.get("TCC_MISS_sum", 0.0)
+            ns = next((avg_ns[n] for n in names if n in avg_ns), None)
+            atom = m.get("TCC_ATOMIC_sum", 0.0)
+            tcc[label] = {
+                "kernel_name": sorted(names)[0],
+                "dispatches": len(per.get("TCC_HIT_sum", [])),
+                "tcc_hit_rate": round(hit / (hit + miss), 4) if hit + miss else None,
+                "tcc_requests_per_launch": int(hit + miss),
+                "atomics_per_launch": int(atom),
+                "atomics_to_memory_per_launch": int(m.get("TCC_EA0_ATOMIC_sum", 0.0)),
+                "atomics_per_s": round(atom / (ns * 1e-9), 1) if ns else None,
+            }
     traffic = {
         "kernel": "insert",
         "kernel_name": ins[1],
@@ -515,11 +545,12 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
         "random64_fetch_factor": round(rand_factor, 4) if rand_factor else None,
         "hbm_bytes_per_launch": int(mixed),
         "hbm_bytes_per_launch_all_streaming_scaled": int(copy_scaled),
-        "commands": [PROFILE_CMDS["fetch"], PROFILE_CMDS["write"], PROFILE_CMDS["summary"]],
+        "commands": [PROFILE_CMDS["fetch"], PROFILE_CMDS["write"], PROFILE_CMDS["tcc"], PROFILE_CMDS["summary"]],
+        "tcc": tcc or None,
     }
     lines = ["# TCF 2^28 slots, 90 % load: rocprofv3 summaries", "",
              "Commands (run from the repository root on one MI355X):", ""]
-    lines += [f"    {PROFILE_CMDS[k]}" for k in ("stats", "fetch", "write", "summary")]
+    lines += [f"    {PROFILE_CMDS[k]}" for k in ("stats", "fetch", "write", "tcc", "summary")]
     lines += ["", "## Kernel statistics (`--kernel-trace --stats`)", "",
               "Kernels whose fourth template argument is `true` are the accounting variants,",
               "run once, untimed, to count bucket reads; the timed kernels are the `false` ones.", "",
@@ -541,6 +572,19 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
               f"= {traffic['hbm_bytes_per_launch']:,} ({traffic['hbm_bytes_per_launch'] / keys_per_launch:.1f} B "
               f"per key); with every read scaled as streaming: "
               f"{traffic['hbm_bytes_per_launch_all_streaming_scaled']:,}.", ""]
+    if tcc:
+        lines += ["## L2 (TCC) hit rate and atomics (`--pmc TCC_HIT_sum TCC_MISS_sum TCC_ATOMIC_sum "
+                  "TCC_EA0_ATOMIC_sum`)", "",
+                  "Per dispatch of the timed kernels; atomics/s divides by the kernel's average duration "
+                  "from the statistics run.", "",
+                  "| kernel | dispatches | TCC requests | hit rate | atomics | atomics to memory | atomics/s |",
+                  "|---|---|---|---|---|---|---|"]
+        for label, d in tcc.items():
+            aps = f"{d['atomics_per_s'] / 1e9:.2f} G" if d["atomics_per_s"] else "-"
+            lines.append(f"| {label} `{_short(d['kernel_name'], 60)}` | {d['dispatches']} | "
+                         f"{d['tcc_requests_per_launch']:,} | {d['tcc_hit_rate']} | {d['atomics_per_launch']:,} | "
+                         f"{d['atomics_to_memory_per_launch']:,} | {aps} |")
+        lines.append("")
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     with open(os.path.join(ROOT, "profiles", "tcf_2p28_kernel_stats.md"), "w") as f:
         f.write("\n".join(lines))
@@ -683,6 +727,8 @@ def _cpu_model() -> str:
 def main():
     args = parse()
     if args.summarize_profiles:
+        if len(args.summarize_profiles) not in (3, 4):
+            raise SystemExit("--summarize-profiles STATS_DIR FETCH_DIR WRITE_DIR [TCC_DIR]")
         summarize_profiles(*args.summarize_profiles)
         return
     if args.summarize_kernel_stats:
@@ -892,6 +938,10 @@ def main():
         "avg_launch_ms": round(acc[dom] / max(1, launches[dom]), 4),
         "measured_random_64B_gbs": rand,
     }
+    tcc_dom = ((traffic_info or {}).get("tcc") or {}).get("insert" if dom == "insert" else "query")
+    if tcc_dom:
+        roofline["tcc_hit_rate"] = tcc_dom["tcc_hit_rate"]
+        roofline["atomics_per_s"] = tcc_dom["atomics_per_s"]
 
     ops_per_step = n_ins + 2 * nq
     value = ops_per_step / (ms_per_step / 1e3) / 1e6
