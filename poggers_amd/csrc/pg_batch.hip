@@ -157,6 +157,7 @@ template <int P, int B, bool QUERY, int I>
 __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(BatchArgs a) {
     const uint64_t *off = a.off;
     Tile<P> t;
+    t.concurrent = true;
     constexpr uint32_t kTilesPerWave = 64 / P;
     const uint32_t tile_in_wave = (threadIdx.x & 63) / P;
     uint64_t fails = 0;

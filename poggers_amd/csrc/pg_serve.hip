@@ -126,6 +126,7 @@ __device__ inline void serve_one(const TableDev &T, const ReqDev &r, uint64_t j,
 template <int P>
 __global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
     Tile<P> t;
+    t.concurrent = true;
     const uint64_t tiles = (uint64_t(gridDim.x) * 256) / P;
     for (uint64_t i = (uint64_t(blockIdx.x) * 256 + threadIdx.x) / P; i < a.total; i += tiles) {
         uint32_t r;

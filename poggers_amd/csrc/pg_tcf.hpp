@@ -107,10 +107,11 @@ struct Bucket {
         return f;
     }
 
-    // insert / insert_delete: claim the lowest empty (or tombstone) slot.
-    // packed_bucket.cuh:64-180; CAS words grouped_storage_sub_bits.cuh:127-157.
+    // insert / insert_delete: claim the lowest empty (or tombstone) slot
+    // among slots [0, lim).  packed_bucket.cuh:64-180; CAS words
+    // grouped_storage_sub_bits.cuh:127-157.
     __device__ inline bool claim(uint32_t *slots, uint64_t b, uint32_t word, bool wd,
-                                 const Tile<P> &t) const {
+                                 const Tile<P> &t, uint32_t lim = (uint32_t)B) const {
         uint32_t *base = slots + b * (uint64_t)B;
 #pragma unroll
         for (int s = 0; s < S; s++) {
@@ -118,7 +119,9 @@ struct Bucket {
             if (ok(s, t)) {
 #pragma unroll
                 for (int v = 0; v < V; v++)
-                    if (w_empty(w[s][v]) || (wd && w_tomb(w[s][v]))) cand |= 1u << v;
+                    if ((w_empty(w[s][v]) || (wd && w_tomb(w[s][v]))) &&
+                        (uint32_t)(s * ROW) + t.rank * V + (uint32_t)v < lim)
+                        cand |= 1u << v;
             }
             uint64_t lanes = t.bits(cand != 0);
             while (lanes) {
@@ -295,10 +298,11 @@ __device__ int gen_fill(const void *base, uint32_t B, const Tile<P> &t) {
 // Claim the lowest empty slot (with `wd`, the lowest empty-or-tombstone one).
 // packed_bucket.cuh:64-180, 12_bit_bucket.cuh:94-144.
 template <int P, class F = Fmt16>
-__device__ bool gen_claim(void *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t) {
-    for (uint32_t s0 = 0; s0 < B; s0 += P) {
+__device__ bool gen_claim(void *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t,
+                          uint32_t lim = ~0u) {
+    for (uint32_t s0 = 0; s0 < B && s0 < lim; s0 += P) {
         const uint32_t i = s0 + t.rank;
-        const bool ok = i < B;
+        const bool ok = i < B && i < lim;
         const uint32_t w = ok ? s_get<F>(base, i) : 0u;
         bool cand = ok && (s_empty<F>(w) || (wd && s_tomb<F>(w)));
         uint64_t lanes = t.bits(cand);
@@ -398,6 +402,23 @@ __device__ bool gen_remove(void *base, uint32_t B, uint32_t fp, const Tile<P> &t
 // Level operations, generic shapes.
 // ---------------------------------------------------------------------------
 
+// The first pass of the shortcut insert claims a slot of a probe bucket only
+// while the bucket's fill is below 3B/4 (power_of_n_shortcut_buckets.cuh:
+// 223).  Applied in key order, the fill read just before the claim is exact.
+// With many keys in flight the fill is read earlier than the claim, and
+// other keys may have filled the bucket in between.  Slots are claimed
+// lowest index first, so in a bucket without tombstones a claim at slot
+// index i means i slots were filled when it landed.  Concurrent kernels
+// therefore also require slot index < ceil(3B/4) in the first pass, which
+// restores the rule at the moment of the claim.  A bucket whose first-pass
+// claim fails then enters the second pass at fill ceil(3B/4) rather than as
+// full.  Key-order (deterministic) kernels keep the unbounded claim, which
+// is then exact.
+template <int P>
+__device__ inline uint32_t first_pass_limit(const Tile<P> &t, uint32_t B) {
+    return t.concurrent ? (3u * B + 3u) / 4u : B;
+}
+
 // power_of_n_shortcut_buckets.cuh:198-306 (insert) and :309-417
 // (insert_with_delete), runtime B.
 template <int P, class F = Fmt16>
@@ -409,6 +430,7 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
     uint64_t bidx[kMaxShortcutProbes];
     int fill[kMaxShortcutProbes];
     int np = 0, min_fill = B;
+    const uint32_t lim1 = first_pass_limit(t, (uint32_t)B);
 #pragma unroll
     for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
         const uint64_t p = pr.at(j);
@@ -418,8 +440,8 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
         t.acc[ai]++;
         int f = gen_fill<P, F>(bucket_ptr(L, b), L.B, t);
         if (4 * f < 3 * B) {
-            if (gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t)) return true;
-            f = B;
+            if (gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t, lim1)) return true;
+            f = (int)lim1;
         }
         fill[j] = f;
         if (f < min_fill) min_fill = f;
@@ -519,6 +541,7 @@ __device__ bool fast_shortcut_insert(const LevelDev &L, const Probe &pr, uint32_
     uint64_t bidx[kMaxShortcutProbes];
     int fill[kMaxShortcutProbes];
     int np = 0, min_fill = B;
+    const uint32_t lim1 = first_pass_limit(t, (uint32_t)B);
 #pragma unroll
     for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
         uint64_t b;
@@ -535,22 +558,22 @@ __device__ bool fast_shortcut_insert(const LevelDev &L, const Probe &pr, uint32_
         if (j == 0) {
             f = bk0.fill(t);
             if (4 * f < 3 * B) {
-                if (bk0.claim(slots, b, word, wd, t)) return true;
-                f = B;
+                if (bk0.claim(slots, b, word, wd, t, lim1)) return true;
+                f = (int)lim1;
             }
         } else if (j == 1) {
             bk1.load(slots, b, t);
             f = bk1.fill(t);
             if (4 * f < 3 * B) {
-                if (bk1.claim(slots, b, word, wd, t)) return true;
-                f = B;
+                if (bk1.claim(slots, b, word, wd, t, lim1)) return true;
+                f = (int)lim1;
             }
         } else {
             t.acc[0]++;
             f = gen_fill<P>(slots + b * B, B, t);
             if (4 * f < 3 * B) {
-                if (gen_claim<P>(slots + b * B, B, word, wd, t)) return true;
-                f = B;
+                if (gen_claim<P>(slots + b * B, B, word, wd, t, lim1)) return true;
+                f = (int)lim1;
             }
         }
         fill[j] = f;
@@ -731,6 +754,7 @@ constexpr int kBlock = 256;
 template <int P, int B, int I, bool ACC, class F = Fmt16>
 __global__ __launch_bounds__(kBlock) void k_tcf_insert(TableDev T, OpArgs a) {
     Tile<P> t;
+    t.concurrent = true;
     const uint64_t tile_id = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
     const uint64_t tiles = (uint64_t(gridDim.x) * kBlock) / P;
     const bool wd = a.with_delete != 0;
@@ -811,6 +835,7 @@ __device__ inline void upsert_one(const TableDev &T, const OpArgs &a, uint64_t i
 template <int P, int B, bool ACC, class F = Fmt16>
 __global__ __launch_bounds__(kBlock) void k_tcf_upsert(TableDev T, OpArgs a) {
     Tile<P> t;
+    t.concurrent = true;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
     uint64_t fails = 0;
     if (tid < a.n) upsert_one<P, B, F>(T, a, tid, t, fails);

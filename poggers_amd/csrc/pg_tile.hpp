@@ -25,12 +25,16 @@ struct Tile {
     // Algorithmic accounting (tile-uniform): [0] level-0 buckets (or KV
     // stripes) examined, [1] backing-level buckets examined, [2] slots written.
     mutable uint32_t acc[3];
+    // Set by kernels whose keys are in flight together: the shortcut insert
+    // then bounds its first pass by slot index (see fast_shortcut_insert).
+    bool concurrent;
 
     __device__ inline Tile() {
         const uint32_t lane = __lane_id();
         rank = lane & (P - 1);
         base = lane & ~uint32_t(P - 1);
         acc[0] = acc[1] = acc[2] = 0;
+        concurrent = false;
     }
     // Tile-local ballot: bit r set when lane `rank == r` of this tile has pred.
     __device__ inline uint64_t bits(bool pred) const {

@@ -132,14 +132,24 @@ static int roundtrip(const char *name, uint64_t nslots, uint64_t backing, uint64
         std::printf("%s ok (%llu keys)\n", name, (unsigned long long)n);
         return 0;
     }
+    // A filter remove takes the lowest slot holding the key's fingerprint in
+    // its buckets, which may belong to another key with the same fingerprint;
+    // that key's own remove (or query) can then miss.  Fill follows the
+    // removes that succeeded; exact-key tables must remove exactly.
+    const bool exact = Table::rep == PG_REP_KV64 || Table::rep == PG_REP_KEY64;
     t->bulk_remove(keys, n / 2, found);
-    CHECK(t->get_fill() == nok - nok_lo);
+    std::vector<uint8_t> hr(n / 2);
+    CHECK(hipMemcpy(hr.data(), found, n / 2, hipMemcpyDeviceToHost) == hipSuccess);
+    uint64_t removed = 0;
+    for (uint64_t i = 0; i < n / 2; i++) removed += hr[i] != 0;
+    CHECK(t->get_fill() == nok - removed);
+    CHECK(exact ? removed == nok_lo : removed + n / 10000 >= nok_lo);
     t->bulk_query(keys, n, out, found);
     CHECK(hipMemcpy(hf.data(), found, n, hipMemcpyDeviceToHost) == hipSuccess);
     uint64_t live = 0;
     for (uint64_t i = n / 2; i < n; i++) live += hf[i] && hs[i];
-    CHECK(live == nok - nok_lo);
-    if (Table::rep == PG_REP_KV64 || Table::rep == PG_REP_KEY64) {
+    CHECK(exact ? live == nok - nok_lo : live + n / 10000 >= nok - nok_lo);
+    if (exact) {
         for (uint64_t i = 0; i < n / 2; i++) CHECK(hf[i] == 0);
     }
     CHECK(t->get_num_slots() >= nslots);
