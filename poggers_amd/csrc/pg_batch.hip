@@ -54,9 +54,8 @@ __global__ __launch_bounds__(64) void k_batch_reset(uint64_t *counter) {
 
 // One key of a batch: the single-table per-key code on the key's table.
 template <int P, int B, bool QUERY>
-__device__ inline void batch_one(const BatchArgs &a, uint64_t i, uint32_t &tb, const Tile<P> &t, uint64_t &fails) {
-    while (a.off[tb + 1] <= i) tb++;
-    const TableDev &T = a.tables[tb];
+__device__ inline void batch_key(const BatchArgs &a, const TableDev &T, uint64_t i, const Tile<P> &t,
+                                 uint64_t &fails) {
     const uint64_t key = a.keys[i];
     if constexpr (QUERY) {
         const uint32_t fp = fp16_of(key);
@@ -89,6 +88,12 @@ __device__ inline void batch_one(const BatchArgs &a, uint64_t i, uint32_t &tb, c
         if (a.flags && t.leader()) a.flags[i] = (uint8_t)st;
         fails += (st == 0 && t.leader());
     }
+}
+
+template <int P, int B, bool QUERY>
+__device__ inline void batch_one(const BatchArgs &a, uint64_t i, uint32_t &tb, const Tile<P> &t, uint64_t &fails) {
+    while (a.off[tb + 1] <= i) tb++;
+    batch_key<P, B, QUERY>(a, a.tables[tb], i, t, fails);
 }
 
 // I keys of one tile at once (register-bucket shapes, B != 0): every key's
@@ -174,8 +179,17 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(BatchArgs a) {
         const uint64_t end = (base + a.chunk < a.total) ? base + a.chunk : a.total;
         uint32_t tb = tb0;
         if constexpr (I == 1 || B == 0) {
-            for (uint64_t i = base + tile_in_wave; i < end; i += kTilesPerWave)
-                batch_one<P, B, QUERY>(a, i, tb, t, fails);
+            // Most chunks lie inside one table: its descriptor is then read
+            // once per wave into scalar registers instead of per key.
+            const uint32_t tu = __builtin_amdgcn_readfirstlane(tb0);
+            if (off[tu + 1] >= end) {
+                const TableDev &T = a.tables[tu];
+                for (uint64_t i = base + tile_in_wave; i < end; i += kTilesPerWave)
+                    batch_key<P, B, QUERY>(a, T, i, t, fails);
+            } else {
+                for (uint64_t i = base + tile_in_wave; i < end; i += kTilesPerWave)
+                    batch_one<P, B, QUERY>(a, i, tb, t, fails);
+            }
         } else {
             for (uint64_t i0 = base + tile_in_wave; i0 < end; i0 += (uint64_t)kTilesPerWave * I)
                 batch_many<P, B, QUERY, I>(a, i0, end, kTilesPerWave, tb, t, fails);
