@@ -28,6 +28,11 @@ int check_level(uint32_t rep, const pg_level_desc &l, uint32_t index) {
     if (!pow2_tile(l.tile)) return PG_ERR_INVALID;
     if (l.prober > PG_PROBE_SPLIT64 || l.hasher > PG_HASH_MURMUR64A) return PG_ERR_INVALID;
     if (l.max_probes == 0) return PG_ERR_INVALID;
+    // SPLIT64 probes are the two 32-bit halves of one hash: a level of more
+    // than 2^32 buckets could not reach its upper buckets.
+    if (l.prober == PG_PROBE_SPLIT64 && l.bucket_size &&
+        (l.nslots - 1) / (l.scheme == PG_SCHEME_SINGLE_SLOT ? 1u : l.bucket_size) + 1 > (1ull << 32))
+        return PG_ERR_UNSUPPORTED;
     // power_of_two.cuh:32, xor_power_of_two.cuh:32: exactly two probes.
     if ((l.prober == PG_PROBE_P2 || l.prober == PG_PROBE_XOR_P2 || l.prober == PG_PROBE_SPLIT64) &&
         l.max_probes != 2)

@@ -58,6 +58,9 @@ def test_create_rejects_bad_descriptors_without_gpu_work():
         # twelve_bucket::tag() returns nothing: no tag-driven probing for it
         pg.TableDesc(pg.REP_FP12, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16, prober=pg.PROBE_XOR_P2)]),
         pg.TableDesc(5, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16)]),
+        # SPLIT64 probes are 32-bit: no level of more than 2^32 buckets
+        pg.TableDesc(pg.REP_FP16_V16, [pg.Level((1 << 36) + 16, pg.SCHEME_SHORTCUT, 16,
+                                                prober=pg.PROBE_SPLIT64, max_probes=2)]),
     ]
     for d in bad:
         h = ctypes.c_void_p()
