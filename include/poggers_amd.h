@@ -126,7 +126,8 @@ enum pg_scheme {
  *   P2       {h1, h2}                    (power_of_two.cuh:68-91)
  *   XOR_P2   {h1, h1 ^ (tag*0x5bd1e995)} (xor_power_of_two.cuh:64-84)
  *   SPLIT64  {lo32(h), hi32(h)}: one 64-bit hash split into halves
- *            (no reference counterpart)                                  */
+ *            (no reference counterpart); its probes are 32-bit, so levels
+ *            of more than 2^32 buckets are refused (PG_ERR_UNSUPPORTED)  */
 enum pg_prober {
     PG_PROBE_LINEAR = 0,
     PG_PROBE_DOUBLE = 1,
