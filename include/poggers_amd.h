@@ -316,6 +316,15 @@ int pg_batch_bytes(pg_batch b, uint64_t *out);
  * keys of one table from different in-flight batches are not ordered with
  * each other (as concurrent bulk calls on two streams would not be). */
 enum pg_op { PG_OP_INSERT = 0, PG_OP_QUERY = 1, PG_OP_REMOVE = 2 };
+/* Request ordering (pg_server_set_order).
+ *   PG_ORDER_NONE   requests in flight together run concurrently, in one
+ *                   batch or in overlapping batches (the default).
+ *   PG_ORDER_TABLE  on each table, a request that writes (insert, remove)
+ *                   takes effect after every request submitted before it on
+ *                   that table, and before every request submitted after
+ *                   it; queries of a table between two writes still run
+ *                   together.  Batches on other tables still overlap. */
+enum pg_order { PG_ORDER_NONE = 0, PG_ORDER_TABLE = 1 };
 typedef struct pg_server_s *pg_server;
 
 int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_bytes, uint32_t nstreams,
@@ -350,6 +359,10 @@ int pg_server_pending(pg_server s, uint64_t *queued, uint64_t *inflight);
  * still forms its own batch) and max_batch_requests (1..4096) requests per
  * batch. */
 int pg_server_set_batch(pg_server s, uint64_t max_batch_keys, uint32_t max_batch_requests);
+/* Ordering of requests (enum pg_order): a batch closes before a request
+ * that conflicts with one already in it on the same table, and a batch
+ * waits on the device for earlier in-flight batches it conflicts with. */
+int pg_server_set_order(pg_server s, uint32_t order);
 /* One request alone: copies in, the table's bulk kernel, copies out, wait.
  * The per-request baseline; only when nothing is queued or in flight. */
 int pg_server_run_plain(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,

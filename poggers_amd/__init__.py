@@ -40,9 +40,11 @@ EXPORTS = [
     "pg_batch_insert", "pg_batch_query", "pg_batch_capture", "pg_batch_replay", "pg_batch_fill",
     "pg_batch_level_ptr", "pg_batch_bytes",
     "pg_server_create", "pg_server_destroy", "pg_server_submit", "pg_server_progress", "pg_server_pending",
-    "pg_server_run_plain", "pg_server_set_batch", "pg_server_submit_many", "pg_save", "pg_load", "pg_file_info",
+    "pg_server_run_plain", "pg_server_set_batch", "pg_server_set_order", "pg_server_submit_many", "pg_save",
+    "pg_load", "pg_file_info",
 ]
 OP_INSERT, OP_QUERY, OP_REMOVE = 0, 1, 2
+ORDER_NONE, ORDER_TABLE = 0, 1
 
 
 class PgError(RuntimeError):
@@ -134,6 +136,7 @@ def lib():
     L.pg_server_pending.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
     L.pg_server_run_plain.argtypes = [vp, u32, u32, vp, vp, u64, vp, vp]
     L.pg_server_set_batch.argtypes = [vp, u64, u32]
+    L.pg_server_set_order.argtypes = [vp, u32]
     L.pg_server_submit_many.argtypes = [vp, vp, u32, vp, vp]
     L.pg_save.argtypes = [vp, ctypes.c_char_p]
     L.pg_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
@@ -515,6 +518,12 @@ class Server:
 
     def set_batch(self, max_batch_keys: int, max_batch_requests: int = 4096):
         check(lib().pg_server_set_batch(self._h, max_batch_keys, max_batch_requests), "pg_server_set_batch")
+
+    def set_order(self, order: int):
+        """ORDER_TABLE: on each table, writes take effect in submission order
+        relative to every other request of that table; ORDER_NONE (default):
+        requests in flight together run concurrently."""
+        check(lib().pg_server_set_order(self._h, order), "pg_server_set_order")
 
     def pending(self):
         q, f = ctypes.c_uint64(), ctypes.c_uint64()

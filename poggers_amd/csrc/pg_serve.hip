@@ -218,8 +218,15 @@ void classify(ServeReq &r) {
     }
 }
 
+// Per-table access of a batch: bit 0 reads (query), bit 1 writes (insert,
+// remove).  Two accesses to one table conflict unless both only read.
+constexpr uint8_t kRead = 1, kWrite = 2;
+inline uint8_t access_of(uint32_t op) { return op == PG_OP_QUERY ? kRead : kWrite; }
+inline bool conflicts(uint8_t a, uint8_t b) { return (a | b) & kWrite ? (a && b) : false; }
+
 struct ServeBatch {
     std::vector<uint64_t> tickets;
+    std::vector<std::pair<uint32_t, uint8_t>> access;  // tables touched (ordered mode only)
     hipEvent_t done = nullptr;
     uint32_t stream = 0;
     bool finished = false;
@@ -251,6 +258,8 @@ struct pg_server_s {
     uint64_t next_ticket = 1;
     uint32_t max_batch_reqs = kMaxBatchReqs;
     uint64_t max_batch_keys = 1ull << 22;  // bounds a batch's time on the device
+    uint32_t order = PG_ORDER_NONE;
+    std::vector<uint8_t> scratch_access;   // per table, while a batch is formed
 };
 
 extern "C" {
@@ -374,18 +383,48 @@ static int launch_batch(pg_server s, uint32_t si) {
         const uint64_t nidx = (nk >> pg::kIdxShift) + 2;
         return al(8 * (k + 1)) + al(sizeof(pg::ReqDev) * k) + al(4 * (nidx < kMaxIdx + 1 ? nidx : kMaxIdx + 1));
     };
+    const bool ordered = s->order == PG_ORDER_TABLE;
+    std::vector<uint8_t> &acc = s->scratch_access;
+    std::vector<std::pair<uint32_t, uint8_t>> touched;
     while (take < s->queue.size() && take < s->max_batch_reqs) {
-        if (fixed + meta_bytes(take + 1, keys + s->queue[take].n) + need + s->queue[take].need > free_bytes) break;
-        if (take > 0 && keys + s->queue[take].n > s->max_batch_keys) break;
-        need += s->queue[take].need;
-        keys += s->queue[take].n;
+        const ServeReq &q = s->queue[take];
+        if (fixed + meta_bytes(take + 1, keys + q.n) + need + q.need > free_bytes) break;
+        if (take > 0 && keys + q.n > s->max_batch_keys) break;
+        if (ordered) {
+            // requests of one batch run together: a conflicting request on a
+            // table already in the batch waits for the next batch
+            const uint8_t a = access_of(q.op);
+            if (conflicts(acc[q.table], a)) break;
+            if (!acc[q.table]) touched.push_back({q.table, 0});
+            acc[q.table] |= a;
+        }
+        need += q.need;
+        keys += q.n;
         take++;
+    }
+    for (auto &tu : touched) {
+        tu.second = acc[tu.first];
+        acc[tu.first] = 0;
     }
     if (take == 0) return PG_OK;
     const uint64_t meta = meta_bytes(take, keys);
     uint64_t at = 0;
     if (!s->ring.alloc(fixed + meta + need, at)) return PG_OK;
     hipStream_t st = s->streams[si];
+    if (ordered) {
+        // earlier batches still in flight that touch a table of this batch
+        // in a conflicting way finish (on the device) before this one starts
+        for (const ServeBatch &e : s->inflight) {
+            if (e.finished) continue;
+            bool dep = false;
+            for (const auto &x : e.access) {
+                for (const auto &y : touched)
+                    if (x.first == y.first && conflicts(x.second, y.second)) dep = true;
+                if (dep) break;
+            }
+            if (dep && hipStreamWaitEvent(st, e.done, 0) != hipSuccess) return PG_ERR_HIP;
+        }
+    }
     char *base = s->arena + at;
     uint64_t *d_off = (uint64_t *)(base + fixed);
     pg::ReqDev *d_desc = (pg::ReqDev *)(base + fixed + al(8 * (take + 1)));
@@ -395,6 +434,7 @@ static int launch_batch(pg_server s, uint32_t si) {
     pg::ReqDev *hd = (pg::ReqDev *)(hm + al(8 * (take + 1)));
     ServeBatch b;
     b.stream = si;
+    if (ordered) b.access = std::move(touched);
     uint64_t total = 0;
     for (size_t i = 0; i < take; i++) {
         const ServeReq &r = s->queue[i];
@@ -509,6 +549,13 @@ int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndon
         if (rc != PG_OK) return rc;
     }
     if (ndone) *ndone = nd;
+    return PG_OK;
+}
+
+int pg_server_set_order(pg_server s, uint32_t order) {
+    if (!s || order > PG_ORDER_TABLE) return PG_ERR_INVALID;
+    s->order = order;
+    s->scratch_access.assign(s->handles.size(), 0);
     return PG_OK;
 }
 

@@ -2,13 +2,16 @@
 result is the same whether it runs alone (per-request plain launch) or in
 the served mix, with buffers used in place (pinned host memory) or staged
 (pageable memory); admission of staged requests respects the HBM budget;
-requests submitted while others are in flight complete."""
+requests submitted while others are in flight complete; with per-table
+ordering, dependent requests submitted at once give their one-by-one
+results."""
 import numpy as np
 import pytest
 import torch
 
 import oracle
 import poggers_amd as pg
+from tests.gpu_util import dev_u64
 
 pytestmark = pytest.mark.gpu
 
@@ -189,4 +192,48 @@ def test_in_place_requests_take_no_staging():
             break
     assert qf.numpy().all()
     assert (qo.numpy().view(np.uint16) == (keys & np.uint64(0xFFFF)).astype(np.uint16)).mean() > 0.99
+    srv.free()
+
+
+def test_ordered_mode_applies_writes_in_submission_order():
+    """ORDER_TABLE: two tables each get a dependent chain, submitted at once
+    and interleaved: insert, query, remove half, query, re-insert that half
+    with new values, query.  Every result is the one-by-one result; with
+    the default ordering the same burst would land in one batch."""
+    tabs = [pg.Table(pg.kv_desc(1 << 16, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20))
+            for _ in range(2)]
+    srv = pg.Server(tabs, budget_bytes=1 << 24, nstreams=4)
+    srv.set_order(pg.ORDER_TABLE)
+    n = 4000
+    chains = []
+    for t in range(2):
+        k = dev_u64(oracle.gen_keys(70 + t, 0, n))
+        v1, v2 = k ^ 0x1111, k ^ 0x2222
+        half = k[::2].contiguous()
+        outs = [(torch.zeros(n, dtype=torch.int64, device="cuda"), torch.zeros(n, dtype=torch.uint8, device="cuda"))
+                for _ in range(3)]
+        steps = [(pg.OP_INSERT, k, v1, None, None), (pg.OP_QUERY, k, None) + outs[0],
+                 (pg.OP_REMOVE, half, None, None, None), (pg.OP_QUERY, k, None) + outs[1],
+                 (pg.OP_INSERT, half, v2[::2].contiguous(), None, None), (pg.OP_QUERY, k, None) + outs[2]]
+        chains.append((k, v1, v2, outs, steps))
+    torch.cuda.synchronize()
+    tickets = []
+    for i in range(6):
+        for t in range(2):
+            op, keys, vals, vo, fl = chains[t][4][i]
+            tickets.append(srv.submit(t, op, keys, vals, vo, fl))
+    done = []
+    for _ in range(1_000_000):
+        done += srv.progress()
+        if len(done) == len(tickets):
+            break
+    assert sorted(done) == sorted(tickets)
+    torch.cuda.synchronize()
+    even = torch.arange(n, device="cuda") % 2 == 0
+    for k, v1, v2, outs, _ in chains:
+        (o1, f1), (o2, f2), (o3, f3) = outs
+        assert bool(f1.all()) and bool((o1 == v1).all())
+        assert not bool(f2[even].any()) and bool(f2[~even].all()) and bool((o2[~even] == v1[~even]).all())
+        assert bool(f3.all()) and bool((o3[even] == v2[even]).all()) and bool((o3[~even] == v1[~even]).all())
+    assert tabs[0].get_fill() == n and tabs[1].get_fill() == n
     srv.free()
