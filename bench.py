@@ -149,7 +149,8 @@ def run_serve(args):
     memory.  For request sizes log-uniform in [1K, 4M] and in [1K, 64K]:
     saturated throughput of the served loop (all requests queued at once)
     against per-request plain launches (copies, one kernel, wait), and
-    p50/p99 latency under Poisson arrivals at half the served throughput."""
+    p50/p99 latency under Poisson arrivals at half the served throughput;
+    then both again with per-table request ordering."""
     import numpy as np
     import torch
     import poggers_amd as pg
@@ -285,6 +286,19 @@ def run_serve(args):
             "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
             "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
             "p99_ms_requests_le_64K": round(float(np.percentile(lat[small], 99)) * 1e3, 3) if small.any() else None,
+        }
+        # the same requests with per-table ordering (writes in submission order)
+        srv.set_order(pg.ORDER_TABLE)
+        to, _ = saturated(srv, reqs)
+        rate_o = 0.5 * (len(reqs) / to)
+        lat_o = poisson(srv, reqs, rate_o, 11)
+        srv.set_order(pg.ORDER_NONE)
+        results[name]["ordered"] = {
+            "served_mops": round(keys / to / 1e6, 1), "speedup": round(keys / to / 1e6 / plain_mops, 2),
+            "poisson_rate_req_per_s": round(rate_o, 1),
+            "p50_ms": round(float(np.percentile(lat_o, 50)) * 1e3, 3),
+            "p99_ms": round(float(np.percentile(lat_o, 99)) * 1e3, 3),
+            "p99_ms_requests_le_64K": round(float(np.percentile(lat_o[small], 99)) * 1e3, 3) if small.any() else None,
         }
         log(f"serve {name}: {results[name]}")
         del reqs

@@ -260,6 +260,8 @@ struct pg_server_s {
     uint64_t max_batch_keys = 1ull << 22;  // bounds a batch's time on the device
     uint32_t order = PG_ORDER_NONE;
     std::vector<uint8_t> scratch_access;   // per table, while a batch is formed
+    std::vector<uint8_t> scratch_blocked;  // per table, while a batch is formed
+    std::vector<size_t> scratch_pick;      // queue positions of the batch being formed
 };
 
 extern "C" {
@@ -386,25 +388,49 @@ static int launch_batch(pg_server s, uint32_t si) {
     const bool ordered = s->order == PG_ORDER_TABLE;
     std::vector<uint8_t> &acc = s->scratch_access;
     std::vector<std::pair<uint32_t, uint8_t>> touched;
-    while (take < s->queue.size() && take < s->max_batch_reqs) {
-        const ServeReq &q = s->queue[take];
-        if (fixed + meta_bytes(take + 1, keys + q.n) + need + q.need > free_bytes) break;
-        if (take > 0 && keys + q.n > s->max_batch_keys) break;
-        if (ordered) {
-            // requests of one batch run together: a conflicting request on a
-            // table already in the batch waits for the next batch
+    std::vector<size_t> &pick = s->scratch_pick;   // queue positions in this batch
+    pick.clear();
+    if (!ordered) {
+        while (take < s->queue.size() && take < s->max_batch_reqs) {
+            const ServeReq &q = s->queue[take];
+            if (fixed + meta_bytes(take + 1, keys + q.n) + need + q.need > free_bytes) break;
+            if (take > 0 && keys + q.n > s->max_batch_keys) break;
+            need += q.need;
+            keys += q.n;
+            pick.push_back(take++);
+        }
+    } else {
+        // Per-table order only constrains requests of one table: a request
+        // that conflicts with the batch's access to its table (any pair with
+        // a write) blocks that table for the rest of this scan, and later
+        // requests of other tables may still join.  Skipped requests keep
+        // their queue positions, at the front of the next scan.
+        std::vector<uint8_t> &blocked = s->scratch_blocked;
+        std::vector<uint32_t> marked;
+        const size_t window = 4 * (size_t)s->max_batch_reqs;
+        for (size_t i = 0; i < s->queue.size() && i < window && take < s->max_batch_reqs; i++) {
+            const ServeReq &q = s->queue[i];
+            if (blocked[q.table]) continue;
             const uint8_t a = access_of(q.op);
-            if (conflicts(acc[q.table], a)) break;
+            if (conflicts(acc[q.table], a)) {
+                blocked[q.table] = 1;
+                marked.push_back(q.table);
+                continue;
+            }
+            if (fixed + meta_bytes(take + 1, keys + q.n) + need + q.need > free_bytes) break;
+            if (take > 0 && keys + q.n > s->max_batch_keys) break;
             if (!acc[q.table]) touched.push_back({q.table, 0});
             acc[q.table] |= a;
+            need += q.need;
+            keys += q.n;
+            pick.push_back(i);
+            take++;
         }
-        need += q.need;
-        keys += q.n;
-        take++;
-    }
-    for (auto &tu : touched) {
-        tu.second = acc[tu.first];
-        acc[tu.first] = 0;
+        for (uint32_t t : marked) blocked[t] = 0;
+        for (auto &tu : touched) {
+            tu.second = acc[tu.first];
+            acc[tu.first] = 0;
+        }
     }
     if (take == 0) return PG_OK;
     const uint64_t meta = meta_bytes(take, keys);
@@ -437,7 +463,7 @@ static int launch_batch(pg_server s, uint32_t si) {
     if (ordered) b.access = std::move(touched);
     uint64_t total = 0;
     for (size_t i = 0; i < take; i++) {
-        const ServeReq &r = s->queue[i];
+        const ServeReq &r = s->queue[pick[i]];
         pg::ReqDev &d = hd[i];
         if (r.in_place) {
             d = r.dev;
@@ -498,7 +524,7 @@ static int launch_batch(pg_server s, uint32_t si) {
         if (rc != PG_OK) return rc;
     }
     for (size_t i = 0; i < take; i++) {
-        const ServeReq &r = s->queue[i];
+        const ServeReq &r = s->queue[pick[i]];
         if (r.in_place) continue;
         const pg::ReqDev &d = hd[i];
         if (r.h_flags && r.n &&
@@ -510,7 +536,17 @@ static int launch_batch(pg_server s, uint32_t si) {
     }
     if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) return PG_ERR_HIP;
     if (hipEventRecord(b.done, st) != hipSuccess) return PG_ERR_HIP;
-    s->queue.erase(s->queue.begin(), s->queue.begin() + (long)take);
+    if (!ordered) {
+        s->queue.erase(s->queue.begin(), s->queue.begin() + (long)take);
+    } else {  // remove the picked positions (ascending), keep the rest in order
+        std::deque<ServeReq> rest;
+        size_t k = 0;
+        for (size_t i = 0; i < s->queue.size(); i++) {
+            if (k < pick.size() && pick[k] == i) k++;
+            else rest.push_back(s->queue[i]);
+        }
+        s->queue.swap(rest);
+    }
     s->busy[si] = 1;
     s->inflight.push_back(std::move(b));
     return PG_OK;
@@ -556,6 +592,7 @@ int pg_server_set_order(pg_server s, uint32_t order) {
     if (!s || order > PG_ORDER_TABLE) return PG_ERR_INVALID;
     s->order = order;
     s->scratch_access.assign(s->handles.size(), 0);
+    s->scratch_blocked.assign(s->handles.size(), 0);
     return PG_OK;
 }
 
