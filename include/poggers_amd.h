@@ -359,9 +359,11 @@ int pg_server_pending(pg_server s, uint64_t *queued, uint64_t *inflight);
  * still forms its own batch) and max_batch_requests (1..4096) requests per
  * batch. */
 int pg_server_set_batch(pg_server s, uint64_t max_batch_keys, uint32_t max_batch_requests);
-/* Ordering of requests (enum pg_order): a batch closes before a request
- * that conflicts with one already in it on the same table, and a batch
- * waits on the device for earlier in-flight batches it conflicts with. */
+/* Ordering of requests (enum pg_order).  Under PG_ORDER_TABLE a request
+ * that conflicts with one already in the forming batch on the same table
+ * waits for a later batch (other tables' requests may still join), and a
+ * batch waits on the device for earlier in-flight batches it conflicts
+ * with; batches launched before the switch count as conflicting. */
 int pg_server_set_order(pg_server s, uint32_t order);
 /* One request alone: copies in, the table's bulk kernel, copies out, wait.
  * The per-request baseline; only when nothing is queued or in flight. */

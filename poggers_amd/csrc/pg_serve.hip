@@ -227,6 +227,7 @@ inline bool conflicts(uint8_t a, uint8_t b) { return (a | b) & kWrite ? (a && b)
 struct ServeBatch {
     std::vector<uint64_t> tickets;
     std::vector<std::pair<uint32_t, uint8_t>> access;  // tables touched (ordered mode only)
+    bool tracked = false;                              // launched in ordered mode
     hipEvent_t done = nullptr;
     uint32_t stream = 0;
     bool finished = false;
@@ -442,7 +443,7 @@ static int launch_batch(pg_server s, uint32_t si) {
         // in a conflicting way finish (on the device) before this one starts
         for (const ServeBatch &e : s->inflight) {
             if (e.finished) continue;
-            bool dep = false;
+            bool dep = !e.tracked;  // launched before ordering was switched on: wait for it
             for (const auto &x : e.access) {
                 for (const auto &y : touched)
                     if (x.first == y.first && conflicts(x.second, y.second)) dep = true;
@@ -460,7 +461,10 @@ static int launch_batch(pg_server s, uint32_t si) {
     pg::ReqDev *hd = (pg::ReqDev *)(hm + al(8 * (take + 1)));
     ServeBatch b;
     b.stream = si;
-    if (ordered) b.access = std::move(touched);
+    if (ordered) {
+        b.access = std::move(touched);
+        b.tracked = true;
+    }
     uint64_t total = 0;
     for (size_t i = 0; i < take; i++) {
         const ServeReq &r = s->queue[pick[i]];
