@@ -1,6 +1,8 @@
-// Exercises the C++ templated front-end (include/poggers_amd/tables.hpp).
+// Exercises the C++ templated front-end (include/poggers_amd/tables.hpp)
+// and the many-table wrappers (include/poggers_amd/serving.hpp).
 //   test_frontend cpu   host-only checks (sizing errors, descriptors)
-//   test_frontend gpu   bulk ops through the C ABI on device 0
+//   test_frontend gpu   bulk ops, a filter batch and the request server
+//                       through the C ABI on device 0
 // Exit status 0 on success; prints the failing check otherwise.
 #include <hip/hip_runtime.h>
 
@@ -8,6 +10,7 @@
 #include <cstring>
 #include <vector>
 
+#include "poggers_amd/serving.hpp"
 #include "poggers_amd/tables.hpp"
 
 using namespace poggers_amd;
@@ -161,6 +164,113 @@ static int roundtrip(const char *name, uint64_t nslots, uint64_t backing, uint64
     return 0;
 }
 
+// filter_batch: 48 filters of uneven sizes, every stored key found, fills
+// match; then a graph capture replayed twice gives the same answers.
+static int batch_checks() {
+    std::vector<uint64_t> sizes;
+    for (int t = 0; t < 48; t++) sizes.push_back(4096ull + 997ull * t * t);
+    filter_batch<fast_tcf> b(sizes);
+    CHECK(b.size() == 48);
+    std::vector<uint64_t> off(1, 0);
+    for (uint64_t s : sizes) off.push_back(off.back() + s * 8 / 10);
+    const uint64_t total = off.back();
+    uint64_t *keys = nullptr, *d_off = nullptr, *fails = nullptr;
+    uint8_t *status = nullptr, *found = nullptr;
+    CHECK(hipMalloc(&keys, total * 8) == hipSuccess && hipMalloc(&d_off, off.size() * 8) == hipSuccess);
+    CHECK(hipMalloc(&status, total) == hipSuccess && hipMalloc(&found, total) == hipSuccess &&
+          hipMalloc(&fails, 8) == hipSuccess);
+    CHECK(hipMemset(fails, 0, 8) == hipSuccess);
+    CHECK(pg_gen_keys(31, 0, total, keys, nullptr) == PG_OK);
+    CHECK(hipMemcpy(d_off, off.data(), off.size() * 8, hipMemcpyHostToDevice) == hipSuccess);
+    b.insert(keys, d_off, total, nullptr, status, fails);
+    b.query(keys, d_off, total, nullptr, found);
+    std::vector<uint8_t> hs(total), hf(total);
+    uint64_t nfail = ~0ull;
+    CHECK(hipMemcpy(hs.data(), status, total, hipMemcpyDeviceToHost) == hipSuccess);
+    CHECK(hipMemcpy(hf.data(), found, total, hipMemcpyDeviceToHost) == hipSuccess);
+    CHECK(hipMemcpy(&nfail, fails, 8, hipMemcpyDeviceToHost) == hipSuccess);
+    uint64_t stored = 0;
+    for (uint64_t i = 0; i < total; i++) {
+        stored += hs[i] != 0;
+        CHECK(!hs[i] || hf[i]);
+    }
+    CHECK(nfail == total - stored && nfail <= total / 10000);
+    for (uint32_t t = 0; t < 48; t += 7) {
+        uint64_t st = 0;
+        for (uint64_t i = off[t]; i < off[t + 1]; i++) st += hs[i] != 0;
+        CHECK(b.get_fill(t) == st);
+    }
+    b.capture(keys, d_off, total, keys, d_off, total, nullptr, found, nullptr, true);
+    for (int r = 0; r < 2; r++) {
+        CHECK(hipMemset(found, 7, total) == hipSuccess);
+        b.replay();
+        CHECK(hipDeviceSynchronize() == hipSuccess);
+        CHECK(hipMemcpy(hf.data(), found, total, hipMemcpyDeviceToHost) == hipSuccess);
+        uint64_t hits = 0;
+        for (uint64_t i = 0; i < total; i++) hits += hf[i] == 1;
+        CHECK(hits + total / 10000 >= total);
+    }
+    filter_batch<fast_tcf> moved(std::move(b));
+    CHECK(moved.size() == 48 && b.handle() == nullptr);
+    for (void *p : {(void *)keys, (void *)d_off, (void *)status, (void *)found, (void *)fails})
+        CHECK(hipFree(p) == hipSuccess);
+    std::printf("filter_batch ok (%llu keys)\n", (unsigned long long)total);
+    return 0;
+}
+
+// request_server over a filter and a KV table: with per-table ordering, an
+// insert and the query of the same keys submitted together give the
+// query its one-by-one answer.
+static int server_checks() {
+    size_in_num_slots<2> s2(1 << 18, 1 << 12);
+    size_in_num_slots<1> s1(1 << 18);
+    tcf *f = tcf::generate_on_device(&s2, 42);
+    hash_table_64 *kv = hash_table_64::generate_on_device(&s1, 42);
+    {
+        request_server srv = request_server::over(1 << 24, 4, f, kv);
+        srv.set_order(PG_ORDER_TABLE);
+        const uint64_t n = 50000;
+        uint64_t *keys = nullptr, *vals = nullptr, *out = nullptr;
+        uint16_t *fvals = nullptr, *fout = nullptr;
+        uint8_t *ff = nullptr, *kf = nullptr;
+        CHECK(hipMalloc(&keys, n * 8) == hipSuccess && hipMalloc(&vals, n * 8) == hipSuccess &&
+              hipMalloc(&out, n * 8) == hipSuccess && hipMalloc(&fvals, n * 2) == hipSuccess &&
+              hipMalloc(&fout, n * 2) == hipSuccess && hipMalloc(&ff, n) == hipSuccess && hipMalloc(&kf, n) == hipSuccess);
+        CHECK(pg_gen_keys(41, 0, n, keys, nullptr) == PG_OK);
+        std::vector<uint64_t> hk(n), hv(n);
+        std::vector<uint16_t> hfv(n);
+        CHECK(hipMemcpy(hk.data(), keys, n * 8, hipMemcpyDeviceToHost) == hipSuccess);
+        for (uint64_t i = 0; i < n; i++) {
+            hv[i] = hk[i] * 7;
+            hfv[i] = (uint16_t)(hk[i] >> 20);
+        }
+        CHECK(hipMemcpy(vals, hv.data(), n * 8, hipMemcpyHostToDevice) == hipSuccess);
+        CHECK(hipMemcpy(fvals, hfv.data(), n * 2, hipMemcpyHostToDevice) == hipSuccess);
+        std::vector<uint64_t> done;
+        srv.submit(0, PG_OP_INSERT, keys, n, fvals);
+        srv.submit(1, PG_OP_INSERT, keys, n, vals);
+        srv.submit(0, PG_OP_QUERY, keys, n, nullptr, fout, ff);
+        srv.submit(1, PG_OP_QUERY, keys, n, nullptr, out, kf);
+        srv.drain(done);
+        CHECK(done.size() == 4);
+        std::vector<uint8_t> hff(n), hkf(n);
+        std::vector<uint64_t> ho(n);
+        CHECK(hipMemcpy(hff.data(), ff, n, hipMemcpyDeviceToHost) == hipSuccess);
+        CHECK(hipMemcpy(hkf.data(), kf, n, hipMemcpyDeviceToHost) == hipSuccess);
+        CHECK(hipMemcpy(ho.data(), out, n * 8, hipMemcpyDeviceToHost) == hipSuccess);
+        for (uint64_t i = 0; i < n; i++) CHECK(hff[i] == 1 && hkf[i] == 1 && ho[i] == hv[i]);
+        CHECK(f->get_fill() == n && kv->get_fill() == n);
+        srv.run_plain(1, PG_OP_REMOVE, keys, n / 2, nullptr, nullptr, kf);
+        CHECK(kv->get_fill() == n - n / 2);
+        for (void *p : {(void *)keys, (void *)vals, (void *)out, (void *)fvals, (void *)fout, (void *)ff, (void *)kf})
+            CHECK(hipFree(p) == hipSuccess);
+    }
+    tcf::free_on_device(f);
+    hash_table_64::free_on_device(kv);
+    std::printf("request_server ok\n");
+    return 0;
+}
+
 static int gpu_checks() {
     if (roundtrip<tcf>("tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<fast_tcf>("fast_tcf", 1 << 20, 1 << 14, 900000)) return 1;
@@ -168,6 +278,8 @@ static int gpu_checks() {
     if (roundtrip<small_tcf>("small_tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<tcqf_twelve_16>("tcqf_twelve_16", 1 << 20, 0, 700000)) return 1;
     if (roundtrip<cms_free_table>("cms_free_table", 1 << 20, 0, 800000)) return 1;
+    if (batch_checks()) return 1;
+    if (server_checks()) return 1;
     return 0;
 }
 

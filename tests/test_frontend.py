@@ -1,5 +1,7 @@
-"""The C++ templated front-end (include/poggers_amd/tables.hpp) driven by its
-own test program: host-only checks on the CPU, bulk ops on the GPU."""
+"""The C++ templated front-end (include/poggers_amd/tables.hpp) and the
+many-table wrappers (include/poggers_amd/serving.hpp) driven by their own
+test program: host-only checks on the CPU; bulk ops, a filter batch and the
+request server on the GPU."""
 import os
 import subprocess
 
@@ -27,5 +29,5 @@ def test_frontend_bulk_ops_on_gpu():
     r = subprocess.run([_binary(), "gpu"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     for name in ("tcf ok", "fast_tcf ok", "hash_table_64 ok", "small_tcf ok", "tcqf_twelve_16 ok",
-                 "cms_free_table ok"):
+                 "cms_free_table ok", "filter_batch ok", "request_server ok"):
         assert name in r.stdout
