@@ -352,7 +352,8 @@ typedef struct pg_request {
  * many were queued before it. */
 int pg_server_submit_many(pg_server s, const pg_request *reqs, uint32_t count, uint64_t *tickets,
                           uint32_t *nqueued);
-/* Engine step: writes up to `cap` finished tickets to done[]. */
+/* Engine step: writes up to `cap` finished tickets to done[]; finished
+ * tickets that do not fit are written by later calls. */
 int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndone);
 int pg_server_pending(pg_server s, uint64_t *queued, uint64_t *inflight);
 /* Batch formation: at most max_batch_keys keys (a larger single request

@@ -510,10 +510,12 @@ class Server:
               "pg_server_submit_many")
         return list(tickets)
 
-    def progress(self):
-        """One engine step; returns the tickets that finished."""
+    def progress(self, cap: int = 65536):
+        """One engine step; returns up to `cap` tickets that finished (the
+        rest are returned by later steps)."""
         nd = ctypes.c_uint32()
-        check(lib().pg_server_progress(self._h, self._done, 65536, ctypes.byref(nd)), "pg_server_progress")
+        check(lib().pg_server_progress(self._h, self._done, min(cap, 65536), ctypes.byref(nd)),
+              "pg_server_progress")
         return [self._done[i] for i in range(nd.value)]
 
     def set_batch(self, max_batch_keys: int, max_batch_requests: int = 4096):

@@ -231,7 +231,8 @@ struct ServeBatch {
     hipEvent_t done = nullptr;
     uint32_t stream = 0;
     bool finished = false;
-    bool reported = false;
+    size_t nreported = 0;                              // tickets handed to the caller so far
+    bool reported() const { return nreported == tickets.size(); }
 };
 
 uint64_t req_bytes(const ServeReq &r) {
@@ -571,13 +572,13 @@ int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndon
                 return PG_ERR_HIP;
             }
         }
-        if (b.finished && !b.reported && (nd + b.tickets.size() <= cap || nd == 0)) {
-            for (uint64_t tk : b.tickets)
-                if (done && nd < cap) done[nd++] = tk;
-            b.reported = true;
+        // a batch with more tickets than fit is reported over several calls
+        while (b.finished && !b.reported() && (nd < cap || !done)) {
+            if (done) done[nd++] = b.tickets[b.nreported];
+            b.nreported++;
         }
     }
-    while (!s->inflight.empty() && s->inflight.front().finished && s->inflight.front().reported) {
+    while (!s->inflight.empty() && s->inflight.front().finished && s->inflight.front().reported()) {
         hipEventDestroy(s->inflight.front().done);
         s->ring.release_front();
         s->inflight.pop_front();

@@ -237,3 +237,27 @@ def test_ordered_mode_applies_writes_in_submission_order():
         assert bool(f3.all()) and bool((o3[even] == v2[even]).all()) and bool((o3[~even] == v1[~even]).all())
     assert tabs[0].get_fill() == n and tabs[1].get_fill() == n
     srv.free()
+
+
+def test_progress_hands_over_a_large_batch_in_parts():
+    """3000 small requests form one batch; with room for 100 tickets per
+    engine step every ticket is still reported, exactly once."""
+    t = pg.Table(pg.kv_desc(1 << 16, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20))
+    srv = pg.Server([t], budget_bytes=1 << 22, nstreams=1)
+    keys = dev_u64(oracle.gen_keys(75, 0, 3000 * 4))
+    flags = torch.zeros(3000 * 4, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    tickets = [srv.submit(0, pg.OP_INSERT, keys[4 * i:4 * i + 4], None, None, flags[4 * i:4 * i + 4])
+               for i in range(3000)]
+    done, steps = [], 0
+    for _ in range(1_000_000):
+        got = srv.progress(cap=100)
+        assert len(got) <= 100
+        done += got
+        steps += 1
+        if len(done) >= len(tickets):
+            break
+    assert sorted(done) == sorted(tickets) and steps >= 30
+    torch.cuda.synchronize()
+    assert bool((flags == 1).all()) and t.get_fill() == 12000
+    srv.free()
