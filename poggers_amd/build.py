@@ -76,8 +76,14 @@ def build_cpp_tests(verbose: bool = False) -> str:
                        text=True)
     if r.returncode != 0:
         raise RuntimeError(f"ring test build failed:\n{r.stdout}\n{r.stderr}")
+    plan_src = os.path.join(root, "tests", "cpp", "test_plan.cpp")
+    plan_out = os.path.join(root, "tests", "cpp", "test_plan")
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", plan_src, "-o", plan_out], capture_output=True,
+                       text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"planner test build failed:\n{r.stdout}\n{r.stderr}")
     if verbose:
-        print(f"built {out} and {ring_out}")
+        print(f"built {out}, {ring_out} and {plan_out}")
     return out
 
 

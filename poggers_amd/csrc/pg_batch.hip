@@ -300,7 +300,9 @@ namespace {
 
 unsigned persistent_grid() {
     int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
     // 8 blocks of 4 waves per CU: one resident wave per SIMD slot pair;
     // enough waves to keep many random reads in flight.
     return (unsigned)(cus > 0 ? cus : 256) * 8u;
@@ -407,15 +409,20 @@ int pg_batch_create(const pg_desc *proto, uint32_t ntables, const uint64_t *nslo
 
 int pg_batch_destroy(pg_batch b) {
     if (!b) return PG_ERR_INVALID;
-    hipDeviceSynchronize();
-    if (b->graph) hipGraphExecDestroy(b->graph);
-    if (b->graph_def) hipGraphDestroy(b->graph_def);
-    if (b->arena) hipFree(b->arena);
-    if (b->d_tables) hipFree(b->d_tables);
-    if (b->d_counter) hipFree(b->d_counter);
-    if (b->d_fill) hipFree(b->d_fill);
+    // release everything even if a call fails; report the first failure
+    int rc = PG_OK;
+    auto note = [&rc](hipError_t e) {
+        if (e != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
+    };
+    note(hipDeviceSynchronize());
+    if (b->graph) note(hipGraphExecDestroy(b->graph));
+    if (b->graph_def) note(hipGraphDestroy(b->graph_def));
+    if (b->arena) note(hipFree(b->arena));
+    if (b->d_tables) note(hipFree(b->d_tables));
+    if (b->d_counter) note(hipFree(b->d_counter));
+    if (b->d_fill) note(hipFree(b->d_fill));
     delete b;
-    return PG_OK;
+    return rc;
 }
 
 int pg_batch_set_mode(pg_batch b, uint32_t mode) {
@@ -482,24 +489,26 @@ int pg_batch_capture(pg_batch b, const uint64_t *d_ins_keys, const uint64_t *d_i
         if (hipStreamCreate(&s) != hipSuccess) return PG_ERR_HIP;
         own_stream = true;
     }
+    int rc = PG_OK;
     if (b->graph) {
-        hipGraphExecDestroy(b->graph);
+        if (hipGraphExecDestroy(b->graph) != hipSuccess) rc = PG_ERR_HIP;
         b->graph = nullptr;
     }
     if (b->graph_def) {
-        hipGraphDestroy(b->graph_def);
+        if (hipGraphDestroy(b->graph_def) != hipSuccess) rc = PG_ERR_HIP;
         b->graph_def = nullptr;
     }
-    int rc = PG_OK;
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = PG_ERR_HIP;
+    const bool capturing =
+        rc == PG_OK && hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (!capturing) rc = PG_ERR_HIP;
     if (rc == PG_OK && clear_first) rc = pg_batch_clear(b, s);
     if (rc == PG_OK) rc = pg_batch_insert(b, d_ins_keys, d_ins_offsets, ins_total, nullptr, d_status, d_fail_count, s);
     if (rc == PG_OK) rc = pg_batch_query(b, d_q_keys, d_q_offsets, q_total, nullptr, d_found, s);
     hipGraph_t g = nullptr;
-    if (hipStreamEndCapture(s, &g) != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
+    if (capturing && hipStreamEndCapture(s, &g) != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
     if (rc == PG_OK && hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0) != hipSuccess) rc = PG_ERR_HIP;
     b->graph_def = g;
-    if (own_stream) hipStreamDestroy(s);
+    if (own_stream && hipStreamDestroy(s) != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
     return rc;
 }
 

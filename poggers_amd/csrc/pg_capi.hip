@@ -166,12 +166,17 @@ int pg_create(const pg_desc *desc, uint64_t seed, pg_handle *out) {
 
 int pg_destroy(pg_handle h) {
     if (!h) return PG_ERR_INVALID;
-    hipDeviceSynchronize();
+    // release everything even if a call fails; report the first failure
+    int rc = PG_OK;
+    auto note = [&rc](hipError_t e) {
+        if (e != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
+    };
+    note(hipDeviceSynchronize());
     for (uint32_t i = 0; i < h->dev.nlevels; i++)
-        if (h->dev.lv[i].slots) hipFree(h->dev.lv[i].slots);
-    if (h->d_counter) hipFree(h->d_counter);
+        if (h->dev.lv[i].slots) note(hipFree(h->dev.lv[i].slots));
+    if (h->d_counter) note(hipFree(h->d_counter));
     delete h;
-    return PG_OK;
+    return rc;
 }
 
 int pg_clear(pg_handle h, void *stream) {
@@ -422,7 +427,7 @@ int pg_save(pg_handle h, const char *path) {
                 rc = PG_ERR_INVALID;
         }
     }
-    if (buf) hipHostFree(buf);
+    if (buf && hipHostFree(buf) != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
     if (fclose(f) != 0 && rc == PG_OK) rc = PG_ERR_INVALID;
     return rc;
 }
@@ -450,7 +455,7 @@ int pg_load(const char *path, pg_handle *out) {
                 rc = PG_ERR_HIP;
         }
     }
-    if (buf) hipHostFree(buf);
+    if (buf && hipHostFree(buf) != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
     fclose(f);
     if (rc != PG_OK) {
         if (h) pg_destroy(h);

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "pg_kv_ops.hpp"
+#include "pg_plan.hpp"
 #include "pg_ring.hpp"
 #include "pg_tcf.hpp"
 
@@ -218,12 +219,6 @@ void classify(ServeReq &r) {
     }
 }
 
-// Per-table access of a batch: bit 0 reads (query), bit 1 writes (insert,
-// remove).  Two accesses to one table conflict unless both only read.
-constexpr uint8_t kRead = 1, kWrite = 2;
-inline uint8_t access_of(uint32_t op) { return op == PG_OP_QUERY ? kRead : kWrite; }
-inline bool conflicts(uint8_t a, uint8_t b) { return (a | b) & kWrite ? (a && b) : false; }
-
 struct ServeBatch {
     std::vector<uint64_t> tickets;
     std::vector<std::pair<uint32_t, uint8_t>> access;  // tables touched (ordered mode only)
@@ -263,7 +258,7 @@ struct pg_server_s {
     uint32_t order = PG_ORDER_NONE;
     std::vector<uint8_t> scratch_access;   // per table, while a batch is formed
     std::vector<uint8_t> scratch_blocked;  // per table, while a batch is formed
-    std::vector<size_t> scratch_pick;      // queue positions of the batch being formed
+    pg::Plan plan;                         // the batch being formed
 };
 
 extern "C" {
@@ -321,15 +316,20 @@ int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_
 
 int pg_server_destroy(pg_server s) {
     if (!s) return PG_ERR_INVALID;
-    for (hipStream_t st : s->streams) hipStreamSynchronize(st);
+    // release everything even if a call fails; report the first failure
+    int rc = PG_OK;
+    auto note = [&rc](hipError_t e) {
+        if (e != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
+    };
+    for (hipStream_t st : s->streams) note(hipStreamSynchronize(st));
     for (auto &b : s->inflight)
-        if (b.done) hipEventDestroy(b.done);
-    for (hipStream_t st : s->streams) hipStreamDestroy(st);
-    for (auto *m : s->h_meta) hipHostFree(m);
-    if (s->arena) hipFree(s->arena);
-    if (s->d_tables) hipFree(s->d_tables);
+        if (b.done) note(hipEventDestroy(b.done));
+    for (hipStream_t st : s->streams) note(hipStreamDestroy(st));
+    for (auto *m : s->h_meta) note(hipHostFree(m));
+    if (s->arena) note(hipFree(s->arena));
+    if (s->d_tables) note(hipFree(s->d_tables));
     delete s;
-    return PG_OK;
+    return rc;
 }
 
 int pg_server_submit(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,
@@ -379,8 +379,6 @@ int pg_server_submit_many(pg_server s, const pg_request *reqs, uint32_t count, u
 // memory and copied with one call; staged requests add their own copies.
 static int launch_batch(pg_server s, uint32_t si) {
     const uint64_t free_bytes = s->ring.largest_free();
-    uint64_t need = 0, keys = 0;
-    size_t take = 0;
     const uint64_t fixed = 0;
     // offsets, descriptors and the coarse index of k requests over nk keys
     auto meta_bytes = [](size_t k, uint64_t nk) {
@@ -388,52 +386,13 @@ static int launch_batch(pg_server s, uint32_t si) {
         return al(8 * (k + 1)) + al(sizeof(pg::ReqDev) * k) + al(4 * (nidx < kMaxIdx + 1 ? nidx : kMaxIdx + 1));
     };
     const bool ordered = s->order == PG_ORDER_TABLE;
-    std::vector<uint8_t> &acc = s->scratch_access;
-    std::vector<std::pair<uint32_t, uint8_t>> touched;
-    std::vector<size_t> &pick = s->scratch_pick;   // queue positions in this batch
-    pick.clear();
-    if (!ordered) {
-        while (take < s->queue.size() && take < s->max_batch_reqs) {
-            const ServeReq &q = s->queue[take];
-            if (fixed + meta_bytes(take + 1, keys + q.n) + need + q.need > free_bytes) break;
-            if (take > 0 && keys + q.n > s->max_batch_keys) break;
-            need += q.need;
-            keys += q.n;
-            pick.push_back(take++);
-        }
-    } else {
-        // Per-table order only constrains requests of one table: a request
-        // that conflicts with the batch's access to its table (any pair with
-        // a write) blocks that table for the rest of this scan, and later
-        // requests of other tables may still join.  Skipped requests keep
-        // their queue positions, at the front of the next scan.
-        std::vector<uint8_t> &blocked = s->scratch_blocked;
-        std::vector<uint32_t> marked;
-        const size_t window = 4 * (size_t)s->max_batch_reqs;
-        for (size_t i = 0; i < s->queue.size() && i < window && take < s->max_batch_reqs; i++) {
-            const ServeReq &q = s->queue[i];
-            if (blocked[q.table]) continue;
-            const uint8_t a = access_of(q.op);
-            if (conflicts(acc[q.table], a)) {
-                blocked[q.table] = 1;
-                marked.push_back(q.table);
-                continue;
-            }
-            if (fixed + meta_bytes(take + 1, keys + q.n) + need + q.need > free_bytes) break;
-            if (take > 0 && keys + q.n > s->max_batch_keys) break;
-            if (!acc[q.table]) touched.push_back({q.table, 0});
-            acc[q.table] |= a;
-            need += q.need;
-            keys += q.n;
-            pick.push_back(i);
-            take++;
-        }
-        for (uint32_t t : marked) blocked[t] = 0;
-        for (auto &tu : touched) {
-            tu.second = acc[tu.first];
-            acc[tu.first] = 0;
-        }
-    }
+    pg::Plan &plan = s->plan;
+    pg::plan_batch(s->queue, pg::PlanLimits{free_bytes - fixed, s->max_batch_keys, s->max_batch_reqs, ordered},
+                   meta_bytes, s->scratch_access, s->scratch_blocked, plan);
+    const std::vector<size_t> &pick = plan.pick;
+    std::vector<std::pair<uint32_t, uint8_t>> &touched = plan.access;
+    const size_t take = pick.size();
+    const uint64_t keys = plan.keys, need = plan.need;
     if (take == 0) return PG_OK;
     const uint64_t meta = meta_bytes(take, keys);
     uint64_t at = 0;
@@ -447,7 +406,7 @@ static int launch_batch(pg_server s, uint32_t si) {
             bool dep = !e.tracked;  // launched before ordering was switched on: wait for it
             for (const auto &x : e.access) {
                 for (const auto &y : touched)
-                    if (x.first == y.first && conflicts(x.second, y.second)) dep = true;
+                    if (x.first == y.first && pg::conflicts(x.second, y.second)) dep = true;
                 if (dep) break;
             }
             if (dep && hipStreamWaitEvent(st, e.done, 0) != hipSuccess) return PG_ERR_HIP;
@@ -579,7 +538,7 @@ int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndon
         }
     }
     while (!s->inflight.empty() && s->inflight.front().finished && s->inflight.front().reported()) {
-        hipEventDestroy(s->inflight.front().done);
+        if (hipEventDestroy(s->inflight.front().done) != hipSuccess) return PG_ERR_HIP;
         s->ring.release_front();
         s->inflight.pop_front();
     }
