@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define PG_MAX_LEVELS 2
+#define PG_MAX_LEVELS 4  /* a primary level and up to three backing levels */
 
 /* status codes */
 enum pg_status {

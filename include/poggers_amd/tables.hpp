@@ -175,7 +175,7 @@ class bucketed_table {
                   "insert scheme does not apply to this representation");
     static_assert(!(Insert_Scheme::scheme == PG_SCHEME_SINGLE_SLOT) || Partition_Size == Bucket_Size,
                   "single_slot_insert needs Partition_Size == Bucket_Size (single_slot_insert.cuh:43)");
-    static_assert(levels <= PG_MAX_LEVELS, "at most one backing level");
+    static_assert(levels <= PG_MAX_LEVELS, "at most PG_MAX_LEVELS levels (a primary and three backing levels)");
 
     // Fill the descriptor levels from `first`, consuming the sizing.
     template <typename Sizing>

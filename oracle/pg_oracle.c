@@ -40,6 +40,7 @@
 #include <unistd.h>
 
 #define OR_END (~(uint64_t)0)
+#define OR_MAX_LEVELS 4   /* a primary level and up to three backing levels */
 
 /* ---- enums (values mirror include/poggers_amd.h so tests can share one
  *      descriptor layout; the oracle does not include that header) ---- */
@@ -297,11 +298,11 @@ typedef struct {
 typedef struct {
     uint32_t rep;
     uint32_t nlevels;
-    or_level lv[2];
+    or_level lv[OR_MAX_LEVELS];
     /* algorithmic accounting, reset by or_stats_reset */
     uint64_t sectors_read;     /* distinct 64 B sectors examined */
     uint64_t sectors_written;  /* 64 B sectors dirtied */
-    uint64_t buckets_read[2];  /* distinct bucket examinations per level */
+    uint64_t buckets_read[2];  /* bucket examinations: [0] level 0, [1] all backing levels */
 } or_table;
 
 static inline int is_kv_rep(uint32_t rep) { return rep == OR_REP_KV64 || rep == OR_REP_KEY64; }
@@ -434,7 +435,7 @@ static inline uint32_t tomb_word(const or_table *t) { return kmask_of(t); } /* j
 static void account_bucket(or_table *t, int level, uint64_t b)
 {
     (void)b;
-    t->buckets_read[level] += 1;
+    t->buckets_read[level ? 1 : 0] += 1;
     t->sectors_read += (t->lv[level].bucket_bytes + 63) / 64;
 }
 
@@ -549,7 +550,7 @@ static int shortcut_insert(or_table *t, or_level *L, uint64_t key, uint32_t word
          p != OR_END && np < MAXP; p = probe_next(&it)) {
         uint64_t b = p % L->nb;
         buckets[np] = b;
-        account_bucket(t, 0, b);
+        account_bucket(t, (int)(L - t->lv), b);
         int f = pb_fill(t, L, b);
         if (4 * f < 3 * B) {
             int ok = with_delete ? pb_insert_delete(t, L, b, word) : pb_insert(t, L, b, word);
@@ -586,7 +587,7 @@ static int shortcut_query(or_table *t, or_level *L, uint64_t key, uint32_t *val)
                                   L->seed, key, fp);
          p != OR_END; p = probe_next(&it)) {
         uint64_t b = p % L->nb;
-        account_bucket(t, 0, b);
+        account_bucket(t, (int)(L - t->lv), b);
         if (pb_query(t, L, b, fp, val)) return 1;
     }
     return 0;
@@ -601,7 +602,7 @@ static int shortcut_remove(or_table *t, or_level *L, uint64_t key)
                                   L->seed, key, fp);
          p != OR_END; p = probe_next(&it)) {
         uint64_t b = p % L->nb;
-        account_bucket(t, 0, b);
+        account_bucket(t, (int)(L - t->lv), b);
         if (pb_remove(t, L, b, fp)) return 1;
     }
     return 0;
@@ -685,7 +686,7 @@ static inline uint64_t kv_skey(const or_table *t, uint64_t key)
  * previous probe already covered (linear probing walks adjacent buckets). */
 static void account_kv_range(or_table *t, int lvl, uint64_t s0, uint64_t s1, uint64_t *last_sector)
 {
-    t->buckets_read[lvl] += 1;
+    t->buckets_read[lvl ? 1 : 0] += 1;
     uint64_t sb = slot_bytes(t);
     uint64_t a = (s0 * sb) / 64, z = (s1 * sb - 1) / 64;
     for (uint64_t s = a; s <= z; s++) {
@@ -795,7 +796,7 @@ static int kv_remove(or_table *t, or_level *L, int lvl, uint64_t key)
  * slot scheme allocates nslots slots (single_slot_insert.cuh:89-96). */
 or_table *or_create(uint32_t rep, uint32_t nlevels, const or_level_desc *lv, uint64_t seed)
 {
-    if (nlevels < 1 || nlevels > 2 || rep > OR_REP_KEY64) return NULL;
+    if (nlevels < 1 || nlevels > OR_MAX_LEVELS || rep > OR_REP_KEY64) return NULL;
     for (uint32_t l = 0; l < nlevels; l++) {
         /* GROUPED levels hold `group` packed buckets per 128-byte line
          * (grouped_power_buckets.cuh:58: sizeof(bucket) * K <= 128). */
@@ -1264,7 +1265,7 @@ uint32_t or_hw_threads(void)
 static int mt_shape_ok(const or_table *t)
 {
     if (t->rep != OR_REP_FP16_V16 || t->lv[0].d.scheme != OR_SCHEME_POWER_OF_N_SHORTCUT) return 0;
-    return t->nlevels == 1 || t->lv[1].d.scheme == OR_SCHEME_LINEAR_BUCKETS;
+    return t->nlevels == 1 || (t->nlevels == 2 && t->lv[1].d.scheme == OR_SCHEME_LINEAR_BUCKETS);
 }
 
 static int mt_run(const or_table *t, const uint64_t *keys, const uint16_t *vals, uint64_t n, int query,

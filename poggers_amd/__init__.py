@@ -26,7 +26,7 @@ SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS, SCHEM
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
 CONCURRENT, DETERMINISTIC = 0, 1
-MAX_LEVELS = 2
+MAX_LEVELS = 4
 
 # Every symbol the header declares (checked by the CPU test suite).
 EXPORTS = [

@@ -370,7 +370,7 @@ int pg_level_ptr(pg_handle h, uint32_t level, void **d_ptr, uint64_t *bytes) {
 
 // ---- raw bucket-array files ----
 static const char kMagic[8] = {'P', 'G', 'A', 'M', 'D', 'T', 'B', '1'};
-static const uint32_t kFileVersion = 2;  // 2: pg_level_desc carries group + reserved
+static const uint32_t kFileVersion = 3;  // 2: pg_level_desc carries group + reserved; 3: four levels
 
 struct FileHeader {
     char magic[8];

@@ -271,6 +271,46 @@ static int server_checks() {
     return 0;
 }
 
+// A recursive composition two backing levels deep: the sizing has three
+// entries, and keys that the first two levels refuse land in the third.
+using tcf3_l2 = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16, 16>, 4, 8, linear_insert_bucket_scheme,
+                               64, linearProber, murmurHasher>;
+using tcf3_l1 = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16, 16>, 4, 8, linear_insert_bucket_scheme,
+                               16, linearProber, murmurHasher, true, tcf3_l2>;
+using tcf3 = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16, 16>, 4, 16,
+                            power_of_n_insert_shortcut_bucket_scheme, 2, XORPowerOfTwoHasher, murmurHasher, true,
+                            tcf3_l1>;
+static_assert(tcf3::levels == 3, "three levels");
+
+static int three_level_checks() {
+    size_in_num_slots<3> sz(1 << 12, 128, 512);
+    tcf3 *t = tcf3::generate_on_device(&sz, 42);
+    const uint64_t n = 4400;  // more than the first two levels hold
+    uint64_t *keys = nullptr;
+    uint8_t *status = nullptr, *found = nullptr;
+    CHECK(hipMalloc(&keys, n * 8) == hipSuccess && hipMalloc(&status, n) == hipSuccess &&
+          hipMalloc(&found, n) == hipSuccess);
+    CHECK(pg_gen_keys(51, 0, n, keys, nullptr) == PG_OK);
+    t->set_mode(PG_DETERMINISTIC);
+    t->bulk_insert(keys, nullptr, n, status);
+    t->bulk_query(keys, n, nullptr, found);
+    std::vector<uint8_t> hs(n), hf(n);
+    CHECK(hipMemcpy(hs.data(), status, n, hipMemcpyDeviceToHost) == hipSuccess);
+    CHECK(hipMemcpy(hf.data(), found, n, hipMemcpyDeviceToHost) == hipSuccess);
+    uint64_t per[4] = {0, 0, 0, 0};
+    for (uint64_t i = 0; i < n; i++) {
+        CHECK(hs[i] <= 3);
+        per[hs[i]]++;
+        CHECK(!hs[i] || hf[i]);
+    }
+    CHECK(per[3] > 0 && t->get_fill() == n - per[0]);
+    CHECK(t->get_num_slots() == (1 << 12) + 128 + 512);
+    for (void *p : {(void *)keys, (void *)status, (void *)found}) CHECK(hipFree(p) == hipSuccess);
+    tcf3::free_on_device(t);
+    std::printf("tcf3 ok (%llu keys in the third level)\n", (unsigned long long)per[3]);
+    return 0;
+}
+
 static int gpu_checks() {
     if (roundtrip<tcf>("tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<fast_tcf>("fast_tcf", 1 << 20, 1 << 14, 900000)) return 1;
@@ -278,6 +318,7 @@ static int gpu_checks() {
     if (roundtrip<small_tcf>("small_tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<tcqf_twelve_16>("tcqf_twelve_16", 1 << 20, 0, 700000)) return 1;
     if (roundtrip<cms_free_table>("cms_free_table", 1 << 20, 0, 800000)) return 1;
+    if (three_level_checks()) return 1;
     if (batch_checks()) return 1;
     if (server_checks()) return 1;
     return 0;

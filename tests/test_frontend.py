@@ -29,5 +29,5 @@ def test_frontend_bulk_ops_on_gpu():
     r = subprocess.run([_binary(), "gpu"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     for name in ("tcf ok", "fast_tcf ok", "hash_table_64 ok", "small_tcf ok", "tcqf_twelve_16 ok",
-                 "cms_free_table ok", "filter_batch ok", "request_server ok"):
+                 "cms_free_table ok", "tcf3 ok", "filter_batch ok", "request_server ok"):
         assert name in r.stdout

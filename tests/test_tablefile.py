@@ -10,14 +10,14 @@ import poggers_amd as pg
 
 class FileHeader(ctypes.Structure):
     _fields_ = [("magic", ctypes.c_char * 8), ("version", ctypes.c_uint32), ("header_bytes", ctypes.c_uint32),
-                ("desc", pg.Desc), ("seed", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64 * 2)]
+                ("desc", pg.Desc), ("seed", ctypes.c_uint64), ("level_bytes", ctypes.c_uint64 * pg.MAX_LEVELS)]
 
 
 def test_file_header_is_read_on_the_host(tmp_path):
     desc = pg.tcf_desc(1 << 20, 1 << 10, tile=2)
     h = FileHeader()
     h.magic = b"PGAMDTB1"
-    h.version = 2
+    h.version = 3
     h.header_bytes = ctypes.sizeof(FileHeader)
     h.desc = desc.to_c()
     h.seed = 4242
