@@ -84,8 +84,18 @@ enum pg_status {
  *             0/1 (and ~0/~0-1) share a slot; probes hash the key as
  *             given.  Schemes BUCKET and SINGLE_SLOT, as KV64; queries
  *             report value 0 (uint16 buffers, the CMS table's Val).
+ *   KV16      dynamic_container<key_val_pair<uint64_t, uint16_t>, uint16_t>
+ *             (representations/dynamic_container.cuh:35-120 over
+ *             key_val_pair.cuh:63-171): one uint32 word per slot =
+ *             val16 << 16 | key16, key16 = the key's low 16 bits with
+ *             0 -> 1 and 0xFFFF -> 0xFFFE; empty key 0, tombstone key 0xFFFF.
+ *             A remove turns the key into the tombstone and keeps the value
+ *             bits (key_val_pair::atomic_reset).  The slot of the
+ *             reference's flat TCF (data_structs/tcf.cuh:29-30).  Schemes
+ *             BUCKET, SINGLE_SLOT, POWER_OF_N and POWER_OF_N_SHORTCUT_SLOTS,
+ *             as KV64 and KEY64.
  *   Value buffers hold the reference's Val type: uint64 (KV64), uint16
- *   (FP16_V16, FP12, KEY64), uint8 (FP8_V8 and FP10_V6, which keep the low
+ *   (FP16_V16, FP12, KEY64, KV16), uint8 (FP8_V8 and FP10_V6, which keep the low
  *   8 / 6 bits).  The 8/8, 10/6 and 12-bit formats and grouped levels run the
  *   generic (runtime bucket size) kernels; batches and the server take KV64
  *   and FP16_V16 tables only.                                             */
@@ -95,7 +105,8 @@ enum pg_rep {
     PG_REP_FP8_V8 = 2,
     PG_REP_FP10_V6 = 3,
     PG_REP_FP12 = 4,
-    PG_REP_KEY64 = 5
+    PG_REP_KEY64 = 5,
+    PG_REP_KV16 = 6
 };
 
 /* Insert scheme of a level.
@@ -110,13 +121,25 @@ enum pg_rep {
  *                 (insert_schemes/grouped_power_buckets.cuh:46-388): the
  *                 shortcut scheme over buckets packed `group` = K to a
  *                 128-byte line (bucket b at byte 128*(b/K) + size*(b%K)),
- *                 packed filter slots only                               */
+ *                 packed filter slots only
+ *   POWER_OF_N    insert_schemes/power_of_n.cuh:38-379: each key goes to
+ *                 the least-filled of its max_probes (at most 4) probe
+ *                 buckets, ties in probe order; a bucket's fill counts every
+ *                 non-empty slot, tombstones included; inserts claim empty
+ *                 slots only; queries and removes check every probe bucket.
+ *                 Key-value slots (KV64, KEY64, KV16) only.
+ *   POWER_OF_N_SHORTCUT_SLOTS  insert_schemes/power_of_n_shortcut.cuh:
+ *                 38-449: POWER_OF_N with the 75 % shortcut (a probe bucket
+ *                 whose fill is below 3B/4 takes the key at once), over
+ *                 key-value slots (the reference's flat TCF primary)       */
 enum pg_scheme {
     PG_SCHEME_BUCKET = 0,
     PG_SCHEME_SINGLE_SLOT = 1,
     PG_SCHEME_POWER_OF_N_SHORTCUT = 2,
     PG_SCHEME_LINEAR_BUCKETS = 3,
-    PG_SCHEME_GROUPED_SHORTCUT = 4
+    PG_SCHEME_GROUPED_SHORTCUT = 4,
+    PG_SCHEME_POWER_OF_N = 5,
+    PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS = 6
 };
 
 /* Probing scheme (the probing_schemes directory).  Probe i of a key, before the

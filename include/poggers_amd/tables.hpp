@@ -52,6 +52,20 @@ struct key_container {
     static constexpr uint32_t rep = PG_REP_KEY64;
 };
 
+// dynamic_container<Rep, SmallKey> (representations/dynamic_container.cuh:
+// 35-120): the slot stores get_smallKey(key), the key's low bits with 0 and
+// all-ones remapped.  Over key_val_pair with a uint16_t small key it is the
+// flat TCF's 4-byte slot (data_structs/tcf.cuh:29-30): one word
+// val16 << 16 | key16; over key_container with uint64_t it is the 8-byte
+// key-only slot above.
+template <typename Rep, typename SmallKey>
+struct dynamic_container {
+    static_assert((std::is_same<Rep, key_val_pair>::value && std::is_same<SmallKey, uint16_t>::value) ||
+                      (std::is_same<Rep, key_container>::value && std::is_same<SmallKey, uint64_t>::value),
+                  "dynamic_container<key_val_pair, uint16_t> or dynamic_container<key_container, uint64_t>");
+    static constexpr uint32_t rep = std::is_same<Rep, key_val_pair>::value ? PG_REP_KV16 : PG_REP_KEY64;
+};
+
 // ---- insert schemes (insert_schemes/*.cuh) ----
 struct bucket_insert { static constexpr uint32_t scheme = PG_SCHEME_BUCKET; };
 struct single_slot_insert { static constexpr uint32_t scheme = PG_SCHEME_SINGLE_SLOT; };
@@ -59,6 +73,12 @@ struct power_of_n_insert_shortcut_bucket_scheme {
     static constexpr uint32_t scheme = PG_SCHEME_POWER_OF_N_SHORTCUT;
 };
 struct linear_insert_bucket_scheme { static constexpr uint32_t scheme = PG_SCHEME_LINEAR_BUCKETS; };
+// The flat power-of-n schemes over key-value slots (power_of_n.cuh:38-379,
+// power_of_n_shortcut.cuh:38-449).
+struct power_of_n_insert_scheme { static constexpr uint32_t scheme = PG_SCHEME_POWER_OF_N; };
+struct power_of_n_insert_shortcut_scheme {
+    static constexpr uint32_t scheme = PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+};
 // blocked_bucket_insert<K> (insert_schemes/grouped_power_buckets.cuh:388-393):
 // the shortcut scheme over K buckets per 128-byte line.
 template <std::size_t Buckets_per_cache_line>
@@ -167,8 +187,14 @@ class bucketed_table {
                     std::is_same<Probing_Scheme, split64Hasher>::value) ||
                       Max_Probes == 2,
                   "two-choice probing schemes take exactly two probes (power_of_two.cuh:32)");
-    static_assert((rep == PG_REP_KV64 || rep == PG_REP_KEY64)
-                      ? (Insert_Scheme::scheme == PG_SCHEME_BUCKET || Insert_Scheme::scheme == PG_SCHEME_SINGLE_SLOT)
+    static_assert(rep != PG_REP_KV16 || std::is_same<Val, uint16_t>::value,
+                  "16/16 key_val_pair values are uint16_t");
+    static constexpr bool flat_pow = Insert_Scheme::scheme == PG_SCHEME_POWER_OF_N ||
+                                     Insert_Scheme::scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+    static_assert(!flat_pow || Max_Probes <= 4, "power-of-n schemes take at most 4 probes");
+    static_assert((rep == PG_REP_KV64 || rep == PG_REP_KEY64 || rep == PG_REP_KV16)
+                      ? (Insert_Scheme::scheme == PG_SCHEME_BUCKET || Insert_Scheme::scheme == PG_SCHEME_SINGLE_SLOT ||
+                         flat_pow)
                       : (Insert_Scheme::scheme == PG_SCHEME_POWER_OF_N_SHORTCUT ||
                          Insert_Scheme::scheme == PG_SCHEME_LINEAR_BUCKETS ||
                          Insert_Scheme::scheme == PG_SCHEME_GROUPED_SHORTCUT),
@@ -275,15 +301,24 @@ using static_table = bucketed_table<Key, Val, Internal_Rep, Partition_Size, Buck
 
 namespace data_structs {
 
-// The TCF as the reference's delete tests compose it
+// The reference's pre-baked TCF (data_structs/tcf.cuh:29-30): flat 16/16
+// key_val_pair slots, P=4, B=16, power_of_n_insert_shortcut_scheme with two
+// double-hashing probes, backed by tiny_static_table_4 (B=4,
+// bucket_insert, 20 probes).  Size with size_in_num_slots<2>.
+using tiny_static_table_4 = static_table<uint64_t, uint16_t, dynamic_container<key_val_pair, uint16_t>, 4, 4,
+                                         bucket_insert, 20, doubleHasher, murmurHasher>;
+using tcf = static_table<uint64_t, uint16_t, dynamic_container<key_val_pair, uint16_t>, 4, 16,
+                         power_of_n_insert_shortcut_scheme, 2, doubleHasher, murmurHasher, true, tiny_static_table_4>;
+
+// The TCF as the reference's delete tests compose it, del_TCF
 // (tests/src/delete_tests.cu:138-152): 16/16 grouped slots, P=4, B=16,
 // shortcut two-choice with XOR power-of-two probing, backed by a linear
 // bucket table (B=8, 4000 probes).  Size with size_in_num_slots<2>.
-using tcf_backing = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16, 16>, 4, 8,
-                                   linear_insert_bucket_scheme, 4000, linearProber, murmurHasher>;
-using tcf = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16, 16>, 4, 16,
-                           power_of_n_insert_shortcut_bucket_scheme, 2, XORPowerOfTwoHasher, murmurHasher, true,
-                           tcf_backing>;
+using del_tcf_backing = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16, 16>, 4, 8,
+                                       linear_insert_bucket_scheme, 4000, linearProber, murmurHasher>;
+using del_tcf = bucketed_table<uint64_t, uint16_t, bit_grouped_container<16, 16>, 4, 16,
+                               power_of_n_insert_shortcut_bucket_scheme, 2, XORPowerOfTwoHasher, murmurHasher, true,
+                               del_tcf_backing>;
 
 // The reference's 8/8 and 10/6 compositions: del_TCF_small
 // (tests/src/delete_tests.cu:165-179) and the TCF of

@@ -19,8 +19,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
-REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12, REP_KEY64 = 0, 1, 2, 3, 4, 5
+REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12, REP_KEY64, REP_KV16 = 0, 1, 2, 3, 4, 5, 6
 SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS, SCHEME_GROUPED = 0, 1, 2, 3, 4
+SCHEME_POWER_OF_N, SCHEME_POWER_OF_N_SHORTCUT_SLOTS = 5, 6
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
 
@@ -286,7 +287,7 @@ class Table:
 
     def level_array(self, lvl: int) -> np.ndarray:
         dt = {REP_KV64: np.uint64, REP_FP16_V16: np.uint32, REP_FP8_V8: np.uint16,
-              REP_FP10_V6: np.uint16, REP_FP12: np.uint16, REP_KEY64: np.uint64}[self.rep]
+              REP_FP10_V6: np.uint16, REP_FP12: np.uint16, REP_KEY64: np.uint64, REP_KV16: np.uint32}[self.rep]
         return np.frombuffer(self.level_bytes(lvl), dtype=dt).copy()
 
     def num_buckets(self, lvl: int) -> int:

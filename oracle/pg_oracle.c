@@ -45,10 +45,10 @@
 /* ---- enums (values mirror include/poggers_amd.h so tests can share one
  *      descriptor layout; the oracle does not include that header) ---- */
 enum { OR_REP_KV64 = 0, OR_REP_FP16_V16 = 1, OR_REP_FP8_V8 = 2, OR_REP_FP10_V6 = 3, OR_REP_FP12 = 4,
-       OR_REP_KEY64 = 5 };
+       OR_REP_KEY64 = 5, OR_REP_KV16 = 6 };
 enum { OR_SCHEME_BUCKET = 0, OR_SCHEME_SINGLE_SLOT = 1,
        OR_SCHEME_POWER_OF_N_SHORTCUT = 2, OR_SCHEME_LINEAR_BUCKETS = 3,
-       OR_SCHEME_GROUPED = 4 };
+       OR_SCHEME_GROUPED = 4, OR_SCHEME_POWER_OF_N = 5, OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS = 6 };
 enum { OR_PROBE_LINEAR = 0, OR_PROBE_DOUBLE = 1, OR_PROBE_P2 = 2,
        OR_PROBE_XOR_P2 = 3, OR_PROBE_SPLIT64 = 4 };
 enum { OR_HASH_FMIX64 = 0, OR_HASH_MURMUR64A = 1 };
@@ -305,7 +305,10 @@ typedef struct {
     uint64_t buckets_read[2];  /* bucket examinations: [0] level 0, [1] all backing levels */
 } or_table;
 
-static inline int is_kv_rep(uint32_t rep) { return rep == OR_REP_KV64 || rep == OR_REP_KEY64; }
+static inline int is_kv_rep(uint32_t rep)
+{
+    return rep == OR_REP_KV64 || rep == OR_REP_KEY64 || rep == OR_REP_KV16;
+}
 static inline int narrow(const or_table *t)
 {
     return t->rep == OR_REP_FP8_V8 || t->rep == OR_REP_FP10_V6 || t->rep == OR_REP_FP12;
@@ -391,6 +394,7 @@ static inline uint32_t kmask_of(const or_table *t) { return (1u << kbits_of(t)) 
 static inline uint32_t slot_bytes(const or_table *t)
 {
     if (t->rep == OR_REP_KEY64) return 8u;
+    if (t->rep == OR_REP_KV16) return 4u;
     return t->rep == OR_REP_KV64 ? 16u : (narrow(t) ? 2u : 4u);
 }
 /* Bytes of one bucket: B slots, or (12*B - 1)/16 + 1 uint16 words for the
@@ -400,6 +404,7 @@ static inline uint64_t bucket_bytes_of(uint32_t rep, uint32_t B)
     if (rep == OR_REP_FP12) return 2ull * ((12ull * B - 1) / 16 + 1);
     if (rep == OR_REP_KV64) return 16ull * B;
     if (rep == OR_REP_KEY64) return 8ull * B;
+    if (rep == OR_REP_KV16) return 4ull * B;
     return (uint64_t)B * ((rep == OR_REP_FP16_V16) ? 4u : 2u);
 }
 
@@ -663,22 +668,60 @@ static int linear_remove(or_table *t, or_level *L, int lvl, uint64_t key)
 /* dynamic_container<key_container, uint64_t> (KEY64) the CMS allocator   */
 /* keeps its free list in (allocators/cms.cuh:203).                     */
 /* ------------------------------------------------------------------ */
+/* Key-value slots.  KV64 = key_val_pair<u64,u64> {key, val} (16 B);
+ * KEY64 = dynamic_container<key_container, u64> (8 B key); KV16 =
+ * dynamic_container<key_val_pair, u16> (4 B {u16 key, u16 val}, as one
+ * little-endian word val << 16 | key).  The dynamic containers store
+ * get_smallKey(key): KEY64 remaps 0 -> 1 and ~0 -> ~0 - 1, KV16 keeps the
+ * low 16 bits with 0 -> 1 and 0xFFFF -> 0xFFFE
+ * (representations/dynamic_container.cuh:35-50); its tag() is that small key
+ * (:52-54), key_val_pair's is the key (key_val_pair.cuh:124-126).  Empty key
+ * 0, tombstone key get_empty() - 1 (~0 or 0xFFFF, key_val_pair.cuh:70,
+ * 115-117); atomic_reset changes the key only (:143-151). */
 static inline int kv_keyonly(const or_table *t) { return t->rep == OR_REP_KEY64; }
-static inline uint64_t *kv_slot_t(const or_table *t, or_level *L, uint64_t s)
+static inline uint64_t kv_tomb_key(const or_table *t) { return t->rep == OR_REP_KV16 ? 0xFFFFull : KV_TOMB; }
+static inline uint64_t kv_key_at(const or_table *t, const or_level *L, uint64_t s)
 {
-    return (uint64_t *)L->slots + (kv_keyonly(t) ? 1 : 2) * s;
+    if (t->rep == OR_REP_KV16) return ((const uint32_t *)L->slots)[s] & 0xFFFFu;
+    return ((const uint64_t *)L->slots)[(kv_keyonly(t) ? 1 : 2) * s];
 }
-#define kv_slot(L, s) kv_slot_t(t, (L), (s))
-/* The stored key: key_val_pair keeps the key itself; the key-only container
- * stores get_smallKey(key) with SmallKey = uint64_t, i.e. 0 -> 1 and
- * ~0 -> ~0 - 1 (representations/dynamic_container.cuh:35-50 over
- * key_only.cuh:41-47,70-73).  Probing always hashes the key as given. */
+static inline uint64_t kv_val_at(const or_table *t, const or_level *L, uint64_t s)
+{
+    if (t->rep == OR_REP_KV16) return ((const uint32_t *)L->slots)[s] >> 16;
+    return kv_keyonly(t) ? 0 : ((const uint64_t *)L->slots)[2 * s + 1];
+}
+static inline void kv_put(const or_table *t, or_level *L, uint64_t s, uint64_t skey, uint64_t val)
+{
+    if (t->rep == OR_REP_KV16) {
+        ((uint32_t *)L->slots)[s] = (uint32_t)((val & 0xFFFFu) << 16 | (skey & 0xFFFFu));
+        return;
+    }
+    uint64_t *p = (uint64_t *)L->slots + (kv_keyonly(t) ? 1 : 2) * s;
+    p[0] = skey;
+    if (!kv_keyonly(t)) p[1] = val;
+}
+static inline void kv_bury(const or_table *t, or_level *L, uint64_t s)
+{
+    if (t->rep == OR_REP_KV16) {
+        uint32_t *w = (uint32_t *)L->slots + s;
+        *w = (*w & 0xFFFF0000u) | 0xFFFFu;
+        return;
+    }
+    ((uint64_t *)L->slots)[(kv_keyonly(t) ? 1 : 2) * s] = KV_TOMB;
+}
+/* The stored key: the key itself (key_val_pair) or get_smallKey(key). */
 static inline uint64_t kv_skey(const or_table *t, uint64_t key)
 {
+    if (t->rep == OR_REP_KV16) return fp_of(OR_REP_FP16_V16, key);
     if (!kv_keyonly(t)) return key;
     if (key == KV_EMPTY) key += 1;
     if (key == KV_TOMB) key -= 1;
     return key;
+}
+/* The probe tag (XOR power-of-two probing). */
+static inline uint64_t kv_tag(const or_table *t, uint64_t key)
+{
+    return t->rep == OR_REP_KV64 ? key : kv_skey(t, key);
 }
 
 /* Sector accounting for KV probes: count distinct 64 B sectors in the
@@ -704,16 +747,14 @@ static int kv_insert(or_table *t, or_level *L, int lvl, uint64_t key, uint64_t v
     or_probe_it it;
     uint64_t last = OR_END;
     for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
-                                  L->seed, key, key);
+                                  L->seed, key, kv_tag(t, key));
          p != OR_END; p = probe_next(&it)) {
         uint64_t b = p % L->nb;
         uint64_t base = b * L->B;
         for (uint32_t i = 0; i < L->B; i++) {
-            uint64_t *s = kv_slot(L, base + i);
-            if (s[0] == KV_EMPTY) {
+            if (kv_key_at(t, L, base + i) == KV_EMPTY) {
                 account_kv_range(t, lvl, base, base + i + 1, &last);
-                s[0] = kv_skey(t, key);
-                if (!kv_keyonly(t)) s[1] = val;
+                kv_put(t, L, base + i, kv_skey(t, key), val);
                 t->sectors_written += 1;
                 return 1;
             }
@@ -735,19 +776,17 @@ static int kv_query(or_table *t, or_level *L, int lvl, uint64_t key, uint64_t *v
     or_probe_it it;
     uint64_t last = OR_END;
     for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
-                                  L->seed, key, key);
+                                  L->seed, key, kv_tag(t, key));
          p != OR_END; p = probe_next(&it)) {
         uint64_t b = p % L->nb;
         uint64_t base = b * L->B;
         for (uint32_t s0 = 0; s0 < L->B; s0 += P) {
             uint32_t s1 = s0 + P < L->B ? s0 + P : L->B;
             account_kv_range(t, lvl, base + s0, base + s1, &last);
-            for (uint32_t i = s0; i < s1; i++) {
-                uint64_t *s = kv_slot(L, base + i);
-                if (s[0] == kv_skey(t, key)) { *val = kv_keyonly(t) ? 0 : s[1]; return 1; }
-            }
             for (uint32_t i = s0; i < s1; i++)
-                if (kv_slot(L, base + i)[0] == KV_EMPTY) return 0;
+                if (kv_key_at(t, L, base + i) == kv_skey(t, key)) { *val = kv_val_at(t, L, base + i); return 1; }
+            for (uint32_t i = s0; i < s1; i++)
+                if (kv_key_at(t, L, base + i) == KV_EMPTY) return 0;
         }
     }
     return 0;
@@ -762,7 +801,7 @@ static int kv_remove(or_table *t, or_level *L, int lvl, uint64_t key)
     or_probe_it it;
     uint64_t last = OR_END;
     for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes,
-                                  L->seed, key, key);
+                                  L->seed, key, kv_tag(t, key));
          p != OR_END; p = probe_next(&it)) {
         uint64_t b = p % L->nb;
         uint64_t base = b * L->B;
@@ -770,18 +809,119 @@ static int kv_remove(or_table *t, or_level *L, int lvl, uint64_t key)
             uint32_t s1 = s0 + P < L->B ? s0 + P : L->B;
             account_kv_range(t, lvl, base + s0, base + s1, &last);
             for (uint32_t i = s0; i < s1; i++) {
-                uint64_t *s = kv_slot(L, base + i);
-                if (s[0] == kv_skey(t, key)) {
-                    s[0] = KV_TOMB;
+                if (kv_key_at(t, L, base + i) == kv_skey(t, key)) {
+                    kv_bury(t, L, base + i);
                     t->sectors_written += 1;
                     return 1;
                 }
             }
             for (uint32_t i = s0; i < s1; i++)
-                if (kv_slot(L, base + i)[0] == KV_EMPTY) return 0;
+                if (kv_key_at(t, L, base + i) == KV_EMPTY) return 0;
         }
     }
     return 0;
+}
+
+/* The flat power-of-n schemes over key-value slots: power_of_n_insert_scheme
+ * (insert_schemes/power_of_n.cuh:333-379) and power_of_n_insert_shortcut_scheme
+ * (power_of_n_shortcut.cuh:325-406, data_structs/tcf.cuh:30), each "bucket"
+ * being B consecutive slots.  Unlike the packed-bucket scheme:
+ *   fill      = slots whose key is not empty, tombstones included
+ *               (check_fill_bucket, power_of_n_shortcut.cuh:297-316);
+ *   claim     = the lowest EMPTY slot only (insert_into_bucket, :140-190);
+ *   query     = every probe bucket in order, lowest match (:192-230, :431-446);
+ *   remove    = the lowest match's key -> tombstone, value kept (:233-295).
+ * insert: the shortcut scheme first tries each probe bucket whose fill is
+ * below 0.75 B (4*fill < 3*B); then, for both schemes, the fill levels in
+ * ascending order, each probe with that fill in probe order. */
+static int kv_pow_fill(or_table *t, or_level *L, uint64_t b)
+{
+    int f = 0;
+    for (uint32_t i = 0; i < L->B; i++) f += kv_key_at(t, L, b * L->B + i) != KV_EMPTY;
+    return f;
+}
+
+static int kv_pow_claim(or_table *t, or_level *L, uint64_t b, uint64_t key, uint64_t val)
+{
+    for (uint32_t i = 0; i < L->B; i++) {
+        if (kv_key_at(t, L, b * L->B + i) == KV_EMPTY) {
+            kv_put(t, L, b * L->B + i, kv_skey(t, key), val);
+            t->sectors_written += 1;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+static int kv_pow_insert(or_table *t, or_level *L, int lvl, uint64_t key, uint64_t val)
+{
+    const int shortcut = L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+    const int B = (int)L->B;
+    uint64_t buckets[MAXP];
+    int fill[MAXP];
+    int np = 0, min_fill = B;
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes, L->seed, key, kv_tag(t, key));
+         p != OR_END && np < MAXP; p = probe_next(&it)) {
+        const uint64_t b = p % L->nb;
+        buckets[np] = b;
+        account_bucket(t, lvl, b);
+        int f = kv_pow_fill(t, L, b);
+        if (shortcut && 4 * f < 3 * B) {
+            if (kv_pow_claim(t, L, b, key, val)) return 1;
+            f = B;
+        }
+        fill[np] = f;
+        if (f < min_fill) min_fill = f;
+        np++;
+    }
+    for (int lvl_fill = min_fill; lvl_fill < B;) {
+        int next = B;
+        for (int j = 0; j < np; j++) {
+            if (fill[j] == lvl_fill && kv_pow_claim(t, L, buckets[j], key, val)) return 1;
+            if (fill[j] > lvl_fill && fill[j] < next) next = fill[j];
+        }
+        lvl_fill = next;
+    }
+    return 0;
+}
+
+static int kv_pow_query(or_table *t, or_level *L, int lvl, uint64_t key, uint64_t *val)
+{
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes, L->seed, key, kv_tag(t, key));
+         p != OR_END; p = probe_next(&it)) {
+        const uint64_t b = p % L->nb;
+        account_bucket(t, lvl, b);
+        for (uint32_t i = 0; i < L->B; i++)
+            if (kv_key_at(t, L, b * L->B + i) == kv_skey(t, key)) {
+                *val = kv_val_at(t, L, b * L->B + i);
+                return 1;
+            }
+    }
+    return 0;
+}
+
+static int kv_pow_remove(or_table *t, or_level *L, int lvl, uint64_t key)
+{
+    or_probe_it it;
+    for (uint64_t p = probe_begin(&it, L->d.prober, L->d.hasher, L->d.max_probes, L->seed, key, kv_tag(t, key));
+         p != OR_END; p = probe_next(&it)) {
+        const uint64_t b = p % L->nb;
+        account_bucket(t, lvl, b);
+        for (uint32_t i = 0; i < L->B; i++)
+            if (kv_key_at(t, L, b * L->B + i) == kv_skey(t, key)) {
+                kv_bury(t, L, b * L->B + i);
+                t->sectors_written += 1;
+                return 1;
+            }
+    }
+    return 0;
+}
+
+static inline int kv_pow_scheme(const or_level *L)
+{
+    return L->d.scheme == OR_SCHEME_POWER_OF_N || L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
 }
 
 /* ------------------------------------------------------------------ */
@@ -796,7 +936,7 @@ static int kv_remove(or_table *t, or_level *L, int lvl, uint64_t key)
  * slot scheme allocates nslots slots (single_slot_insert.cuh:89-96). */
 or_table *or_create(uint32_t rep, uint32_t nlevels, const or_level_desc *lv, uint64_t seed)
 {
-    if (nlevels < 1 || nlevels > OR_MAX_LEVELS || rep > OR_REP_KEY64) return NULL;
+    if (nlevels < 1 || nlevels > OR_MAX_LEVELS || rep > OR_REP_KV16) return NULL;
     for (uint32_t l = 0; l < nlevels; l++) {
         /* GROUPED levels hold `group` packed buckets per 128-byte line
          * (grouped_power_buckets.cuh:58: sizeof(bucket) * K <= 128). */
@@ -805,7 +945,13 @@ or_table *or_create(uint32_t rep, uint32_t nlevels, const or_level_desc *lv, uin
         if (grouped && (is_kv_rep(rep) || lv[l].bucket_size == 0 ||
                         lv[l].group * bucket_bytes_of(rep, lv[l].bucket_size) > 128))
             return NULL;
-        if (is_kv_rep(rep) != (lv[l].scheme <= OR_SCHEME_SINGLE_SLOT)) return NULL;
+        const int kv_scheme = lv[l].scheme <= OR_SCHEME_SINGLE_SLOT || lv[l].scheme == OR_SCHEME_POWER_OF_N ||
+                              lv[l].scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+        if (is_kv_rep(rep) != kv_scheme) return NULL;
+        /* the flat power-of-n schemes keep one fill per probe (MAXP) */
+        if ((lv[l].scheme == OR_SCHEME_POWER_OF_N || lv[l].scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS) &&
+            (lv[l].max_probes == 0 || lv[l].max_probes > MAXP))
+            return NULL;
     }
     or_table *t = (or_table *)calloc(1, sizeof(or_table));
     if (!t) return NULL;
@@ -874,8 +1020,11 @@ static int lvl_remove(or_table *t, int l, uint64_t key)
 int or_insert(or_table *t, uint64_t key, uint64_t val, int with_delete)
 {
     if (is_kv_rep(t->rep)) {
-        for (uint32_t l = 0; l < t->nlevels; l++)
-            if (kv_insert(t, &t->lv[l], (int)l, key, val)) return 1 + (int)l;
+        for (uint32_t l = 0; l < t->nlevels; l++) {
+            or_level *L = &t->lv[l];
+            if (kv_pow_scheme(L) ? kv_pow_insert(t, L, (int)l, key, val) : kv_insert(t, L, (int)l, key, val))
+                return 1 + (int)l;
+        }
         return 0;
     }
     /* twelve_bucket has no insert_delete that compiles (12_bit_bucket.cuh:
@@ -891,8 +1040,11 @@ int or_insert(or_table *t, uint64_t key, uint64_t val, int with_delete)
 int or_query(or_table *t, uint64_t key, uint64_t *val)
 {
     if (is_kv_rep(t->rep)) {
-        for (uint32_t l = 0; l < t->nlevels; l++)
-            if (kv_query(t, &t->lv[l], (int)l, key, val)) return 1;
+        for (uint32_t l = 0; l < t->nlevels; l++) {
+            or_level *L = &t->lv[l];
+            if (kv_pow_scheme(L) ? kv_pow_query(t, L, (int)l, key, val) : kv_query(t, L, (int)l, key, val))
+                return 1;
+        }
         return 0;
     }
     for (uint32_t l = 0; l < t->nlevels; l++) {
@@ -905,9 +1057,11 @@ int or_query(or_table *t, uint64_t key, uint64_t *val)
 int or_remove(or_table *t, uint64_t key)
 {
     if (is_kv_rep(t->rep)) {
-        for (uint32_t l = 0; l < t->nlevels; l++) {
+        for (uint32_t l = 0; l < t->nlevels; l++)
             if (t->lv[l].d.scheme == OR_SCHEME_SINGLE_SLOT) return -1; /* no remove in the reference */
-            if (kv_remove(t, &t->lv[l], (int)l, key)) return 1;
+        for (uint32_t l = 0; l < t->nlevels; l++) {
+            or_level *L = &t->lv[l];
+            if (kv_pow_scheme(L) ? kv_pow_remove(t, L, (int)l, key) : kv_remove(t, L, (int)l, key)) return 1;
         }
         return 0;
     }
@@ -1052,8 +1206,8 @@ uint64_t or_fill(const or_table *t)
         const or_level *L = &t->lv[l];
         if (is_kv_rep(t->rep)) {
             for (uint64_t s = 0; s < L->nslot_alloc; s++) {
-                uint64_t k = ((const uint64_t *)L->slots)[(kv_keyonly(t) ? 1 : 2) * s];
-                if (k != KV_EMPTY && k != KV_TOMB) f++;
+                uint64_t k = kv_key_at(t, L, s);
+                if (k != KV_EMPTY && k != kv_tomb_key(t)) f++;
             }
             continue;
         }

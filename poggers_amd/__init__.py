@@ -21,8 +21,9 @@ LIB_PATH = os.path.join(_HERE, "libpoggers_amd.so")
 
 # enum values of include/poggers_amd.h
 PG_OK = 0
-REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12, REP_KEY64 = 0, 1, 2, 3, 4, 5
+REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12, REP_KEY64, REP_KV16 = 0, 1, 2, 3, 4, 5, 6
 SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS, SCHEME_GROUPED = 0, 1, 2, 3, 4
+SCHEME_POWER_OF_N, SCHEME_POWER_OF_N_SHORTCUT_SLOTS = 5, 6
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
 CONCURRENT, DETERMINISTIC = 0, 1
@@ -243,6 +244,20 @@ def key_only_desc(nslots: int, bucket_size: int = 4, tile: int = 1, prober: int 
     the defaults are the CMS allocator's free table (allocators/cms.cuh:203:
     tile 1, 4-slot buckets, bucket_insert, double hashing, 20 probes)."""
     return TableDesc(REP_KEY64, [Level(nslots, SCHEME_BUCKET, bucket_size, tile, prober, hasher, max_probes)])
+
+
+def flat_tcf_desc(nslots: int, backing_slots: int | None = None, tile: int = 4) -> TableDesc:
+    """The reference's flat TCF (data_structs/tcf.cuh:29-30): 16/16
+    key_val_pair slots (REP_KV16) in 16-slot buckets under
+    power_of_n_insert_shortcut_scheme with two double-hashing probes, backed
+    by tiny_static_table_4 (4-slot buckets, bucket_insert, 20 double-hashing
+    probes).  Without backing_slots the backing level gets nslots / 100 slots
+    (at least one bucket)."""
+    if backing_slots is None:
+        backing_slots = max(4, nslots // 100)
+    return TableDesc(REP_KV16, [
+        Level(nslots, SCHEME_POWER_OF_N_SHORTCUT_SLOTS, 16, tile, PROBE_DOUBLE, HASH_MURMUR64A, 2),
+        Level(backing_slots, SCHEME_BUCKET, 4, tile, PROBE_DOUBLE, HASH_MURMUR64A, 20)])
 
 
 class Table:

@@ -37,9 +37,13 @@ int check_level(uint32_t rep, const pg_level_desc &l, uint32_t index) {
     if ((l.prober == PG_PROBE_P2 || l.prober == PG_PROBE_XOR_P2 || l.prober == PG_PROBE_SPLIT64) &&
         l.max_probes != 2)
         return PG_ERR_INVALID;
-    if (rep == PG_REP_KV64 || rep == PG_REP_KEY64) {
-        if (l.scheme != PG_SCHEME_BUCKET && l.scheme != PG_SCHEME_SINGLE_SLOT) return PG_ERR_UNSUPPORTED;
-        if (l.scheme == PG_SCHEME_BUCKET && (l.bucket_size == 0 || l.bucket_size > 64)) return PG_ERR_INVALID;
+    if (pg::is_kv_rep(rep)) {
+        const bool pow = l.scheme == PG_SCHEME_POWER_OF_N || l.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+        if (l.scheme != PG_SCHEME_BUCKET && l.scheme != PG_SCHEME_SINGLE_SLOT && !pow) return PG_ERR_UNSUPPORTED;
+        if (l.scheme != PG_SCHEME_SINGLE_SLOT && (l.bucket_size == 0 || l.bucket_size > 64)) return PG_ERR_INVALID;
+        // the per-probe fills live in registers (power_of_n.cuh:336-337
+        // keeps Max_Probes of them)
+        if (pow && l.max_probes > pg::kMaxShortcutProbes) return PG_ERR_UNSUPPORTED;
     } else if (rep <= PG_REP_FP12) {  // the packed filter slots
         if (l.scheme != PG_SCHEME_POWER_OF_N_SHORTCUT && l.scheme != PG_SCHEME_LINEAR_BUCKETS &&
             l.scheme != PG_SCHEME_GROUPED_SHORTCUT)
@@ -208,7 +212,7 @@ static uint64_t value_bytes(uint32_t rep) {
 // with its key, value and output pointers advanced.
 static int run_op(pg_handle h, pg::Op op, const pg::OpArgs &a, hipStream_t s) {
     const uint32_t tile = h->desc.level[0].tile;
-    const bool kv = h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_KEY64;
+    const bool kv = pg::is_kv_rep(h->dev.rep);
     const uint64_t max_keys = (1ull << 31) / tile;
     const uint64_t vb = value_bytes(h->dev.rep);
     uint64_t done = 0;
@@ -255,7 +259,7 @@ int pg_bulk_insert_with_delete(pg_handle h, const uint64_t *d_keys, const void *
     if (!h) return PG_ERR_INVALID;
     // bucket_insert / single_slot_insert have no insert_with_delete, and
     // twelve_bucket's does not compile (12_bit_bucket.cuh:148-203).
-    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_KEY64 || h->dev.rep == PG_REP_FP12)
+    if (pg::is_kv_rep(h->dev.rep) || h->dev.rep == PG_REP_FP12)
         return PG_ERR_UNSUPPORTED;
     return do_insert(h, d_keys, d_vals, n, d_status, d_fail_count, stream, 1);
 }
@@ -265,7 +269,7 @@ static int do_upsert(pg_handle h, const uint64_t *d_keys, const void *d_vals, ui
     if (!h || (n && !d_keys)) return PG_ERR_INVALID;
     // bucket_insert / single_slot_insert have no insert_if_not_exists; not
     // offered for the 12-bit bucket (its _delete form needs remove).
-    if (h->dev.rep == PG_REP_KV64 || h->dev.rep == PG_REP_KEY64 || h->dev.rep == PG_REP_FP12)
+    if (pg::is_kv_rep(h->dev.rep) || h->dev.rep == PG_REP_FP12)
         return PG_ERR_UNSUPPORTED;
     pg::OpArgs a{};
     a.keys = d_keys;

@@ -123,7 +123,12 @@ __host__ __device__ inline uint64_t bucket_bytes(uint32_t rep, uint32_t B) {
     if (rep == PG_REP_FP12) return 2ull * ((12ull * B - 1) / 16 + 1);
     if (rep == PG_REP_KV64) return 16ull * B;
     if (rep == PG_REP_KEY64) return 8ull * B;
-    return (uint64_t)B * (rep == PG_REP_FP16_V16 ? 4u : 2u);
+    return (uint64_t)B * (rep == PG_REP_FP16_V16 || rep == PG_REP_KV16 ? 4u : 2u);
+}
+// Key-value slot tables (the bucket_insert / single_slot_insert / flat
+// power-of-n family); the others are the packed filter slots.
+__host__ __device__ inline bool is_kv_rep(uint32_t rep) {
+    return rep == PG_REP_KV64 || rep == PG_REP_KEY64 || rep == PG_REP_KV16;
 }
 
 // Start of bucket b: plain levels are arrays of buckets; a grouped level puts

@@ -1,5 +1,5 @@
-// pg_kv.hip -- bulk kernels for key_val_pair<u64,u64> tables and key-only
-// tables (device ops in pg_kv_ops.hpp).
+// pg_kv.hip -- bulk kernels for the key-value slot tables: key_val_pair<u64,
+// u64>, key-only and 16/16 key_val_pair slots (device ops in pg_kv_ops.hpp).
 #include "pg_kv_ops.hpp"
 
 namespace pg {
@@ -9,6 +9,7 @@ constexpr int kBlock = 256;
 template <int P, bool ACC, class SL>
 __global__ __launch_bounds__(kBlock) void k_kv_insert(TableDev T, OpArgs a) {
     Tile<P> t;
+    t.concurrent = true;
     const uint64_t tid = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
     const bool act = tid < a.n;
     int st = 1;
@@ -100,6 +101,7 @@ static int launch_kv_ps(const TableDev &T, Op op, const OpArgs &a, hipStream_t s
 template <int P>
 static int launch_kv_p(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
     if (T.rep == PG_REP_KEY64) return launch_kv_ps<P, SlotKey>(T, op, a, s);
+    if (T.rep == PG_REP_KV16) return launch_kv_ps<P, SlotKV16>(T, op, a, s);
     return launch_kv_ps<P, SlotKV>(T, op, a, s);
 }
 

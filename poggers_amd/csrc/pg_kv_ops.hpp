@@ -1,11 +1,13 @@
 // pg_kv_ops.hpp -- per-key device ops for key_val_pair<u64,u64>
-// tables (representations/key_val_pair.cuh:63-171) and for the key-only
+// tables (representations/key_val_pair.cuh:63-171), for the key-only
 // dynamic_container<key_container, uint64_t> tables (the CMS allocator's
-// free table, allocators/cms.cuh:203), gfx950 wave64.
+// free table, allocators/cms.cuh:203) and for the 4-byte
+// dynamic_container<key_val_pair<u64,u16>, u16> slots of the flat TCF,
+// gfx950 wave64.
 //
-// Layout in HBM: 16 B slots {key, val} (SlotKV) or 8 B keys (SlotKey), B
-// contiguous slots per bucket, bucket b at slot b*B.  Empty key 0,
-// tombstone key ~0.
+// Layout in HBM: 16 B slots {key, val} (SlotKV), 8 B keys (SlotKey) or
+// 32-bit words val16 << 16 | key16 (SlotKV16), B contiguous slots per
+// bucket, bucket b at slot b*B.  Empty key 0, tombstone key all ones.
 //
 // Two tile shapes, chosen per table:
 //   * stripe mode (B > 1): lane r of a P-lane tile owns slot stripe*P + r of
@@ -33,19 +35,33 @@ constexpr uint64_t kKvTomb = ~0ULL;
 
 // Slot layouts.  Probing always hashes the key as given; the stored key is
 // stored(key): the key itself for key_val_pair, get_smallKey(key) for the
-// key-only container (0 -> 1, ~0 -> ~0 - 1, dynamic_container.cuh:35-50).
+// dynamic_container slots (0 -> 1, all-ones -> all-ones - 1,
+// dynamic_container.cuh:35-50).  tag(key) is rep_type::tag, the probe tag of
+// the xor prober: the key for key_val_pair, the small key otherwise.
+// claim(p, cur, sk, v) takes slot p, seen as `cur` with an empty key;
+// bury(p, cur, sk) turns the key of slot p, seen as `cur` holding sk, into
+// the tombstone.  Both return whether their swap landed.
 struct SlotKV {
     using S = ulonglong2;
     using V = uint64_t;  // value buffer type
     __device__ static uint64_t stored(uint64_t key) { return key; }
+    __device__ static uint64_t tag(uint64_t key) { return key; }
     __device__ static S load(const S *p) { return *p; }
     __device__ static S none() { return make_ulonglong2(kKvTomb, 0); }
     __device__ static uint64_t key(const S &s) { return s.x; }
     __device__ static uint64_t val(const S &s) { return s.y; }
-    __device__ static unsigned long long *key_ptr(S *p) { return (unsigned long long *)&p->x; }
     // key_val_pair::atomic_swap: CAS key 0 -> key, then store the value
     // (key_val_pair.cuh:84-95)
-    __device__ static void store_val(S *p, uint64_t v) { p->y = v; }
+    __device__ static bool claim(S *p, const S &, uint64_t sk, uint64_t v) {
+        if (atomicCAS((unsigned long long *)&p->x, kKvEmpty, sk) != kKvEmpty) return false;
+        p->y = v;
+        return true;
+    }
+    // key_val_pair::atomic_reset: CAS key -> tombstone key, value kept
+    // (key_val_pair.cuh:143-151)
+    __device__ static bool bury(S *p, const S &, uint64_t sk) {
+        return atomicCAS((unsigned long long *)&p->x, sk, kKvTomb) == sk;
+    }
 };
 struct SlotKey {
     using S = uint64_t;
@@ -55,12 +71,39 @@ struct SlotKey {
         key -= (key == kKvTomb);
         return key;
     }
+    __device__ static uint64_t tag(uint64_t key) { return stored(key); }
     __device__ static S load(const S *p) { return *p; }
     __device__ static S none() { return kKvTomb; }
     __device__ static uint64_t key(const S &s) { return s; }
     __device__ static uint64_t val(const S &) { return 0; }
-    __device__ static unsigned long long *key_ptr(S *p) { return (unsigned long long *)p; }
-    __device__ static void store_val(S *, uint64_t) {}
+    __device__ static bool claim(S *p, const S &, uint64_t sk, uint64_t) {
+        return atomicCAS((unsigned long long *)p, kKvEmpty, sk) == kKvEmpty;
+    }
+    __device__ static bool bury(S *p, const S &, uint64_t sk) {
+        return atomicCAS((unsigned long long *)p, sk, kKvTomb) == sk;
+    }
+};
+// dynamic_container<key_val_pair<uint64_t, uint16_t>, uint16_t>: one 32-bit
+// word val16 << 16 | key16 (the flat TCF's slot, data_structs/tcf.cuh:29-30).
+// The reference swaps the 16-bit key and then writes the value; here one
+// 32-bit CAS writes both, from the word seen empty, so a reader never sees
+// the key without its value.  A tombstone keeps the value bits
+// (key_val_pair::atomic_reset sets the key only).
+struct SlotKV16 {
+    using S = uint32_t;
+    using V = uint16_t;
+    __device__ static uint64_t stored(uint64_t key) { return fp16_of(key); }
+    __device__ static uint64_t tag(uint64_t key) { return fp16_of(key); }
+    __device__ static S load(const S *p) { return *p; }
+    __device__ static S none() { return 0xFFFFu; }
+    __device__ static uint64_t key(const S &s) { return s & 0xFFFFu; }
+    __device__ static uint64_t val(const S &s) { return s >> 16; }
+    __device__ static bool claim(S *p, const S &cur, uint64_t sk, uint64_t v) {
+        return atomicCAS(p, cur, ((uint32_t)v << 16) | (uint32_t)sk) == cur;
+    }
+    __device__ static bool bury(S *p, const S &cur, uint64_t) {
+        return atomicCAS(p, cur, (cur & 0xFFFF0000u) | 0xFFFFu) == cur;
+    }
 };
 
 // Lanes strictly below the lowest set bit of `endm` (all lanes when 0).
@@ -87,7 +130,7 @@ template <int P, int NS, class SL>
 __device__ bool kv_reg_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
     using S = typename SL::S;
     Probe pr;
-    pr.begin(L, key, key);
+    pr.begin(L, key, SL::tag(key));
     pr.max_probes = pr.useful_probes(L.nb.d);
     const uint64_t sk = SL::stored(key);
     S *slots = (S *)L.slots;
@@ -104,11 +147,7 @@ __device__ bool kv_reg_insert(const LevelDev &L, uint64_t key, uint64_t val, con
             while (cm) {
                 const uint32_t l = (uint32_t)ctz64(cm);
                 bool success = false;
-                if (t.rank == l) {
-                    S *sp = slots + base + k * P + t.rank;
-                    success = atomicCAS(SL::key_ptr(sp), kKvEmpty, sk) == kKvEmpty;
-                    if (success) SL::store_val(sp, val);
-                }
+                if (t.rank == l) success = SL::claim(slots + base + k * P + t.rank, bk.s[k], sk, val);
                 if (t.any(success)) {
                     t.acc[2]++;
                     return true;
@@ -123,7 +162,7 @@ template <int P, int NS, class SL>
 __device__ bool kv_reg_query(const LevelDev &L, uint64_t key, const Tile<P> &t, uint64_t &val, bool &owner) {
     using S = typename SL::S;
     Probe pr;
-    pr.begin(L, key, key);
+    pr.begin(L, key, SL::tag(key));
     pr.max_probes = pr.useful_probes(L.nb.d);
     const uint64_t sk = SL::stored(key);
     const S *slots = (const S *)L.slots;
@@ -151,7 +190,7 @@ template <int P, int NS, class SL>
 __device__ bool kv_reg_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
     using S = typename SL::S;
     Probe pr;
-    pr.begin(L, key, key);
+    pr.begin(L, key, SL::tag(key));
     pr.max_probes = pr.useful_probes(L.nb.d);
     const uint64_t sk = SL::stored(key);
     S *slots = (S *)L.slots;
@@ -169,7 +208,7 @@ __device__ bool kv_reg_remove(const LevelDev &L, uint64_t key, const Tile<P> &t)
             while (mm) {
                 const uint32_t l = (uint32_t)ctz64(mm);
                 bool success = false;
-                if (t.rank == l) success = atomicCAS(SL::key_ptr(slots + base + k * P + t.rank), sk, kKvTomb) == sk;
+                if (t.rank == l) success = SL::bury(slots + base + k * P + t.rank, bk.s[k], sk);
                 if (t.any(success)) {
                     t.acc[2]++;
                     return true;
@@ -181,6 +220,202 @@ __device__ bool kv_reg_remove(const LevelDev &L, uint64_t key, const Tile<P> &t)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Flat power-of-n schemes over key-value slots: power_of_n_insert_scheme
+// (insert_schemes/power_of_n.cuh:38-379) and power_of_n_insert_shortcut_scheme
+// (power_of_n_shortcut.cuh:38-449).  A bucket's fill counts every non-empty
+// slot, tombstones included (check_fill_bucket, power_of_n_shortcut.cuh:
+// 297-316); a claim takes the lowest empty slot (insert_into_bucket,
+// :130-190); queries and removes check every probe bucket in probe order and
+// act on the lowest matching slot (:192-295).  Runtime B, stripes of P slots,
+// each bucket read again for the claim as in the reference.
+// ---------------------------------------------------------------------------
+template <int P, class SL>
+__device__ int kv_pow_fill(const typename SL::S *bk, uint32_t B, const Tile<P> &t) {
+    int f = 0;
+    for (uint32_t s0 = 0; s0 < B; s0 += P) {
+        const uint32_t i = s0 + t.rank;
+        f += t.popc(i < B && SL::key(SL::load(bk + i)) != kKvEmpty);
+    }
+    t.acc[0] += B;
+    return f;
+}
+
+// The lowest empty slot among [0, lim).
+template <int P, class SL>
+__device__ bool kv_pow_claim(typename SL::S *bk, uint32_t B, uint32_t lim, uint64_t sk, uint64_t val,
+                             const Tile<P> &t) {
+    using S = typename SL::S;
+    const uint32_t end = lim < B ? lim : B;
+    for (uint32_t s0 = 0; s0 < end; s0 += P) {
+        const uint32_t i = s0 + t.rank;
+        const bool ok = i < end;
+        const S s = ok ? SL::load(bk + i) : SL::none();
+        t.acc[0] += (end - s0 < P) ? end - s0 : P;
+        uint64_t cm = t.bits(ok && SL::key(s) == kKvEmpty);
+        while (cm) {
+            const uint32_t l = (uint32_t)ctz64(cm);
+            bool success = false;
+            if (t.rank == l) success = SL::claim(bk + i, s, sk, val);
+            if (t.any(success)) {
+                t.acc[2]++;
+                return true;
+            }
+            cm &= cm - 1ull;
+        }
+    }
+    return false;
+}
+
+// power_of_n.cuh:333-379 / power_of_n_shortcut.cuh:325-406.  The shortcut
+// claims in a probe bucket whose fill is below 3B/4.  Since empties are
+// claimed lowest first and slots never return to empty, the non-empty slots
+// of a bucket are always a prefix, so a claim at slot i lands at fill i:
+// concurrent kernels bound the shortcut claim to slots below ceil(3B/4),
+// which applies the rule at the moment of the claim, and a bucket whose
+// bounded claim fails enters the fill-ordered pass at fill ceil(3B/4).  In
+// key order the bound is B, and a failed claim means the bucket is full.
+template <int P, class SL>
+__device__ bool kv_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
+    using S = typename SL::S;
+    Probe pr;
+    pr.begin(L, key, SL::tag(key));
+    const bool shortcut = L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+    const uint64_t sk = SL::stored(key);
+    S *slots = (S *)L.slots;
+    const int B = (int)L.B;
+    const uint32_t lim1 = t.concurrent ? (3u * L.B + 3u) / 4u : L.B;
+    uint64_t bidx[kMaxShortcutProbes];
+    int fill[kMaxShortcutProbes];
+    int np = 0, min_fill = B;
+#pragma unroll
+    for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) break;
+        const uint64_t b = fastmod(p, L.nb);
+        bidx[j] = b;
+        int f = kv_pow_fill<P, SL>(slots + b * L.B, L.B, t);
+        if (shortcut && 4 * f < 3 * B) {
+            if (kv_pow_claim<P, SL>(slots + b * L.B, L.B, lim1, sk, val, t)) return true;
+            f = (int)lim1;
+        }
+        fill[j] = f;
+        if (f < min_fill) min_fill = f;
+        np = j + 1;
+    }
+    if (t.concurrent) {
+        // Fills read while many keys are in flight are stale: every key of a
+        // fresh table reads zeros and would pile into its first probe bucket.
+        // So the least-filled bucket (first in probe order on ties) is
+        // claimed only at a slot index no larger than the next-least fill,
+        // which by the prefix property keeps it no fuller than the others at
+        // the claim; a failed claim re-reads the fills.  Each failure means
+        // some probe bucket gained a key since its read, so at most np*B
+        // rounds are needed; the cap makes that explicit.
+        for (int round = 0; round <= np * B; round++) {
+            if (round) {
+#pragma unroll
+                for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+                    if (j < np) fill[j] = kv_pow_fill<P, SL>(slots + bidx[j] * L.B, L.B, t);
+            }
+            int best = -1, bf = B, second = B;
+#pragma unroll
+            for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+                if (j < np) {
+                    if (fill[j] < bf) {
+                        second = bf;
+                        bf = fill[j];
+                        best = j;
+                    } else if (fill[j] < second) {
+                        second = fill[j];
+                    }
+                }
+            }
+            if (best < 0) return false;  // every probe bucket is full
+            uint64_t bb = bidx[0];
+#pragma unroll
+            for (int j = 1; j < (int)kMaxShortcutProbes; j++) bb = (j == best) ? bidx[j] : bb;
+            if (kv_pow_claim<P, SL>(slots + bb * L.B, L.B, (uint32_t)second + 1u, sk, val, t)) return true;
+        }
+        return false;
+    }
+    int lvl = min_fill;
+    while (lvl < B) {
+        int next = B;
+#pragma unroll
+        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+            if (j < np) {
+                if (fill[j] == lvl && kv_pow_claim<P, SL>(slots + bidx[j] * L.B, L.B, L.B, sk, val, t)) return true;
+                if (fill[j] > lvl && fill[j] < next) next = fill[j];
+            }
+        }
+        lvl = next;
+    }
+    return false;
+}
+
+template <int P, class SL>
+__device__ bool kv_pow_query(const LevelDev &L, uint64_t key, const Tile<P> &t, uint64_t &val, bool &owner) {
+    using S = typename SL::S;
+    Probe pr;
+    pr.begin(L, key, SL::tag(key));
+    const uint64_t sk = SL::stored(key);
+    const S *slots = (const S *)L.slots;
+    for (uint32_t j = 0;; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) return false;
+        const S *bk = slots + fastmod(p, L.nb) * L.B;
+        for (uint32_t s0 = 0; s0 < L.B; s0 += P) {
+            const uint32_t i = s0 + t.rank;
+            const bool ok = i < L.B;
+            const S s = ok ? SL::load(bk + i) : SL::none();
+            t.acc[0] += (L.B - s0 < P) ? L.B - s0 : P;
+            const uint64_t mm = t.bits(ok && SL::key(s) == sk);
+            if (mm) {
+                const uint32_t l = (uint32_t)ctz64(mm);
+                owner = (t.rank == l);
+                if (owner) val = SL::val(s);
+                return true;
+            }
+        }
+    }
+}
+
+template <int P, class SL>
+__device__ bool kv_pow_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
+    using S = typename SL::S;
+    Probe pr;
+    pr.begin(L, key, SL::tag(key));
+    const uint64_t sk = SL::stored(key);
+    S *slots = (S *)L.slots;
+    for (uint32_t j = 0;; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) return false;
+        S *bk = slots + fastmod(p, L.nb) * L.B;
+        for (uint32_t s0 = 0; s0 < L.B; s0 += P) {
+            const uint32_t i = s0 + t.rank;
+            const bool ok = i < L.B;
+            const S s = ok ? SL::load(bk + i) : SL::none();
+            t.acc[0] += (L.B - s0 < P) ? L.B - s0 : P;
+            uint64_t mm = t.bits(ok && SL::key(s) == sk);
+            while (mm) {
+                const uint32_t l = (uint32_t)ctz64(mm);
+                bool success = false;
+                if (t.rank == l) success = SL::bury(bk + i, s, sk);
+                if (t.any(success)) {
+                    t.acc[2]++;
+                    return true;
+                }
+                mm &= mm - 1ull;
+            }
+        }
+    }
+}
+
+__device__ inline bool pow_scheme(const LevelDev &L) {
+    return L.scheme == PG_SCHEME_POWER_OF_N || L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+}
+
 // bucket_insert::insert (bucket_insert.cuh:127-203) / single_slot_insert::
 // insert (single_slot_insert.cuh:125-166): lowest empty slot in probe order;
 // key_val_pair::atomic_swap = CAS key 0 -> key, then store the value
@@ -188,12 +423,13 @@ __device__ bool kv_reg_remove(const LevelDev &L, uint64_t key, const Tile<P> &t)
 template <int P, class SL = SlotKV>
 __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
     using S = typename SL::S;
+    if (pow_scheme(L)) return kv_pow_insert<P, SL>(L, key, val, t);
     if constexpr (P <= 16) {
         if (L.B == 4 * P) return kv_reg_insert<P, 4, SL>(L, key, val, t);
         if (L.B == 2 * P) return kv_reg_insert<P, 2, SL>(L, key, val, t);
     }
     Probe pr;
-    pr.begin(L, key, key);
+    pr.begin(L, key, SL::tag(key));
     pr.max_probes = pr.useful_probes(L.nb.d);
     const uint64_t sk = SL::stored(key);
     S *slots = (S *)L.slots;
@@ -214,10 +450,7 @@ __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, c
             while (cm) {
                 const uint32_t l = (uint32_t)ctz64(cm);
                 bool success = false;
-                if (t.rank == l) {
-                    success = atomicCAS(SL::key_ptr(slots + si), kKvEmpty, sk) == kKvEmpty;
-                    if (success) SL::store_val(slots + si, val);
-                }
+                if (t.rank == l) success = SL::claim(slots + si, s, sk, val);
                 if (t.any(success)) {
                     t.acc[2]++;
                     return true;
@@ -240,10 +473,7 @@ __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, c
             while (cm) {
                 const uint32_t l = (uint32_t)ctz64(cm);
                 bool success = false;
-                if (t.rank == l) {
-                    success = atomicCAS(SL::key_ptr(slots + base + i), kKvEmpty, sk) == kKvEmpty;
-                    if (success) SL::store_val(slots + base + i, val);
-                }
+                if (t.rank == l) success = SL::claim(slots + base + i, s, sk, val);
                 if (t.any(success)) {
                     t.acc[2]++;
                     return true;
@@ -260,12 +490,13 @@ template <int P, class SL = SlotKV>
 __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t, uint64_t &val,
                                bool &owner) {
     using S = typename SL::S;
+    if (pow_scheme(L)) return kv_pow_query<P, SL>(L, key, t, val, owner);
     if constexpr (P <= 16) {
         if (L.B == 4 * P) return kv_reg_query<P, 4, SL>(L, key, t, val, owner);
         if (L.B == 2 * P) return kv_reg_query<P, 2, SL>(L, key, t, val, owner);
     }
     Probe pr;
-    pr.begin(L, key, key);
+    pr.begin(L, key, SL::tag(key));
     pr.max_probes = pr.useful_probes(L.nb.d);
     const uint64_t sk = SL::stored(key);
     const S *slots = (const S *)L.slots;
@@ -320,12 +551,13 @@ __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t
 template <int P, class SL = SlotKV>
 __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
     using S = typename SL::S;
+    if (pow_scheme(L)) return kv_pow_remove<P, SL>(L, key, t);
     if constexpr (P <= 16) {
         if (L.B == 4 * P) return kv_reg_remove<P, 4, SL>(L, key, t);
         if (L.B == 2 * P) return kv_reg_remove<P, 2, SL>(L, key, t);
     }
     Probe pr;
-    pr.begin(L, key, key);
+    pr.begin(L, key, SL::tag(key));
     pr.max_probes = pr.useful_probes(L.nb.d);
     const uint64_t sk = SL::stored(key);
     S *slots = (S *)L.slots;
@@ -351,8 +583,7 @@ __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &
                 const uint32_t l = (uint32_t)ctz64(dec);
                 if (!((mm >> l) & 1ull)) return false;
                 bool success = false;
-                if (t.rank == l)
-                    success = atomicCAS(SL::key_ptr(slots + si), sk, kKvTomb) == sk;
+                if (t.rank == l) success = SL::bury(slots + si, s, sk);
                 if (t.any(success)) {
                     t.acc[2]++;
                     return true;
@@ -376,8 +607,7 @@ __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &
             while (mm) {
                 const uint32_t l = (uint32_t)ctz64(mm);
                 bool success = false;
-                if (t.rank == l)
-                    success = atomicCAS(SL::key_ptr(slots + base + i), sk, kKvTomb) == sk;
+                if (t.rank == l) success = SL::bury(slots + base + i, s, sk);
                 if (t.any(success)) {
                     t.acc[2]++;
                     return true;

@@ -29,7 +29,7 @@ static int cpu_checks() {
     size_in_num_slots<1> one(1 << 12);
     bool threw = false;
     try {
-        tcf::generate_on_device(&one, 42);
+        del_tcf::generate_on_device(&one, 42);
     } catch (const std::runtime_error &) {
         threw = true;
     }
@@ -37,7 +37,7 @@ static int cpu_checks() {
     size_in_num_slots<3> three(1 << 12, 64, 64);
     threw = false;
     try {
-        tcf::generate_on_device(&three, 42);
+        del_tcf::generate_on_device(&three, 42);
     } catch (const std::runtime_error &) {
         threw = true;
     }
@@ -45,11 +45,18 @@ static int cpu_checks() {
     size_in_num_slots<2> two(1000, 10);
     CHECK(two.total() == 1010);
     pg_desc d{};
-    tcf::describe(d, 0, &two);
+    del_tcf::describe(d, 0, &two);
     CHECK(d.level[0].nslots == 1000 && d.level[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT);
     CHECK(d.level[0].prober == PG_PROBE_XOR_P2 && d.level[0].tile == 4 && d.level[0].bucket_size == 16);
     CHECK(d.level[1].nslots == 10 && d.level[1].scheme == PG_SCHEME_LINEAR_BUCKETS && d.level[1].max_probes == 4000);
-    static_assert(tcf::levels == 2 && hash_table_64::levels == 1, "levels");
+    static_assert(del_tcf::levels == 2 && hash_table_64::levels == 1, "levels");
+    // the reference's flat TCF (data_structs/tcf.cuh:29-30)
+    static_assert(tcf::levels == 2 && tcf::rep == PG_REP_KV16 && tiny_static_table_4::rep == PG_REP_KV16, "flat tcf");
+    size_in_num_slots<2> two_f(1000, 10);
+    tcf::describe(d, 0, &two_f);
+    CHECK(d.level[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS && d.level[0].prober == PG_PROBE_DOUBLE &&
+          d.level[0].max_probes == 2 && d.level[0].tile == 4 && d.level[0].bucket_size == 16);
+    CHECK(d.level[1].scheme == PG_SCHEME_BUCKET && d.level[1].bucket_size == 4 && d.level[1].max_probes == 20);
     static_assert(small_tcf::rep == PG_REP_FP8_V8 && tcf_10_6::rep == PG_REP_FP10_V6, "packed formats");
     static_assert(tcqf_twelve::rep == PG_REP_FP12 && tcqf_twelve_16::rep == PG_REP_FP12, "12-bit bucket");
     size_in_num_slots<1> one_12(4096);
@@ -70,6 +77,11 @@ static int cpu_checks() {
 // fail to insert are allowed up to 1 in 10 000 (an overfull probe path, e.g.
 // double hashing whose step shares factors with a power-of-two bucket count
 // cycles through few buckets); every stored key must then be found.
+// tests/src/test_components.cu:54: u64 -> u64, tile 8, 8-slot buckets,
+// power_of_n_insert_scheme with three double-hashing probes.
+using p2_table = static_table<uint64_t, uint64_t, key_val_pair, 8, 8, power_of_n_insert_scheme, 3, doubleHasher,
+                              murmurHasher>;
+
 template <typename Table>
 static int roundtrip(const char *name, uint64_t nslots, uint64_t backing, uint64_t n) {
     size_in_num_slots<2> sz2(nslots, backing);
@@ -122,7 +134,7 @@ static int roundtrip(const char *name, uint64_t nslots, uint64_t backing, uint64
     } else {
         const uint64_t slot_bytes = Table::rep == PG_REP_KV64 ? 16
                                     : Table::rep == PG_REP_KEY64  ? 8
-                                    : Table::rep == PG_REP_FP16_V16 ? 4 : 2;
+                                    : (Table::rep == PG_REP_FP16_V16 || Table::rep == PG_REP_KV16) ? 4 : 2;
         CHECK(t->host_bytes_in_use() == t->get_num_slots() * slot_bytes);
     }
     if (Table::rep == PG_REP_FP8_V8 || Table::rep == PG_REP_FP12) {
@@ -224,7 +236,7 @@ static int batch_checks() {
 static int server_checks() {
     size_in_num_slots<2> s2(1 << 18, 1 << 12);
     size_in_num_slots<1> s1(1 << 18);
-    tcf *f = tcf::generate_on_device(&s2, 42);
+    del_tcf *f = del_tcf::generate_on_device(&s2, 42);
     hash_table_64 *kv = hash_table_64::generate_on_device(&s1, 42);
     {
         request_server srv = request_server::over(1 << 24, 4, f, kv);
@@ -265,7 +277,7 @@ static int server_checks() {
         for (void *p : {(void *)keys, (void *)vals, (void *)out, (void *)fvals, (void *)fout, (void *)ff, (void *)kf})
             CHECK(hipFree(p) == hipSuccess);
     }
-    tcf::free_on_device(f);
+    del_tcf::free_on_device(f);
     hash_table_64::free_on_device(kv);
     std::printf("request_server ok\n");
     return 0;
@@ -312,7 +324,9 @@ static int three_level_checks() {
 }
 
 static int gpu_checks() {
-    if (roundtrip<tcf>("tcf", 1 << 20, 1 << 14, 900000)) return 1;
+    if (roundtrip<del_tcf>("del_tcf", 1 << 20, 1 << 14, 900000)) return 1;
+    if (roundtrip<tcf>("flat tcf", 1 << 20, 1 << 14, 900000)) return 1;
+    if (roundtrip<p2_table>("p2_table", 1 << 20, 0, 900000)) return 1;
     if (roundtrip<fast_tcf>("fast_tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<hash_table_64>("hash_table_64", 1 << 20, 0, 900000)) return 1;
     if (roundtrip<small_tcf>("small_tcf", 1 << 20, 1 << 14, 900000)) return 1;

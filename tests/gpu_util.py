@@ -93,5 +93,6 @@ class Ops:
 
     def level(self, lvl):
         dt = {pg.REP_KV64: np.uint64, pg.REP_FP16_V16: np.uint32, pg.REP_FP8_V8: np.uint16,
-              pg.REP_FP10_V6: np.uint16, pg.REP_FP12: np.uint16, pg.REP_KEY64: np.uint64}[self.t.desc.rep]
+              pg.REP_FP10_V6: np.uint16, pg.REP_FP12: np.uint16, pg.REP_KEY64: np.uint64,
+              pg.REP_KV16: np.uint32}[self.t.desc.rep]
         return np.frombuffer(self.t.level_bytes(lvl), dtype=dt)
