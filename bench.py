@@ -46,11 +46,12 @@ def parse():
                          "FETCH_SIZE run, WRITE_SIZE run, and optionally the TCC counter run) and exit")
     ap.add_argument("--summarize-kernel-stats", nargs=2, metavar=("STATS_DIR", "NAME"),
                     help="write profiles/NAME_kernel_stats.md from a rocprofv3 --stats csv directory "
-                         "(NAME: kv_2p30, batched_4096, serve_64_tables) and exit")
+                         "(NAME: kv_2p30, flat_tcf_2p28, batched_4096, serve_64_tables) and exit")
     ap.add_argument("--requests", type=int, default=400, help="serve: requests per experiment")
-    ap.add_argument("--workload", choices=["tcf", "batched", "kv", "serve"], default="tcf",
+    ap.add_argument("--workload", choices=["tcf", "batched", "kv", "serve", "flat"], default="tcf",
                     help="tcf: the headline 2^28-slot filter; batched: 4096 uneven filters in one graph; "
-                         "kv: 2^30-slot double-hashing key-value table")
+                         "kv: 2^30-slot double-hashing key-value table; flat: the flat TCF "
+                         "(16/16 key-value slots, 2^28 + 1 %% slots)")
     ap.add_argument("--tables", type=int, default=4096, help="batched: number of filters")
     ap.add_argument("--kv-tile", type=int, default=4, help="kv: tile width (4 is hash_table_64's)")
     return ap.parse_args()
@@ -137,6 +138,78 @@ def run_kv(args):
         "hit_queries_found": hits1, "hit_queries": nq // 2, "hit_queries_of_inserted_keys": hits_expected,
         "hit_values_exact": ok_vals,
         "absent_found": fp1, "requery_hits": hits2, "requery_hits_expected": live_hits,
+        "ms": {k: round(v, 3) for k, v in res.items()},
+    }
+    print(json.dumps(out_line), flush=True)
+
+
+def run_flat(args):
+    """The flat TCF (16/16 key-value slots, 16-slot buckets, two double-hash
+    choices with the 75 % shortcut, tile 4, a 1 % backing level of 4-slot
+    buckets with 20 probes) at 2^28 primary slots: fill to 0.9 of the primary
+    in batches of 2^24 keys, query 2^26 inserted and 2^26 absent keys, remove
+    10 % of the keys, re-query."""
+    import torch
+    import poggers_amd as pg
+
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    m = 1 << args.log2_slots
+    n = int(0.9 * m)
+    nq = 1 << (args.log2_slots - 2)
+    table = pg.Table(pg.flat_tcf_desc(m, m // 100))
+    keys = torch.empty(n, dtype=torch.int64, device=dev)
+    pg.gen_keys(98, 0, n, keys)
+    vals = ((keys >> 20) & 0xFFFF).to(torch.int16)
+    absent = torch.empty(nq // 2, dtype=torch.int64, device=dev)
+    pg.gen_keys(98, 1 << 40, absent.numel(), absent)
+    q = torch.cat([keys[: nq // 2], absent])
+    found = torch.empty(nq, dtype=torch.uint8, device=dev)
+    out = torch.empty(nq, dtype=torch.int16, device=dev)
+    fails = torch.zeros(1, dtype=torch.int64, device=dev)
+    status = torch.empty(n, dtype=torch.uint8, device=dev)
+    dels = keys[::10].contiguous()
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    batch = 1 << 24
+
+    def timed(fn):
+        e0, e1 = ev(), ev()
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    res = {"insert": 0.0, "query": 0.0, "delete": 0.0, "requery": 0.0}
+    for step in range(args.warmup + args.steps):
+        table.clear()
+        fails.zero_()
+        ti = timed(lambda: [table.bulk_insert(keys[a:a + batch], vals[a:a + batch], status[a:a + batch], fails)
+                            for a in range(0, n, batch)])
+        tq = timed(lambda: table.bulk_query(q, out, found))
+        hits = int(found[: nq // 2].sum().item())
+        fp = int(found[nq // 2:].sum().item())
+        td = timed(lambda: table.bulk_remove(dels))
+        tr = timed(lambda: table.bulk_query(q, out, found))
+        if step >= args.warmup:
+            for k, v in zip(res, (ti, tq, td, tr)):
+                res[k] += v / args.steps
+    inserted = int((status > 0).sum().item())
+    in_backing = int((status == 2).sum().item())
+    out_line = {
+        "metric": "flat TCF: insert/query/delete Mops/s at 0.9 load",
+        "value": round((n + 2 * nq + dels.numel()) / (sum(res.values()) / 1e3) / 1e6, 1),
+        "unit": "Mops/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(sum(res.values()), 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32", "data": "synthetic (splitmix64 keys)",
+        "config": {"workload": f"flat TCF 2^{args.log2_slots} + 1 % slots x 4 B (16/16 key_val_pair), "
+                               f"B 16, tile 4, double hashing, shortcut; {n} inserts, {nq} queries "
+                               f"(50 % inserted, 50 % absent), delete {dels.numel()}, re-query"},
+        "insert_mops": round(n / res["insert"] / 1e3, 1), "query_mops": round(nq / res["query"] / 1e3, 1),
+        "delete_mops": round(dels.numel() / res["delete"] / 1e3, 1),
+        "requery_mops": round(nq / res["requery"] / 1e3, 1),
+        "insert_failures": int(fails.item()), "inserted": inserted, "in_backing": in_backing,
+        "hit_queries_found": hits, "hit_queries": nq // 2, "fp_rate": fp / (nq // 2),
         "ms": {k: round(v, 3) for k, v in res.items()},
     }
     print(json.dumps(out_line), flush=True)
@@ -617,6 +690,9 @@ WORKLOAD_PROFILES = {
                      "after the top kernels)",
                      "rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_batch -o batch -- "
                      "python3 bench.py --workload batched --steps 3 --warmup 1 > gpurun_out/prof_batch.log"),
+    "flat_tcf_2p28": ("flat TCF 2^28 + 1 % slots (16/16 key-value slots, B 16, tile 4), 0.9 load",
+                      "rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_flat -o flat -- "
+                      "python3 bench.py --workload flat --steps 2 --warmup 1"),
     "serve_64_tables": ("serving loop over 64 resident tables against per-request plain launches",
                         "rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_serve -o serve -- "
                         "python3 bench.py --workload serve --requests 200"),
@@ -755,6 +831,8 @@ def main():
     if args.workload == "batched":
         run_batched(args)
         return
+    if args.workload == "flat":
+        return run_flat(args)
     if args.workload == "kv":
         run_kv(args)
         return

@@ -25,6 +25,8 @@
 //     the reference's probe-by-probe order.
 #pragma once
 
+#include <type_traits>
+
 #include "pg_internal.hpp"
 #include "pg_tile.hpp"
 
@@ -412,6 +414,246 @@ __device__ bool kv_pow_remove(const LevelDev &L, uint64_t key, const Tile<P> &t)
     }
 }
 
+// The flat schemes over 4-byte slots with B % 4 == 0 and B <= 4P (the flat
+// TCF's 16-slot buckets at tile 4): the whole bucket is one row, lane r
+// holding slots 4r..4r+3 from one 16 B load.  A row read once serves the fill
+// and the claim: slots only fill up, so a row can only show more empties
+// than the table holds, and trying them lowest first with a CAS each claims
+// the lowest slot that is still empty.
+template <int V>
+__device__ inline uint32_t row_word(const uint32_t (&w)[V], int v) {
+    uint32_t x = w[0];
+#pragma unroll
+    for (int i = 1; i < V; i++) x = (v == i) ? w[i] : x;
+    return x;
+}
+
+template <int P>
+struct Row16 {
+    uint32_t w[4];
+    bool ok;
+    __device__ inline void load(const uint32_t *bk, uint32_t B, const Tile<P> &t) {
+        ok = 4u * t.rank < B;
+        if (ok) {
+            load_words<4>(bk + 4 * t.rank, w);
+        } else {
+            w[0] = w[1] = w[2] = w[3] = 0u;
+        }
+        t.acc[0] += B;
+    }
+    __device__ inline int fill(const Tile<P> &t) const {
+        int f = 0;
+#pragma unroll
+        for (int v = 0; v < 4; v++) f += t.popc(ok && (w[v] & 0xFFFFu) != 0u);
+        return f;
+    }
+    // The lowest slot below `lim` seen empty.
+    __device__ inline bool claim(uint32_t *bk, uint32_t lim, uint32_t word, const Tile<P> &t) const {
+        uint32_t cand = 0;
+        if (ok) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if ((w[v] & 0xFFFFu) == 0u && 4u * t.rank + (uint32_t)v < lim) cand |= 1u << v;
+        }
+        uint64_t lanes = t.bits(cand != 0);
+        while (lanes) {
+            const uint32_t l = (uint32_t)ctz64(lanes);
+            bool success = false;
+            if (t.rank == l) {
+                const int v = __builtin_ctz(cand);
+                const uint32_t cur = row_word<4>(w, v);
+                success = atomicCAS(bk + 4 * t.rank + v, cur, word) == cur;
+                cand &= cand - 1u;
+            }
+            if (t.any(success)) {
+                t.acc[2]++;
+                return true;
+            }
+            lanes = t.bits(cand != 0);
+        }
+        return false;
+    }
+    // The lowest slot holding key16 `sk`; its lane receives the value.
+    __device__ inline bool find(uint32_t sk, const Tile<P> &t, uint64_t &val, bool &owner) const {
+        uint32_t m = 0;
+        if (ok) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if ((w[v] & 0xFFFFu) == sk) m |= 1u << v;
+        }
+        const uint64_t lanes = t.bits(m != 0);
+        if (!lanes) return false;
+        const uint32_t l = (uint32_t)ctz64(lanes);
+        owner = (t.rank == l);
+        if (owner) val = row_word<4>(w, __builtin_ctz(m)) >> 16;
+        return true;
+    }
+    // Tombstone the lowest slot holding `sk`, keeping its value bits.
+    __device__ inline bool bury(uint32_t *bk, uint32_t sk, const Tile<P> &t) const {
+        uint32_t m = 0;
+        if (ok) {
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                if ((w[v] & 0xFFFFu) == sk) m |= 1u << v;
+        }
+        uint64_t lanes = t.bits(m != 0);
+        while (lanes) {
+            const uint32_t l = (uint32_t)ctz64(lanes);
+            bool success = false;
+            if (t.rank == l) {
+                const int v = __builtin_ctz(m);
+                const uint32_t cur = row_word<4>(w, v);
+                success = atomicCAS(bk + 4 * t.rank + v, cur, (cur & 0xFFFF0000u) | 0xFFFFu) == cur;
+                m &= m - 1u;
+            }
+            if (t.any(success)) {
+                t.acc[2]++;
+                return true;
+            }
+            lanes = t.bits(m != 0);
+        }
+        return false;
+    }
+};
+
+// kv_pow_insert over rows: the same decisions; the plain scheme needs every
+// probe bucket's fill, so their loads are issued together; the shortcut
+// loads the next probe bucket only when the ones before it are at or above
+// 3B/4.
+template <int P>
+__device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
+    const uint32_t sk = fp16_of(key);
+    Probe pr;
+    pr.begin(L, key, sk);
+    const bool shortcut = L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+    const uint32_t word = ((uint32_t)val << 16) | sk;
+    uint32_t *slots = (uint32_t *)L.slots;
+    const int B = (int)L.B;
+    const uint32_t lim1 = t.concurrent ? (3u * L.B + 3u) / 4u : L.B;
+    Row16<P> row[kMaxShortcutProbes];
+    uint64_t bidx[kMaxShortcutProbes];
+    int fill[kMaxShortcutProbes];
+    int np = 0, min_fill = B;
+    if (!shortcut) {
+#pragma unroll
+        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+            const uint64_t p = pr.at(j);
+            if (p == kEnd) break;
+            bidx[j] = fastmod(p, L.nb);
+            np = j + 1;
+        }
+#pragma unroll
+        for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+            if (j < np) row[j].load(slots + bidx[j] * L.B, L.B, t);
+#pragma unroll
+        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+            if (j < np) {
+                fill[j] = row[j].fill(t);
+                if (fill[j] < min_fill) min_fill = fill[j];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+            const uint64_t p = pr.at(j);
+            if (p == kEnd) break;
+            const uint64_t b = fastmod(p, L.nb);
+            bidx[j] = b;
+            row[j].load(slots + b * L.B, L.B, t);
+            int f = row[j].fill(t);
+            if (4 * f < 3 * B) {
+                if (row[j].claim(slots + b * L.B, lim1, word, t)) return true;
+                f = (int)lim1;
+            }
+            fill[j] = f;
+            if (f < min_fill) min_fill = f;
+            np = j + 1;
+        }
+    }
+    if (t.concurrent) {
+        // the bounded fill-ordered pass of kv_pow_insert, over rows
+        for (int round = 0; round <= np * B; round++) {
+            if (round) {
+#pragma unroll
+                for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+                    if (j < np) row[j].load(slots + bidx[j] * L.B, L.B, t);
+#pragma unroll
+                for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+                    if (j < np) fill[j] = row[j].fill(t);
+            }
+            int best = -1, bf = B, second = B;
+#pragma unroll
+            for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+                if (j < np) {
+                    if (fill[j] < bf) {
+                        second = bf;
+                        bf = fill[j];
+                        best = j;
+                    } else if (fill[j] < second) {
+                        second = fill[j];
+                    }
+                }
+            }
+            if (best < 0) return false;
+            bool done = false;
+#pragma unroll
+            for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+                if (j == best) done = row[j].claim(slots + bidx[j] * L.B, (uint32_t)second + 1u, word, t);
+            if (done) return true;
+        }
+        return false;
+    }
+    int lvl = min_fill;
+    while (lvl < B) {
+        int next = B;
+#pragma unroll
+        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+            if (j < np) {
+                if (fill[j] == lvl && row[j].claim(slots + bidx[j] * L.B, L.B, word, t)) return true;
+                if (fill[j] > lvl && fill[j] < next) next = fill[j];
+            }
+        }
+        lvl = next;
+    }
+    return false;
+}
+
+template <int P>
+__device__ bool kv16_pow_query(const LevelDev &L, uint64_t key, const Tile<P> &t, uint64_t &val, bool &owner) {
+    const uint32_t sk = fp16_of(key);
+    Probe pr;
+    pr.begin(L, key, sk);
+    const uint32_t *slots = (const uint32_t *)L.slots;
+    for (uint32_t j = 0;; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) return false;
+        Row16<P> row;
+        row.load(slots + fastmod(p, L.nb) * L.B, L.B, t);
+        if (row.find(sk, t, val, owner)) return true;
+    }
+}
+
+template <int P>
+__device__ bool kv16_pow_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
+    const uint32_t sk = fp16_of(key);
+    Probe pr;
+    pr.begin(L, key, sk);
+    uint32_t *slots = (uint32_t *)L.slots;
+    for (uint32_t j = 0;; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) return false;
+        uint32_t *bk = slots + fastmod(p, L.nb) * L.B;
+        Row16<P> row;
+        row.load(bk, L.B, t);
+        if (row.bury(bk, sk, t)) return true;
+    }
+}
+
+template <int P, class SL>
+__device__ inline bool row16_shape(const LevelDev &L) {
+    return std::is_same<SL, SlotKV16>::value && (L.B & 3u) == 0 && L.B <= 4u * P;
+}
+
 __device__ inline bool pow_scheme(const LevelDev &L) {
     return L.scheme == PG_SCHEME_POWER_OF_N || L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
 }
@@ -423,7 +665,11 @@ __device__ inline bool pow_scheme(const LevelDev &L) {
 template <int P, class SL = SlotKV>
 __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
     using S = typename SL::S;
-    if (pow_scheme(L)) return kv_pow_insert<P, SL>(L, key, val, t);
+    if (pow_scheme(L)) {
+        if constexpr (std::is_same<SL, SlotKV16>::value)
+            if (row16_shape<P, SL>(L)) return kv16_pow_insert<P>(L, key, val, t);
+        return kv_pow_insert<P, SL>(L, key, val, t);
+    }
     if constexpr (P <= 16) {
         if (L.B == 4 * P) return kv_reg_insert<P, 4, SL>(L, key, val, t);
         if (L.B == 2 * P) return kv_reg_insert<P, 2, SL>(L, key, val, t);
@@ -490,7 +736,11 @@ template <int P, class SL = SlotKV>
 __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t, uint64_t &val,
                                bool &owner) {
     using S = typename SL::S;
-    if (pow_scheme(L)) return kv_pow_query<P, SL>(L, key, t, val, owner);
+    if (pow_scheme(L)) {
+        if constexpr (std::is_same<SL, SlotKV16>::value)
+            if (row16_shape<P, SL>(L)) return kv16_pow_query<P>(L, key, t, val, owner);
+        return kv_pow_query<P, SL>(L, key, t, val, owner);
+    }
     if constexpr (P <= 16) {
         if (L.B == 4 * P) return kv_reg_query<P, 4, SL>(L, key, t, val, owner);
         if (L.B == 2 * P) return kv_reg_query<P, 2, SL>(L, key, t, val, owner);
@@ -551,7 +801,11 @@ __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t
 template <int P, class SL = SlotKV>
 __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
     using S = typename SL::S;
-    if (pow_scheme(L)) return kv_pow_remove<P, SL>(L, key, t);
+    if (pow_scheme(L)) {
+        if constexpr (std::is_same<SL, SlotKV16>::value)
+            if (row16_shape<P, SL>(L)) return kv16_pow_remove<P>(L, key, t);
+        return kv_pow_remove<P, SL>(L, key, t);
+    }
     if constexpr (P <= 16) {
         if (L.B == 4 * P) return kv_reg_remove<P, 4, SL>(L, key, t);
         if (L.B == 2 * P) return kv_reg_remove<P, 2, SL>(L, key, t);

