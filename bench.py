@@ -54,6 +54,7 @@ def parse():
                          "(16/16 key-value slots, 2^28 + 1 %% slots)")
     ap.add_argument("--tables", type=int, default=4096, help="batched: number of filters")
     ap.add_argument("--kv-tile", type=int, default=4, help="kv: tile width (4 is hash_table_64's)")
+    ap.add_argument("--flat-tile", type=int, default=4, help="flat: tile width (4 is the reference's)")
     return ap.parse_args()
 
 
@@ -157,7 +158,7 @@ def run_flat(args):
     m = 1 << args.log2_slots
     n = int(0.9 * m)
     nq = 1 << (args.log2_slots - 2)
-    table = pg.Table(pg.flat_tcf_desc(m, m // 100))
+    table = pg.Table(pg.flat_tcf_desc(m, m // 100, tile=args.flat_tile))
     keys = torch.empty(n, dtype=torch.int64, device=dev)
     pg.gen_keys(98, 0, n, keys)
     vals = ((keys >> 20) & 0xFFFF).to(torch.int16)
@@ -203,7 +204,7 @@ def run_flat(args):
         "ms_per_step": round(sum(res.values()), 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32", "data": "synthetic (splitmix64 keys)",
         "config": {"workload": f"flat TCF 2^{args.log2_slots} + 1 % slots x 4 B (16/16 key_val_pair), "
-                               f"B 16, tile 4, double hashing, shortcut; {n} inserts, {nq} queries "
+                               f"B 16, tile {args.flat_tile}, double hashing, shortcut; {n} inserts, {nq} queries "
                                f"(50 % inserted, 50 % absent), delete {dels.numel()}, re-query"},
         "insert_mops": round(n / res["insert"] / 1e3, 1), "query_mops": round(nq / res["query"] / 1e3, 1),
         "delete_mops": round(dels.numel() / res["delete"] / 1e3, 1),

@@ -414,9 +414,9 @@ __device__ bool kv_pow_remove(const LevelDev &L, uint64_t key, const Tile<P> &t)
     }
 }
 
-// The flat schemes over 4-byte slots with B % 4 == 0 and B <= 4P (the flat
-// TCF's 16-slot buckets at tile 4): the whole bucket is one row, lane r
-// holding slots 4r..4r+3 from one 16 B load.  A row read once serves the fill
+// The flat schemes over 4-byte slots with B % V == 0 and B <= V*P, V = 4 or
+// 8 (the flat TCF's 16-slot buckets at tile 4 or 2): the whole bucket is one
+// row, lane r holding slots V*r..V*r+V-1 from V/4 16 B loads.  A row read once serves the fill
 // and the claim: slots only fill up, so a row can only show more empties
 // than the table holds, and trying them lowest first with a CAS each claims
 // the lowest slot that is still empty.
@@ -428,23 +428,26 @@ __device__ inline uint32_t row_word(const uint32_t (&w)[V], int v) {
     return x;
 }
 
-template <int P>
+template <int P, int V>
 struct Row16 {
-    uint32_t w[4];
+    static_assert(V == 4 || V == 8, "rows of 4 or 8 words per lane");
+    uint32_t w[V];
     bool ok;
     __device__ inline void load(const uint32_t *bk, uint32_t B, const Tile<P> &t) {
-        ok = 4u * t.rank < B;
-        if (ok) {
-            load_words<4>(bk + 4 * t.rank, w);
-        } else {
-            w[0] = w[1] = w[2] = w[3] = 0u;
+        ok = (uint32_t)V * t.rank < B;
+#pragma unroll
+        for (int c = 0; c < V / 4; c++) {
+            uint32_t q[4] = {0u, 0u, 0u, 0u};
+            if (ok) load_words<4>(bk + V * t.rank + 4 * c, q);
+#pragma unroll
+            for (int v = 0; v < 4; v++) w[4 * c + v] = q[v];
         }
         t.acc[0] += B;
     }
     __device__ inline int fill(const Tile<P> &t) const {
         int f = 0;
 #pragma unroll
-        for (int v = 0; v < 4; v++) f += t.popc(ok && (w[v] & 0xFFFFu) != 0u);
+        for (int v = 0; v < V; v++) f += t.popc(ok && (w[v] & 0xFFFFu) != 0u);
         return f;
     }
     // The lowest slot below `lim` seen empty.
@@ -452,8 +455,8 @@ struct Row16 {
         uint32_t cand = 0;
         if (ok) {
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                if ((w[v] & 0xFFFFu) == 0u && 4u * t.rank + (uint32_t)v < lim) cand |= 1u << v;
+            for (int v = 0; v < V; v++)
+                if ((w[v] & 0xFFFFu) == 0u && (uint32_t)V * t.rank + (uint32_t)v < lim) cand |= 1u << v;
         }
         uint64_t lanes = t.bits(cand != 0);
         while (lanes) {
@@ -461,8 +464,8 @@ struct Row16 {
             bool success = false;
             if (t.rank == l) {
                 const int v = __builtin_ctz(cand);
-                const uint32_t cur = row_word<4>(w, v);
-                success = atomicCAS(bk + 4 * t.rank + v, cur, word) == cur;
+                const uint32_t cur = row_word<V>(w, v);
+                success = atomicCAS(bk + V * t.rank + v, cur, word) == cur;
                 cand &= cand - 1u;
             }
             if (t.any(success)) {
@@ -478,14 +481,14 @@ struct Row16 {
         uint32_t m = 0;
         if (ok) {
 #pragma unroll
-            for (int v = 0; v < 4; v++)
+            for (int v = 0; v < V; v++)
                 if ((w[v] & 0xFFFFu) == sk) m |= 1u << v;
         }
         const uint64_t lanes = t.bits(m != 0);
         if (!lanes) return false;
         const uint32_t l = (uint32_t)ctz64(lanes);
         owner = (t.rank == l);
-        if (owner) val = row_word<4>(w, __builtin_ctz(m)) >> 16;
+        if (owner) val = row_word<V>(w, __builtin_ctz(m)) >> 16;
         return true;
     }
     // Tombstone the lowest slot holding `sk`, keeping its value bits.
@@ -493,7 +496,7 @@ struct Row16 {
         uint32_t m = 0;
         if (ok) {
 #pragma unroll
-            for (int v = 0; v < 4; v++)
+            for (int v = 0; v < V; v++)
                 if ((w[v] & 0xFFFFu) == sk) m |= 1u << v;
         }
         uint64_t lanes = t.bits(m != 0);
@@ -502,8 +505,8 @@ struct Row16 {
             bool success = false;
             if (t.rank == l) {
                 const int v = __builtin_ctz(m);
-                const uint32_t cur = row_word<4>(w, v);
-                success = atomicCAS(bk + 4 * t.rank + v, cur, (cur & 0xFFFF0000u) | 0xFFFFu) == cur;
+                const uint32_t cur = row_word<V>(w, v);
+                success = atomicCAS(bk + V * t.rank + v, cur, (cur & 0xFFFF0000u) | 0xFFFFu) == cur;
                 m &= m - 1u;
             }
             if (t.any(success)) {
@@ -519,8 +522,8 @@ struct Row16 {
 // kv_pow_insert over rows: the same decisions; the plain scheme needs every
 // probe bucket's fill, so their loads are issued together; the shortcut
 // loads the next probe bucket only when the ones before it are at or above
-// 3B/4.
-template <int P>
+// 3B/4.  NP rows are held (at least the level's max_probes).
+template <int P, int V, int NP>
 __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
     const uint32_t sk = fp16_of(key);
     Probe pr;
@@ -530,23 +533,23 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
     uint32_t *slots = (uint32_t *)L.slots;
     const int B = (int)L.B;
     const uint32_t lim1 = t.concurrent ? (3u * L.B + 3u) / 4u : L.B;
-    Row16<P> row[kMaxShortcutProbes];
-    uint64_t bidx[kMaxShortcutProbes];
-    int fill[kMaxShortcutProbes];
+    Row16<P, V> row[NP];
+    uint64_t bidx[NP];
+    int fill[NP];
     int np = 0, min_fill = B;
     if (!shortcut) {
 #pragma unroll
-        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+        for (int j = 0; j < (int)NP; j++) {
             const uint64_t p = pr.at(j);
             if (p == kEnd) break;
             bidx[j] = fastmod(p, L.nb);
             np = j + 1;
         }
 #pragma unroll
-        for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+        for (int j = 0; j < (int)NP; j++)
             if (j < np) row[j].load(slots + bidx[j] * L.B, L.B, t);
 #pragma unroll
-        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+        for (int j = 0; j < (int)NP; j++) {
             if (j < np) {
                 fill[j] = row[j].fill(t);
                 if (fill[j] < min_fill) min_fill = fill[j];
@@ -554,7 +557,7 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
         }
     } else {
 #pragma unroll
-        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+        for (int j = 0; j < (int)NP; j++) {
             const uint64_t p = pr.at(j);
             if (p == kEnd) break;
             const uint64_t b = fastmod(p, L.nb);
@@ -575,15 +578,15 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
         for (int round = 0; round <= np * B; round++) {
             if (round) {
 #pragma unroll
-                for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+                for (int j = 0; j < (int)NP; j++)
                     if (j < np) row[j].load(slots + bidx[j] * L.B, L.B, t);
 #pragma unroll
-                for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+                for (int j = 0; j < (int)NP; j++)
                     if (j < np) fill[j] = row[j].fill(t);
             }
             int best = -1, bf = B, second = B;
 #pragma unroll
-            for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+            for (int j = 0; j < (int)NP; j++) {
                 if (j < np) {
                     if (fill[j] < bf) {
                         second = bf;
@@ -597,7 +600,7 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
             if (best < 0) return false;
             bool done = false;
 #pragma unroll
-            for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+            for (int j = 0; j < (int)NP; j++)
                 if (j == best) done = row[j].claim(slots + bidx[j] * L.B, (uint32_t)second + 1u, word, t);
             if (done) return true;
         }
@@ -607,7 +610,7 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
     while (lvl < B) {
         int next = B;
 #pragma unroll
-        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+        for (int j = 0; j < (int)NP; j++) {
             if (j < np) {
                 if (fill[j] == lvl && row[j].claim(slots + bidx[j] * L.B, L.B, word, t)) return true;
                 if (fill[j] > lvl && fill[j] < next) next = fill[j];
@@ -618,7 +621,7 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
     return false;
 }
 
-template <int P>
+template <int P, int V>
 __device__ bool kv16_pow_query(const LevelDev &L, uint64_t key, const Tile<P> &t, uint64_t &val, bool &owner) {
     const uint32_t sk = fp16_of(key);
     Probe pr;
@@ -627,13 +630,13 @@ __device__ bool kv16_pow_query(const LevelDev &L, uint64_t key, const Tile<P> &t
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
         if (p == kEnd) return false;
-        Row16<P> row;
+        Row16<P, V> row;
         row.load(slots + fastmod(p, L.nb) * L.B, L.B, t);
         if (row.find(sk, t, val, owner)) return true;
     }
 }
 
-template <int P>
+template <int P, int V>
 __device__ bool kv16_pow_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
     const uint32_t sk = fp16_of(key);
     Probe pr;
@@ -643,15 +646,21 @@ __device__ bool kv16_pow_remove(const LevelDev &L, uint64_t key, const Tile<P> &
         const uint64_t p = pr.at(j);
         if (p == kEnd) return false;
         uint32_t *bk = slots + fastmod(p, L.nb) * L.B;
-        Row16<P> row;
+        Row16<P, V> row;
         row.load(bk, L.B, t);
         if (row.bury(bk, sk, t)) return true;
     }
 }
 
-template <int P, class SL>
-__device__ inline bool row16_shape(const LevelDev &L) {
-    return std::is_same<SL, SlotKV16>::value && (L.B & 3u) == 0 && L.B <= 4u * P;
+// Words per lane of the row shape of a level (0: no row shape).  Rows of
+// 8 words are offered to tiles of one or two lanes only: in a wider tile's
+// kernel their registers (up to four probe rows) would lower the occupancy
+// of the 4-word path, measured 12 % slower at tile 4.
+template <int P>
+__device__ inline int row16_words(const LevelDev &L) {
+    if ((L.B & 3u) == 0 && L.B <= 4u * P) return 4;
+    if (P <= 2 && (L.B & 7u) == 0 && L.B <= 8u * P) return 8;
+    return 0;
 }
 
 __device__ inline bool pow_scheme(const LevelDev &L) {
@@ -666,8 +675,14 @@ template <int P, class SL = SlotKV>
 __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
     using S = typename SL::S;
     if (pow_scheme(L)) {
-        if constexpr (std::is_same<SL, SlotKV16>::value)
-            if (row16_shape<P, SL>(L)) return kv16_pow_insert<P>(L, key, val, t);
+        if constexpr (std::is_same<SL, SlotKV16>::value) {
+            // 8-word rows hold two probes at most, so the kernel's largest
+            // row array stays at 16 words
+            const int rw = row16_words<P>(L);
+            if (rw == 4) return kv16_pow_insert<P, 4, kMaxShortcutProbes>(L, key, val, t);
+            if constexpr (P <= 2)
+                if (rw == 8 && L.max_probes <= 2) return kv16_pow_insert<P, 8, 2>(L, key, val, t);
+        }
         return kv_pow_insert<P, SL>(L, key, val, t);
     }
     if constexpr (P <= 16) {
@@ -737,8 +752,12 @@ __device__ bool kv_level_query(const LevelDev &L, uint64_t key, const Tile<P> &t
                                bool &owner) {
     using S = typename SL::S;
     if (pow_scheme(L)) {
-        if constexpr (std::is_same<SL, SlotKV16>::value)
-            if (row16_shape<P, SL>(L)) return kv16_pow_query<P>(L, key, t, val, owner);
+        if constexpr (std::is_same<SL, SlotKV16>::value) {
+            const int rw = row16_words<P>(L);
+            if (rw == 4) return kv16_pow_query<P, 4>(L, key, t, val, owner);
+            if constexpr (P <= 2)
+                if (rw == 8) return kv16_pow_query<P, 8>(L, key, t, val, owner);
+        }
         return kv_pow_query<P, SL>(L, key, t, val, owner);
     }
     if constexpr (P <= 16) {
@@ -802,8 +821,12 @@ template <int P, class SL = SlotKV>
 __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &t) {
     using S = typename SL::S;
     if (pow_scheme(L)) {
-        if constexpr (std::is_same<SL, SlotKV16>::value)
-            if (row16_shape<P, SL>(L)) return kv16_pow_remove<P>(L, key, t);
+        if constexpr (std::is_same<SL, SlotKV16>::value) {
+            const int rw = row16_words<P>(L);
+            if (rw == 4) return kv16_pow_remove<P, 4>(L, key, t);
+            if constexpr (P <= 2)
+                if (rw == 8) return kv16_pow_remove<P, 8>(L, key, t);
+        }
         return kv_pow_remove<P, SL>(L, key, t);
     }
     if constexpr (P <= 16) {

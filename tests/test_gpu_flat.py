@@ -46,10 +46,13 @@ def key_only_shortcut(nslots, tile=2):
 CASES = {
     "flat_tcf": (pg.flat_tcf_desc(1 << 12, 1 << 7), 42, 1.05),
     "flat_tcf_tile16": (pg.flat_tcf_desc(1 << 12, 1 << 7, tile=16), 42, 1.03),
+    "flat_tcf_tile2": (pg.flat_tcf_desc(1 << 12, 1 << 7, tile=2), 42, 1.04),
+    "flat_tcf_tile1": (pg.flat_tcf_desc(1 << 12, 1 << 7, tile=1), 42, 1.02),
     "p2_table": (p2_table(1 << 12), 1, 0.97),
     "p2_table_tile2": (p2_table(1 << 12, tile=2), 1, 0.97),
     "tier_two": (tier_two(1 << 12), 42, 0.95),
     "kv16_pow4": (kv16_pow(1 << 12), 42, 0.99),
+    "kv16_pow4_tile2_b16": (kv16_pow(1 << 12, tile=2, B=16), 42, 0.99),
     "key64_shortcut": (key_only_shortcut(1 << 12), 42, 0.98),
 }
 
@@ -209,11 +212,13 @@ def test_flat_shortcut_claims_follow_the_fill_rule_concurrent():
     assert bad_g == 0 and n_g > 0.1 * n, (bad_g, n_g)
 
 
-@pytest.mark.parametrize("name,nslots,load", [("p2_table", 1 << 20, 0.93), ("flat_tcf", 1 << 20, 0.97)])
+@pytest.mark.parametrize("name,nslots,load", [("p2_table", 1 << 20, 0.93), ("flat_tcf", 1 << 20, 0.97),
+                                             ("flat_tcf_tile2", 1 << 20, 0.97)])
 def test_concurrent_large_failures_near_key_order(name, nslots, load):
     """At 2^20 slots every key of a launch reads its fills before most
     claims land; the bounded claims keep failures near the key-order count."""
-    desc = p2_table(nslots) if name == "p2_table" else pg.flat_tcf_desc(nslots, nslots // 64)
+    tile = 2 if name.endswith("tile2") else 4
+    desc = p2_table(nslots) if name == "p2_table" else pg.flat_tcf_desc(nslots, nslots // 64, tile=tile)
     seed = 1 if name == "p2_table" else 42
     n = int(load * nslots)
     keys = oracle.gen_keys(125, 0, n)
