@@ -1,5 +1,5 @@
 """Empty and ragged inputs across the entry points the other GPU tests cover
-at ordinary sizes: zero-key calls on key-value and key-only tables, calls
+at ordinary sizes: zero-key calls on key-value, key-only and flat TCF tables, calls
 without optional outputs, batches whose CSR offsets give some tables (first,
 middle, last, or all) no keys, and served batches that mix zero-key requests
 with others."""
@@ -45,6 +45,32 @@ def test_zero_key_calls_and_missing_outputs(rep):
     f, _ = ops.query(keys)
     assert not f[:1000].any() and (f[1000:] > 0).tolist() == stored[1000:].tolist()
     assert g.get_fill() == int(stored[1000:].sum())
+
+
+def test_flat_tcf_zero_key_calls_and_missing_outputs():
+    """The flat TCF (16/16 key-value slots): zero-key calls, and calls
+    without status, counter, value or found outputs.  16-bit keys can
+    collide, so a remove may take a colliding key's slot: fill then follows
+    the removes that landed."""
+    g = pg.Table(pg.flat_tcf_desc(1 << 12, 1 << 7))
+    ops = Ops(g)
+    empty = np.zeros(0, dtype=np.uint64)
+    st, fails = ops.insert(empty)
+    assert len(st) == 0 and fails == 0
+    assert len(ops.query(empty)[0]) == 0 and len(ops.remove(empty)) == 0
+    keys = oracle.gen_keys(42, 0, 3000)
+    kd = dev_u64(keys)
+    g.bulk_insert(kd, (kd & 0x7FFF).to(torch.int16))   # no status, no fail counter
+    g.bulk_query(kd)                                    # no outputs at all
+    torch.cuda.synchronize()
+    assert g.get_fill() == 3000                         # 73 % of level 0: every key fits
+    f, v = ops.query(keys)
+    assert f.all() and (v == (keys & np.uint64(0x7FFF)).astype(np.uint16)).mean() > 0.998
+    g.bulk_remove(kd[:1000])                            # no found flags
+    torch.cuda.synchronize()
+    assert 1998 <= g.get_fill() <= 2002
+    f, _ = ops.query(keys)
+    assert f[1000:].mean() > 0.998 and f[:1000].mean() < 0.002
 
 
 @pytest.mark.parametrize("mode", [pg.CONCURRENT, pg.DETERMINISTIC])

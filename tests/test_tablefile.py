@@ -43,7 +43,10 @@ def test_save_load_roundtrip(tmp_path):
     import oracle
     from tests.gpu_util import Ops
     for desc in (pg.tcf_desc(1 << 16, 1 << 9, tile=4),
-                 pg.kv_desc(1 << 16, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20)):
+                 pg.kv_desc(1 << 16, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20),
+                 pg.flat_tcf_desc(1 << 16, 1 << 9),
+                 pg.TableDesc(pg.REP_KV64, [pg.Level(1 << 16, pg.SCHEME_POWER_OF_N, 8, 8, pg.PROBE_DOUBLE,
+                                                     pg.HASH_MURMUR64A, 3)])):
         t = pg.Table(desc, seed=99)
         ops = Ops(t)
         keys = oracle.gen_keys(5, 0, 50000)
