@@ -28,6 +28,11 @@
  *   KV insert     -- tests/src/test_components.cu:88-127 and :345-366 known
  *                    answers, linear-probing closed-form probe lengths,
  *                    brute force on tiny tables.
+ *   flat power-of-n (kv_pow_*) and the 16/16 key-value slot -- the
+ *                    tests/src/test_components.cu:88-200 p2_table checks;
+ *                    hand-derived slot words, kept tombstone values, fills
+ *                    that count tombstones, least-filled choice and tie
+ *                    order (tests/test_oracle_flat.py).
  *   multi-thread  -- with one thread, byte-identical layouts to the
  *                    sequential functions; with several, zero false
  *                    negatives, fill = stored keys, queries identical to
