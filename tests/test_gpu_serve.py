@@ -33,8 +33,17 @@ def host_u16(a, pinned=True):
 def build_tables():
     tabs = [pg.Table(pg.tcf_desc(1 << 19, tile=4)), pg.Table(pg.tcf_desc(1 << 18, tile=4)),
             pg.Table(pg.kv_desc(1 << 19, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20)),
-            pg.Table(pg.kv_desc(1 << 19, bucket_size=1, tile=4, prober=pg.PROBE_LINEAR))]
+            pg.Table(pg.kv_desc(1 << 19, bucket_size=1, tile=4, prober=pg.PROBE_LINEAR)),
+            pg.Table(pow_kv_desc(1 << 19))]
     return tabs
+
+
+def pow_kv_desc(nslots):
+    """key_val_pair<u64,u64> slots under power_of_n_insert_scheme: 8-slot
+    buckets, three double-hashing probes (tests/src/test_components.cu:54),
+    at the served tile width."""
+    return pg.TableDesc(pg.REP_KV64, [pg.Level(nslots, pg.SCHEME_POWER_OF_N, 8, 4, pg.PROBE_DOUBLE,
+                                                pg.HASH_MURMUR64A, 3)])
 
 
 def populate(tabs, base):
@@ -196,12 +205,13 @@ def test_in_place_requests_take_no_staging():
 
 
 def test_ordered_mode_applies_writes_in_submission_order():
-    """ORDER_TABLE: two tables each get a dependent chain, submitted at once
+    """ORDER_TABLE: two tables (bucket_insert and power-of-n key-value
+    tables) each get a dependent chain, submitted at once
     and interleaved: insert, query, remove half, query, re-insert that half
     with new values, query.  Every result is the one-by-one result; with
     the default ordering the same burst would land in one batch."""
-    tabs = [pg.Table(pg.kv_desc(1 << 16, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20))
-            for _ in range(2)]
+    tabs = [pg.Table(pg.kv_desc(1 << 16, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20)),
+            pg.Table(pow_kv_desc(1 << 16))]
     srv = pg.Server(tabs, budget_bytes=1 << 24, nstreams=4)
     srv.set_order(pg.ORDER_TABLE)
     n = 4000
