@@ -58,6 +58,17 @@ def test_create_rejects_bad_descriptors_without_gpu_work():
         # twelve_bucket::tag() returns nothing: no tag-driven probing for it
         pg.TableDesc(pg.REP_FP12, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16, prober=pg.PROBE_XOR_P2)]),
         pg.TableDesc(5, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16)]),
+        pg.TableDesc(7, [pg.Level(1024, pg.SCHEME_BUCKET, 16)]),
+        # the flat power-of-n schemes take key-value slots and at most 4 probes;
+        # the 16/16 key-value slot takes no packed-bucket scheme
+        pg.TableDesc(pg.REP_FP16_V16, [pg.Level(1024, pg.SCHEME_POWER_OF_N, 16, prober=pg.PROBE_DOUBLE,
+                                                max_probes=2)]),
+        pg.TableDesc(pg.REP_KV64, [pg.Level(1024, pg.SCHEME_POWER_OF_N, 8, prober=pg.PROBE_DOUBLE,
+                                            max_probes=5)]),
+        pg.TableDesc(pg.REP_KV16, [pg.Level(1024, pg.SCHEME_POWER_OF_N_SHORTCUT_SLOTS, 16,
+                                            prober=pg.PROBE_DOUBLE, max_probes=8)]),
+        pg.TableDesc(pg.REP_KV16, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16, prober=pg.PROBE_DOUBLE)]),
+        pg.TableDesc(pg.REP_KV16, [pg.Level(1024, pg.SCHEME_POWER_OF_N, 0, prober=pg.PROBE_DOUBLE)]),
         # SPLIT64 probes are 32-bit: no level of more than 2^32 buckets
         pg.TableDesc(pg.REP_FP16_V16, [pg.Level((1 << 36) + 16, pg.SCHEME_SHORTCUT, 16,
                                                 prober=pg.PROBE_SPLIT64, max_probes=2)]),
