@@ -519,11 +519,11 @@ struct Row16 {
     }
 };
 
-// kv_pow_insert over rows: the same decisions; the plain scheme needs every
-// probe bucket's fill, so their loads are issued together; the shortcut
-// loads the next probe bucket only when the ones before it are at or above
-// 3B/4.  NP rows are held (at least the level's max_probes).
-template <int P, int V, int NP>
+// kv_pow_insert over rows: the same decisions.  Only the fills are kept
+// across probes: the shortcut claims from the row it has just read, and the
+// fill-ordered pass (rare under the shortcut) reads the chosen bucket's row
+// again before its claim, so one row of registers serves any probe count.
+template <int P, int V>
 __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
     const uint32_t sk = fp16_of(key);
     Probe pr;
@@ -533,60 +533,42 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
     uint32_t *slots = (uint32_t *)L.slots;
     const int B = (int)L.B;
     const uint32_t lim1 = t.concurrent ? (3u * L.B + 3u) / 4u : L.B;
-    Row16<P, V> row[NP];
-    uint64_t bidx[NP];
-    int fill[NP];
+    uint64_t bidx[kMaxShortcutProbes];
+    int fill[kMaxShortcutProbes];
     int np = 0, min_fill = B;
-    if (!shortcut) {
 #pragma unroll
-        for (int j = 0; j < (int)NP; j++) {
-            const uint64_t p = pr.at(j);
-            if (p == kEnd) break;
-            bidx[j] = fastmod(p, L.nb);
-            np = j + 1;
+    for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+        const uint64_t p = pr.at(j);
+        if (p == kEnd) break;
+        const uint64_t b = fastmod(p, L.nb);
+        bidx[j] = b;
+        Row16<P, V> row;
+        row.load(slots + b * L.B, L.B, t);
+        int f = row.fill(t);
+        if (shortcut && 4 * f < 3 * B) {
+            if (row.claim(slots + b * L.B, lim1, word, t)) return true;
+            f = (int)lim1;
         }
-#pragma unroll
-        for (int j = 0; j < (int)NP; j++)
-            if (j < np) row[j].load(slots + bidx[j] * L.B, L.B, t);
-#pragma unroll
-        for (int j = 0; j < (int)NP; j++) {
-            if (j < np) {
-                fill[j] = row[j].fill(t);
-                if (fill[j] < min_fill) min_fill = fill[j];
-            }
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < (int)NP; j++) {
-            const uint64_t p = pr.at(j);
-            if (p == kEnd) break;
-            const uint64_t b = fastmod(p, L.nb);
-            bidx[j] = b;
-            row[j].load(slots + b * L.B, L.B, t);
-            int f = row[j].fill(t);
-            if (4 * f < 3 * B) {
-                if (row[j].claim(slots + b * L.B, lim1, word, t)) return true;
-                f = (int)lim1;
-            }
-            fill[j] = f;
-            if (f < min_fill) min_fill = f;
-            np = j + 1;
-        }
+        fill[j] = f;
+        if (f < min_fill) min_fill = f;
+        np = j + 1;
     }
     if (t.concurrent) {
-        // the bounded fill-ordered pass of kv_pow_insert, over rows
+        // the bounded fill-ordered pass of kv_pow_insert
         for (int round = 0; round <= np * B; round++) {
             if (round) {
 #pragma unroll
-                for (int j = 0; j < (int)NP; j++)
-                    if (j < np) row[j].load(slots + bidx[j] * L.B, L.B, t);
-#pragma unroll
-                for (int j = 0; j < (int)NP; j++)
-                    if (j < np) fill[j] = row[j].fill(t);
+                for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+                    if (j < np) {
+                        Row16<P, V> row;
+                        row.load(slots + bidx[j] * L.B, L.B, t);
+                        fill[j] = row.fill(t);
+                    }
+                }
             }
             int best = -1, bf = B, second = B;
 #pragma unroll
-            for (int j = 0; j < (int)NP; j++) {
+            for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
                 if (j < np) {
                     if (fill[j] < bf) {
                         second = bf;
@@ -598,11 +580,12 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
                 }
             }
             if (best < 0) return false;
-            bool done = false;
+            uint64_t bb = bidx[0];
 #pragma unroll
-            for (int j = 0; j < (int)NP; j++)
-                if (j == best) done = row[j].claim(slots + bidx[j] * L.B, (uint32_t)second + 1u, word, t);
-            if (done) return true;
+            for (int j = 1; j < (int)kMaxShortcutProbes; j++) bb = (j == best) ? bidx[j] : bb;
+            Row16<P, V> row;
+            row.load(slots + bb * L.B, L.B, t);
+            if (row.claim(slots + bb * L.B, (uint32_t)second + 1u, word, t)) return true;
         }
         return false;
     }
@@ -610,9 +593,13 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
     while (lvl < B) {
         int next = B;
 #pragma unroll
-        for (int j = 0; j < (int)NP; j++) {
+        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
             if (j < np) {
-                if (fill[j] == lvl && row[j].claim(slots + bidx[j] * L.B, L.B, word, t)) return true;
+                if (fill[j] == lvl) {
+                    Row16<P, V> row;
+                    row.load(slots + bidx[j] * L.B, L.B, t);
+                    if (row.claim(slots + bidx[j] * L.B, L.B, word, t)) return true;
+                }
                 if (fill[j] > lvl && fill[j] < next) next = fill[j];
             }
         }
@@ -676,12 +663,10 @@ __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, c
     using S = typename SL::S;
     if (pow_scheme(L)) {
         if constexpr (std::is_same<SL, SlotKV16>::value) {
-            // 8-word rows hold two probes at most, so the kernel's largest
-            // row array stays at 16 words
             const int rw = row16_words<P>(L);
-            if (rw == 4) return kv16_pow_insert<P, 4, kMaxShortcutProbes>(L, key, val, t);
+            if (rw == 4) return kv16_pow_insert<P, 4>(L, key, val, t);
             if constexpr (P <= 2)
-                if (rw == 8 && L.max_probes <= 2) return kv16_pow_insert<P, 8, 2>(L, key, val, t);
+                if (rw == 8) return kv16_pow_insert<P, 8>(L, key, val, t);
         }
         return kv_pow_insert<P, SL>(L, key, val, t);
     }
