@@ -309,6 +309,13 @@ using tiny_static_table_4 = static_table<uint64_t, uint16_t, dynamic_container<k
                                          bucket_insert, 20, doubleHasher, murmurHasher>;
 using tcf = static_table<uint64_t, uint16_t, dynamic_container<key_val_pair, uint16_t>, 4, 16,
                          power_of_n_insert_shortcut_scheme, 2, doubleHasher, murmurHasher, true, tiny_static_table_4>;
+// The same composition at tile 2, the fastest measured shape at 2^28 slots
+// (each lane holds 8 slots of a bucket).
+using fast_flat_tcf_backing = static_table<uint64_t, uint16_t, dynamic_container<key_val_pair, uint16_t>, 2, 4,
+                                           bucket_insert, 20, doubleHasher, murmurHasher>;
+using fast_flat_tcf = static_table<uint64_t, uint16_t, dynamic_container<key_val_pair, uint16_t>, 2, 16,
+                                   power_of_n_insert_shortcut_scheme, 2, doubleHasher, murmurHasher, true,
+                                   fast_flat_tcf_backing>;
 
 // The TCF as the reference's delete tests compose it, del_TCF
 // (tests/src/delete_tests.cu:138-152): 16/16 grouped slots, P=4, B=16,

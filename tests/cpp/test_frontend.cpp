@@ -52,6 +52,7 @@ static int cpu_checks() {
     static_assert(del_tcf::levels == 2 && hash_table_64::levels == 1, "levels");
     // the reference's flat TCF (data_structs/tcf.cuh:29-30)
     static_assert(tcf::levels == 2 && tcf::rep == PG_REP_KV16 && tiny_static_table_4::rep == PG_REP_KV16, "flat tcf");
+    static_assert(fast_flat_tcf::levels == 2 && fast_flat_tcf::rep == PG_REP_KV16, "fast flat tcf");
     size_in_num_slots<2> two_f(1000, 10);
     tcf::describe(d, 0, &two_f);
     CHECK(d.level[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS && d.level[0].prober == PG_PROBE_DOUBLE &&
@@ -326,6 +327,7 @@ static int three_level_checks() {
 static int gpu_checks() {
     if (roundtrip<del_tcf>("del_tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<tcf>("flat tcf", 1 << 20, 1 << 14, 900000)) return 1;
+    if (roundtrip<fast_flat_tcf>("fast_flat_tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<p2_table>("p2_table", 1 << 20, 0, 900000)) return 1;
     if (roundtrip<fast_tcf>("fast_tcf", 1 << 20, 1 << 14, 900000)) return 1;
     if (roundtrip<hash_table_64>("hash_table_64", 1 << 20, 0, 900000)) return 1;

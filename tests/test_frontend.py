@@ -28,6 +28,6 @@ def test_frontend_host_checks():
 def test_frontend_bulk_ops_on_gpu():
     r = subprocess.run([_binary(), "gpu"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    for name in ("del_tcf ok", "flat tcf ok", "p2_table ok", "fast_tcf ok", "hash_table_64 ok", "small_tcf ok", "tcqf_twelve_16 ok",
+    for name in ("del_tcf ok", "flat tcf ok", "fast_flat_tcf ok", "p2_table ok", "fast_tcf ok", "hash_table_64 ok", "small_tcf ok", "tcqf_twelve_16 ok",
                  "cms_free_table ok", "tcf3 ok", "filter_batch ok", "request_server ok"):
         assert name in r.stdout
