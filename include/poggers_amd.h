@@ -53,7 +53,8 @@ enum pg_status {
 
 /* Slot representation.
  *   KV64      key_val_pair<uint64_t,uint64_t>: 16 B {key, val}
- *             (representations/key_val_pair.cuh:63-171)
+ *             (representations/key_val_pair.cuh:63-171).  Schemes BUCKET,
+ *             SINGLE_SLOT, POWER_OF_N and POWER_OF_N_SHORTCUT_SLOTS.
  *   FP16_V16  dynamic_container<bit_grouped_container<16,16>, uint16_t>:
  *             one uint32 word = val16 << 16 | fp16
  *             (representations/dynamic_container.cuh:23-141,
@@ -82,7 +83,7 @@ enum pg_status {
  *             The stored key is get_smallKey(key): 0 -> 1, ~0 -> ~0 - 1
  *             (dynamic_container.cuh:35-50), so every key is accepted and
  *             0/1 (and ~0/~0-1) share a slot; probes hash the key as
- *             given.  Schemes BUCKET and SINGLE_SLOT, as KV64; queries
+ *             given.  The schemes of KV64; queries
  *             report value 0 (uint16 buffers, the CMS table's Val).
  *   KV16      dynamic_container<key_val_pair<uint64_t, uint16_t>, uint16_t>
  *             (representations/dynamic_container.cuh:35-120 over
