@@ -98,8 +98,8 @@ enum pg_status {
  *   Value buffers hold the reference's Val type: uint64 (KV64), uint16
  *   (FP16_V16, FP12, KEY64, KV16), uint8 (FP8_V8 and FP10_V6, which keep the low
  *   8 / 6 bits).  The 8/8, 10/6 and 12-bit formats and grouped levels run the
- *   generic (runtime bucket size) kernels; batches and the server take KV64
- *   and FP16_V16 tables only.                                             */
+ *   generic (runtime bucket size) kernels; batches take FP16_V16 tables, the
+ *   server KV64, FP16_V16 and KV16 tables.                                             */
 enum pg_rep {
     PG_REP_KV64 = 0,
     PG_REP_FP16_V16 = 1,
@@ -336,7 +336,8 @@ int pg_batch_bytes(pg_batch b, uint64_t *out);
  * admitted group as one batch (one copy of the request table, one kernel
  * with a tile per key over all its requests, plus the staged requests'
  * copies).  Up to `nstreams` batches overlap on the device.  Every served table
- * must use the same tile width (at most 16).  Requests that touch the same
+ * must use the same tile width (at most 16) and hold KV64, FP16_V16 or KV16
+ * slots (PG_ERR_UNSUPPORTED otherwise).  Requests that touch the same
  * keys of one table from different in-flight batches are not ordered with
  * each other (as concurrent bulk calls on two streams would not be). */
 enum pg_op { PG_OP_INSERT = 0, PG_OP_QUERY = 1, PG_OP_REMOVE = 2 };

@@ -83,6 +83,25 @@ __device__ inline void serve_one(const TableDev &T, const ReqDev &r, uint64_t j,
         }
         return;
     }
+    if (T.rep == PG_REP_KV16) {  // the flat TCF's 16/16 key-value slots
+        if (r.op == PG_OP_INSERT) {
+            const uint64_t v = r.vals ? ((const uint16_t *)r.vals)[j] : 0u;
+            const int st = kv_insert_key<P, SlotKV16>(T, key, v, t);
+            if (r.flags && t.leader()) r.flags[j] = (uint8_t)st;
+        } else if (r.op == PG_OP_QUERY) {
+            uint64_t v = 0;
+            bool own = false;
+            const bool f = kv_query_key<P, SlotKV16>(T, key, t, v, own);
+            if (f ? own : t.leader()) {
+                if (r.flags) r.flags[j] = f ? 1 : 0;
+                if (r.vals_out) ((uint16_t *)r.vals_out)[j] = f ? (uint16_t)v : 0;
+            }
+        } else {
+            const bool f = kv_remove_key<P, SlotKV16>(T, key, t);
+            if (r.flags && t.leader()) r.flags[j] = f ? 1 : 0;
+        }
+        return;
+    }
     const bool fast16 = T.lv[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT && T.lv[0].B == 16;
     if (r.op == PG_OP_INSERT) {
         const uint32_t v = r.vals ? ((const uint16_t *)r.vals)[j] : 0u;
@@ -275,9 +294,10 @@ int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_
     for (uint32_t i = 0; i < ntables; i++) {
         uint32_t tile = 0;
         const pg::TableDev *T = pg::table_dev(tables[i], &tile);
-        // one tile width (<= 16) for every served table; KV64 and 16/16 filters
+        // one tile width (<= 16) for every served table; KV64, 16/16 filters
+        // and 16/16 key-value slots
         if (!T || (i > 0 && tile != s->P) || tile > 16 ||
-            (T->rep != PG_REP_KV64 && T->rep != PG_REP_FP16_V16)) {
+            (T->rep != PG_REP_KV64 && T->rep != PG_REP_FP16_V16 && T->rep != PG_REP_KV16)) {
             delete s;
             return PG_ERR_UNSUPPORTED;
         }

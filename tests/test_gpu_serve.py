@@ -271,3 +271,58 @@ def test_progress_hands_over_a_large_batch_in_parts():
     torch.cuda.synchronize()
     assert bool((flags == 1).all()) and t.get_fill() == 12000
     srv.free()
+
+
+def test_flat_tcf_served_like_plain_calls():
+    """The reference's flat TCF (16/16 key-value slots): insert requests,
+    then queries of every key and removes of the last quarter, served and
+    as per-request plain calls on a twin table.  Every key is stored and
+    found both ways, every remove lands, and both tables end with the same
+    fill.  16-bit keys can collide, and the two layouts differ in slot order,
+    so a query of a colliding key may read the other key's slot (the values
+    here are the keys' low 16 bits, which such keys share)."""
+    tabs = [pg.Table(pg.flat_tcf_desc(1 << 18, 1 << 12)) for _ in range(2)]
+    srv = pg.Server([tabs[0]], budget_bytes=64 << 20, nstreams=4)
+    plain = pg.Server([tabs[1]], budget_bytes=64 << 20, nstreams=1)
+    keys = oracle.gen_keys(80, 0, 200000)          # 76 % of level 0
+    chunks = [keys[a:a + 20000] for a in range(0, len(keys), 20000)]
+    n_rm = len(keys) // 4
+
+    def drain(s, want):
+        done = []
+        for _ in range(1_000_000):
+            done += s.progress()
+            if len(done) == want:
+                return done
+        raise AssertionError("server did not drain")
+
+    results = []
+    for s, run in ((srv, False), (plain, True)):
+        tab = tabs[1] if run else tabs[0]
+        ins = [buffers(tab, pg.OP_INSERT, k) for k in chunks]
+        qry = [buffers(tab, pg.OP_QUERY, k) for k in chunks]
+        rm = buffers(tab, pg.OP_REMOVE, keys[-n_rm:])
+        phases = [[(pg.OP_INSERT, b) for b in ins], [(pg.OP_QUERY, b) for b in qry], [(pg.OP_REMOVE, rm)]]
+        for phase in phases:
+            for op, (kk, vv, oo, ff) in phase:
+                if run:
+                    s.run_plain(0, op, kk, vv, oo, ff)
+                else:
+                    s.submit(0, op, kk, vv, oo, ff)
+            if not run:
+                drain(s, len(phase))
+        results.append((ins, qry, rm))
+    (ins_s, qry_s, rm_s), (ins_p, qry_p, rm_p) = results
+    for a, b in zip(ins_s, ins_p):
+        assert (a[3].numpy() == 1).all() and (b[3].numpy() == 1).all()
+    same = 0
+    for a, b, k in zip(qry_s, qry_p, chunks):
+        assert a[3].numpy().all() and b[3].numpy().all()
+        want = (k & np.uint64(0xFFFF)).astype(np.uint16).view(np.int16)
+        assert (a[2].numpy() == want).mean() > 0.999 and (b[2].numpy() == want).mean() > 0.999
+        same += int((a[2].numpy() == b[2].numpy()).sum())
+    assert same >= len(keys) - 20
+    assert rm_s[3].numpy().all() and rm_p[3].numpy().all()
+    assert tabs[0].get_fill() == tabs[1].get_fill() == len(keys) - n_rm
+    srv.free()
+    plain.free()
