@@ -291,6 +291,9 @@ class Table:
 
     @property
     def is_kv(self) -> bool:
+        """True for key_val_pair<u64,u64> tables, whose value buffers are
+        64-bit; the other tables (the 16/16 key-value slots included) take
+        16- or 8-bit value buffers."""
         return self.desc.rep == REP_KV64
 
     def set_mode(self, mode: int):
