@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--tables", type=int, default=4096, help="batched: number of filters")
     ap.add_argument("--kv-tile", type=int, default=4, help="kv: tile width (4 is hash_table_64's)")
     ap.add_argument("--flat-tile", type=int, default=4, help="flat: tile width (4 is the reference's)")
+    ap.add_argument("--kv-scheme", choices=["bucket", "pow"], default="bucket",
+                    help="kv: bucket = hash_table_64's bucket_insert (B 16, 20 probes); pow = "
+                         "power_of_n_insert_scheme (B 8, 3 probes, tile 8: the shape of the reference's p2_table)")
     return ap.parse_args()
 
 
@@ -73,8 +76,14 @@ def run_kv(args):
     m = 1 << log2
     n = int(0.9 * m)
     nq = 1 << (log2 - 2)
-    table = pg.Table(pg.kv_desc(m, bucket_size=16, tile=args.kv_tile, prober=pg.PROBE_DOUBLE,
-                                hasher=pg.HASH_FMIX64, max_probes=20))
+    if args.kv_scheme == "pow":
+        table = pg.Table(pg.TableDesc(pg.REP_KV64, [pg.Level(m, pg.SCHEME_POWER_OF_N, 8, 8, pg.PROBE_DOUBLE,
+                                                             pg.HASH_FMIX64, 3)]))
+        shape = "power_of_n_insert_scheme, B=8, tile 8, 3 probes"
+    else:
+        table = pg.Table(pg.kv_desc(m, bucket_size=16, tile=args.kv_tile, prober=pg.PROBE_DOUBLE,
+                                    hasher=pg.HASH_FMIX64, max_probes=20))
+        shape = f"bucket_insert, B=16, tile {args.kv_tile}, 20 probes"
     keys = torch.empty(n, dtype=torch.int64, device=dev)
     pg.gen_keys(99, 0, n, keys)
     vals = keys ^ torch.tensor(0x9E3779B97F4A7C15 - (1 << 64), dtype=torch.int64, device=dev)
@@ -130,8 +139,8 @@ def run_kv(args):
         "unit": "Mops/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(sum(res.values()), 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic (splitmix64 keys, Zipf(0.99) hits)",
-        "config": {"workload": f"KV 2^{log2} slots x 16 B ({m * 16 >> 30} GiB), double hashing, B=16, tile {args.kv_tile}, "
-                               f"20 probes; {n} inserts, {nq} queries (50% Zipf(0.99) hits, 50% absent), "
+        "config": {"workload": f"KV 2^{log2} slots x 16 B ({m * 16 >> 30} GiB), double hashing, {shape}; "
+                               f"{n} inserts, {nq} queries (50% Zipf(0.99) hits, 50% absent), "
                                f"delete {ndel}, re-query"},
         "insert_mops": round(n / res["insert"] / 1e3, 1), "query_mops": round(nq / res["query"] / 1e3, 1),
         "delete_mops": round(ndel / res["delete"] / 1e3, 1), "requery_mops": round(nq / res["requery"] / 1e3, 1),
