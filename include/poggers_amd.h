@@ -123,14 +123,14 @@ enum pg_rep {
  *                 shortcut scheme over buckets packed `group` = K to a
  *                 128-byte line (bucket b at byte 128*(b/K) + size*(b%K)),
  *                 packed filter slots only
- *   POWER_OF_N    insert_schemes/power_of_n.cuh:38-379: each key goes to
+ *   POWER_OF_N    insert_schemes/power_of_n.cuh:39-492: each key goes to
  *                 the least-filled of its max_probes (at most 4) probe
  *                 buckets, ties in probe order; a bucket's fill counts every
  *                 non-empty slot, tombstones included; inserts claim empty
  *                 slots only; queries and removes check every probe bucket.
  *                 Key-value slots (KV64, KEY64, KV16) only.
  *   POWER_OF_N_SHORTCUT_SLOTS  insert_schemes/power_of_n_shortcut.cuh:
- *                 38-449: POWER_OF_N with the 75 % shortcut (a probe bucket
+ *                 39-505: POWER_OF_N with the 75 % shortcut (a probe bucket
  *                 whose fill is below 3B/4 takes the key at once), over
  *                 key-value slots (the reference's flat TCF primary)       */
 enum pg_scheme {

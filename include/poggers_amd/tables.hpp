@@ -73,8 +73,8 @@ struct power_of_n_insert_shortcut_bucket_scheme {
     static constexpr uint32_t scheme = PG_SCHEME_POWER_OF_N_SHORTCUT;
 };
 struct linear_insert_bucket_scheme { static constexpr uint32_t scheme = PG_SCHEME_LINEAR_BUCKETS; };
-// The flat power-of-n schemes over key-value slots (power_of_n.cuh:38-379,
-// power_of_n_shortcut.cuh:38-449).
+// The flat power-of-n schemes over key-value slots (power_of_n.cuh:39-492,
+// power_of_n_shortcut.cuh:39-505).
 struct power_of_n_insert_scheme { static constexpr uint32_t scheme = PG_SCHEME_POWER_OF_N; };
 struct power_of_n_insert_shortcut_scheme {
     static constexpr uint32_t scheme = PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;

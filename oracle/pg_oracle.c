@@ -828,14 +828,14 @@ static int kv_remove(or_table *t, or_level *L, int lvl, uint64_t key)
 }
 
 /* The flat power-of-n schemes over key-value slots: power_of_n_insert_scheme
- * (insert_schemes/power_of_n.cuh:333-379) and power_of_n_insert_shortcut_scheme
- * (power_of_n_shortcut.cuh:325-406, data_structs/tcf.cuh:30), each "bucket"
+ * (insert_schemes/power_of_n.cuh:326-422) and power_of_n_insert_shortcut_scheme
+ * (power_of_n_shortcut.cuh:327-435, data_structs/tcf.cuh:30), each "bucket"
  * being B consecutive slots.  Unlike the packed-bucket scheme:
  *   fill      = slots whose key is not empty, tombstones included
- *               (check_fill_bucket, power_of_n_shortcut.cuh:297-316);
- *   claim     = the lowest EMPTY slot only (insert_into_bucket, :140-190);
- *   query     = every probe bucket in order, lowest match (:192-230, :431-446);
- *   remove    = the lowest match's key -> tombstone, value kept (:233-295).
+ *               (check_fill_bucket, power_of_n_shortcut.cuh:234-261);
+ *   claim     = the lowest EMPTY slot only (insert_into_bucket, :134-191);
+ *   query     = every probe bucket in order, lowest match (:193-232, :437-468);
+ *   remove    = the lowest match's key -> tombstone, value kept (:263-325, :470-505).
  * insert: the shortcut scheme first tries each probe bucket whose fill is
  * below 0.75 B (4*fill < 3*B); then, for both schemes, the fill levels in
  * ascending order, each probe with that fill in probe order. */

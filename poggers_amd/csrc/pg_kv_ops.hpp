@@ -224,12 +224,12 @@ __device__ bool kv_reg_remove(const LevelDev &L, uint64_t key, const Tile<P> &t)
 
 // ---------------------------------------------------------------------------
 // Flat power-of-n schemes over key-value slots: power_of_n_insert_scheme
-// (insert_schemes/power_of_n.cuh:38-379) and power_of_n_insert_shortcut_scheme
-// (power_of_n_shortcut.cuh:38-449).  A bucket's fill counts every non-empty
+// (insert_schemes/power_of_n.cuh:39-492) and power_of_n_insert_shortcut_scheme
+// (power_of_n_shortcut.cuh:39-505).  A bucket's fill counts every non-empty
 // slot, tombstones included (check_fill_bucket, power_of_n_shortcut.cuh:
-// 297-316); a claim takes the lowest empty slot (insert_into_bucket,
-// :130-190); queries and removes check every probe bucket in probe order and
-// act on the lowest matching slot (:192-295).  Runtime B, stripes of P slots,
+// 234-261); a claim takes the lowest empty slot (insert_into_bucket,
+// :134-191); queries and removes check every probe bucket in probe order and
+// act on the lowest matching slot (:193-232, :263-325, :437-505).  Runtime B, stripes of P slots,
 // each bucket read again for the claim as in the reference.
 // ---------------------------------------------------------------------------
 template <int P, class SL>
@@ -269,7 +269,7 @@ __device__ bool kv_pow_claim(typename SL::S *bk, uint32_t B, uint32_t lim, uint6
     return false;
 }
 
-// power_of_n.cuh:333-379 / power_of_n_shortcut.cuh:325-406.  The shortcut
+// power_of_n.cuh:326-422 / power_of_n_shortcut.cuh:327-435.  The shortcut
 // claims in a probe bucket whose fill is below 3B/4.  Since empties are
 // claimed lowest first and slots never return to empty, the non-empty slots
 // of a bucket are always a prefix, so a claim at slot i lands at fill i:

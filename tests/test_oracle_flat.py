@@ -6,8 +6,8 @@ hand-derived placements and the reference's own checks.
       the reference's data_structs::tcf, data_structs/tcf.cuh:29-30):
       like the packed-bucket shortcut scheme, but a bucket's fill counts
       every non-empty slot, tombstones included (check_fill_bucket
-      :297-316), and an insert claims empty slots only (:140-190);
-  power_of_n_insert_scheme (power_of_n.cuh:333-379): no 75 % shortcut, the
+      :234-261), and an insert claims empty slots only (:134-191);
+  power_of_n_insert_scheme (power_of_n.cuh:326-422): no 75 % shortcut, the
       key goes to the least-filled probe bucket, ties in probe order;
   key_val_pair::atomic_reset (key_val_pair.cuh:143-151) turns the key into
       the tombstone and leaves the value.
