@@ -272,6 +272,18 @@ int pg_bulk_remove(pg_handle h, const uint64_t *d_keys, uint64_t n, uint8_t *d_f
  * *out is a HOST pointer. */
 int pg_fill(pg_handle h, uint64_t *out, void *stream);
 
+/* pg_fill without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (tight serving loops that
+ * report fill between batches, harnesses that forbid hipMalloc).  Counts into
+ * a 32-byte file-scope device array (one 8-byte counter per level, for the
+ * PG_MAX_LEVELS levels a table can have; a handle claiming more is refused
+ * with PG_ERR_UNSUPPORTED) instead of the table's own counter, calls no
+ * hipMalloc / hipFree, and returns the same total as pg_fill.  The table's device must be current (else
+ * PG_ERR_INVALID).  All calls on one device share that device's array, so
+ * calls that overlap on one device race: order them (one stream, or
+ * external synchronisation).  Synchronises `stream`; *out is a HOST pointer. */
+int pg_fill_noalloc(pg_handle h, uint64_t *out, void *stream);
+
 /* Bytes of slot memory over all levels: replaces host_bytes_in_use
  * (tables/bucketed_table.cuh:405-425). */
 int pg_bytes(pg_handle h, uint64_t *out);

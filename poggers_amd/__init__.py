@@ -42,7 +42,7 @@ EXPORTS = [
     "pg_batch_level_ptr", "pg_batch_bytes",
     "pg_server_create", "pg_server_destroy", "pg_server_submit", "pg_server_progress", "pg_server_pending",
     "pg_server_run_plain", "pg_server_set_batch", "pg_server_set_order", "pg_server_submit_many", "pg_save",
-    "pg_load", "pg_file_info",
+    "pg_load", "pg_file_info", "pg_fill_noalloc",
 ]
 OP_INSERT, OP_QUERY, OP_REMOVE = 0, 1, 2
 ORDER_NONE, ORDER_TABLE = 0, 1
@@ -108,6 +108,7 @@ def lib():
     L.pg_bulk_query.argtypes = [vp, vp, u64, vp, vp, vp]
     L.pg_bulk_remove.argtypes = [vp, vp, u64, vp, vp]
     L.pg_fill.argtypes = [vp, ctypes.POINTER(u64), vp]
+    L.pg_fill_noalloc.argtypes = [vp, ctypes.POINTER(u64), vp]
     L.pg_bytes.argtypes = [vp, ctypes.POINTER(u64)]
     L.pg_num_slots.argtypes = [vp, ctypes.POINTER(u64)]
     L.pg_level_ptr.argtypes = [vp, u32, ctypes.POINTER(vp), ctypes.POINTER(u64)]

@@ -1,4 +1,5 @@
 // pg_capi.hip -- host side of the C ABI (include/poggers_amd.h).
+#include <mutex>
 #include <new>
 #include <stdio.h>
 #include <stdlib.h>
@@ -347,6 +348,57 @@ int pg_fill(pg_handle h, uint64_t *out, void *stream) {
     if (hipMemcpyAsync(out, h->d_counter, sizeof(uint64_t), hipMemcpyDeviceToHost, s) != hipSuccess)
         return PG_ERR_HIP;
     return hipStreamSynchronize(s) == hipSuccess ? PG_OK : PG_ERR_HIP;
+}
+
+// Scratch of pg_fill_noalloc: one 8-byte fill counter per level, in a
+// file-scope device array instead of the table's hipMalloc'd counter.  Each
+// device holds its own copy at its own address, looked up once per device
+// with hipGetSymbolAddress; host code uses only that address (the array's
+// name is a host-side stand-in, not a device pointer).  Every call on a
+// device shares that device's copy: calls that overlap on one device race.
+constexpr uint32_t kNoAllocCounters = PG_MAX_LEVELS;  // 32 B: one per level a table can have
+static __device__ __align__(256) uint64_t scratch[kNoAllocCounters];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static uint64_t *g_scratch_ptr[kMaxScratchDevices];
+
+static int noalloc_scratch(int dev, uint64_t **out) {
+    if (dev < 0 || dev >= kMaxScratchDevices) return PG_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PG_ERR_HIP;
+        g_scratch_ptr[dev] = (uint64_t *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PG_OK;
+}
+
+int pg_fill_noalloc(pg_handle h, uint64_t *out, void *stream) {
+    if (!h || !out) return PG_ERR_INVALID;
+    if (h->dev.nlevels > kNoAllocCounters) return PG_ERR_UNSUPPORTED;  // more levels than counters (a guard)
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return PG_ERR_HIP;
+    if (dev != h->device) return PG_ERR_INVALID;  // the scratch of the table's device must be current
+    uint64_t *ctr = nullptr;
+    int rc = noalloc_scratch(dev, &ctr);
+    if (rc != PG_OK) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    for (uint32_t l = 0; l < h->dev.nlevels; l++) {  // level l counts into ctr[l]
+        pg::TableDev one = h->dev;
+        one.nlevels = 1;
+        one.lv[0] = h->dev.lv[l];
+        rc = pg::launch_fill(one, ctr + l, s);
+        if (rc != PG_OK) return rc;
+    }
+    uint64_t per_level[PG_MAX_LEVELS] = {0, 0, 0, 0};
+    if (hipMemcpyAsync(per_level, ctr, sizeof(uint64_t) * h->dev.nlevels, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return PG_ERR_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return PG_ERR_HIP;
+    uint64_t total = 0;
+    for (uint32_t l = 0; l < h->dev.nlevels; l++) total += per_level[l];
+    *out = total;
+    return PG_OK;
 }
 
 int pg_bytes(pg_handle h, uint64_t *out) {
