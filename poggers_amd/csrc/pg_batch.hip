@@ -465,7 +465,9 @@ static int batch_op(pg_batch b, bool query, const uint64_t *d_keys, const uint64
     a.fail_count = d_fail;
     a.counter = b->d_counter;
     a.chunk = b->chunk;
-    return pg::launch_batch(b->proto.level[0].tile, b->proto.level[0].bucket_size, a, query,
+    // the register fast path takes levels of at most two probes
+    const uint32_t fastB = b->proto.level[0].max_probes <= 2 ? b->proto.level[0].bucket_size : 0u;
+    return pg::launch_batch(b->proto.level[0].tile, fastB, a, query,
                             b->mode == PG_DETERMINISTIC, b->blocks, b->items, (hipStream_t)stream);
 }
 

@@ -70,6 +70,11 @@ __device__ inline uint32_t select_word(const uint32_t (&w)[V], int v) {
     return x;
 }
 
+// Outcome of a bounded slot claim.
+constexpr int kClaimed = 1;  // the CAS landed
+constexpr int kNoSlot = 0;   // no empty (or reusable) slot left: the bucket cannot take the key
+constexpr int kBudget = -1;  // the allowed number of CAS attempts ran out
+
 // ---------------------------------------------------------------------------
 // A bucket held in registers: compile-time B and P.
 // ---------------------------------------------------------------------------
@@ -107,25 +112,39 @@ struct Bucket {
         return f;
     }
 
-    // insert / insert_delete: claim the lowest empty (or tombstone) slot
-    // among slots [0, lim).  packed_bucket.cuh:64-180; CAS words
-    // grouped_storage_sub_bits.cuh:127-157.
-    __device__ inline bool claim(uint32_t *slots, uint64_t b, uint32_t word, bool wd,
-                                 const Tile<P> &t, uint32_t lim = (uint32_t)B) const {
+    // insert / insert_delete: claim the lowest empty (or, with wd, tombstone)
+    // slot (packed_bucket.cuh:64-180; CAS words grouped_storage_sub_bits.cuh:
+    // 127-157) with at most `tries` CAS attempts, passing over the first
+    // `taken` candidates of the view: slots an earlier attempt on this bucket
+    // found taken since the load (candidates are tried lowest first and a
+    // taken slot stays taken, so they are exactly the lowest ones).  Every
+    // failed attempt adds one to `taken`, so fill() + taken is the bucket's
+    // fill as far as this key knows.  Returns kClaimed, kNoSlot (no candidate
+    // left in the view: the bucket cannot take the key) or kBudget (tries ran
+    // out).  `taken` and `tries` are tile-uniform.
+    __device__ inline int claim(uint32_t *slots, uint64_t b, uint32_t word, bool wd, const Tile<P> &t,
+                                uint32_t &taken, uint32_t tries = ~0u) const {
         uint32_t *base = slots + b * (uint64_t)B;
+        uint32_t skip = taken;
 #pragma unroll
         for (int s = 0; s < S; s++) {
             uint32_t cand = 0;
             if (ok(s, t)) {
 #pragma unroll
                 for (int v = 0; v < V; v++)
-                    if ((w_empty(w[s][v]) || (wd && w_tomb(w[s][v]))) &&
-                        (uint32_t)(s * ROW) + t.rank * V + (uint32_t)v < lim)
-                        cand |= 1u << v;
+                    if (w_empty(w[s][v]) || (wd && w_tomb(w[s][v]))) cand |= 1u << v;
             }
             uint64_t lanes = t.bits(cand != 0);
             while (lanes) {
                 const uint32_t l = (uint32_t)ctz64(lanes);
+                if (skip) {
+                    skip--;
+                    if (t.rank == l) cand &= cand - 1u;
+                    lanes = t.bits(cand != 0);
+                    continue;
+                }
+                if (tries == 0) return kBudget;
+                tries--;
                 bool success = false;
                 if (t.rank == l) {
                     const int v = __builtin_ctz(cand);
@@ -136,12 +155,13 @@ struct Bucket {
                 }
                 if (t.any(success)) {
                     t.acc[2]++;
-                    return true;
+                    return kClaimed;
                 }
+                taken++;
                 lanes = t.bits(cand != 0);
             }
         }
-        return false;
+        return kNoSlot;
     }
 
     // query: lowest matching slot, packed_bucket.cuh:182-219.  The owner lane
@@ -295,18 +315,21 @@ __device__ int gen_fill(const void *base, uint32_t B, const Tile<P> &t) {
     return f;
 }
 
-// Claim the lowest empty slot (with `wd`, the lowest empty-or-tombstone one).
-// packed_bucket.cuh:64-180, 12_bit_bucket.cuh:94-144.
+// Claim the lowest empty slot (with `wd`, the lowest empty-or-tombstone one)
+// with at most `tries` CAS attempts.  packed_bucket.cuh:64-180,
+// 12_bit_bucket.cuh:94-144.  Returns kClaimed, kNoSlot or kBudget.
 template <int P, class F = Fmt16>
-__device__ bool gen_claim(void *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t,
-                          uint32_t lim = ~0u) {
-    for (uint32_t s0 = 0; s0 < B && s0 < lim; s0 += P) {
+__device__ int gen_claim(void *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t,
+                         uint32_t tries = ~0u) {
+    for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
-        const bool ok = i < B && i < lim;
+        const bool ok = i < B;
         const uint32_t w = ok ? s_get<F>(base, i) : 0u;
         bool cand = ok && (s_empty<F>(w) || (wd && s_tomb<F>(w)));
         uint64_t lanes = t.bits(cand);
         while (lanes) {
+            if (tries == 0) return kBudget;
+            tries--;
             const uint32_t l = (uint32_t)ctz64(lanes);
             bool success = false;
             if (t.rank == l) {
@@ -315,12 +338,12 @@ __device__ bool gen_claim(void *base, uint32_t B, uint32_t word, bool wd, const 
             }
             if (t.any(success)) {
                 t.acc[2]++;
-                return true;
+                return kClaimed;
             }
             lanes = t.bits(cand);
         }
     }
-    return false;
+    return kNoSlot;
 }
 
 // Lowest matching slot; when none, *fill receives the bucket's fill (same
@@ -402,21 +425,63 @@ __device__ bool gen_remove(void *base, uint32_t B, uint32_t fp, const Tile<P> &t
 // Level operations, generic shapes.
 // ---------------------------------------------------------------------------
 
-// The first pass of the shortcut insert claims a slot of a probe bucket only
-// while the bucket's fill is below 3B/4 (power_of_n_shortcut_buckets.cuh:
-// 223).  Applied in key order, the fill read just before the claim is exact.
-// With many keys in flight the fill is read earlier than the claim, and
-// other keys may have filled the bucket in between.  Slots are claimed
-// lowest index first, so in a bucket without tombstones a claim at slot
-// index i means i slots were filled when it landed.  Concurrent kernels
-// therefore also require slot index < ceil(3B/4) in the first pass, which
-// restores the rule at the moment of the claim.  A bucket whose first-pass
-// claim fails then enters the second pass at fill ceil(3B/4) rather than as
-// full.  Key-order (deterministic) kernels keep the unbounded claim, which
-// is then exact.
+// Concurrent kernels and the two passes of the shortcut insert.
+//
+// The first pass claims a slot of a probe bucket only while the bucket's fill
+// is below 3B/4 (power_of_n_shortcut_buckets.cuh:223); the second pass claims
+// in the least-filled bucket (:254-300).  Applied in key order the fills read
+// just before each claim are exact.  With many keys in flight they are read
+// one or more memory round trips before the claim lands, and other keys land
+// in between, so a rule checked on the read fills can be broken when the claim
+// lands.  The checks are therefore made on the fill at the moment of the
+// claim: slots go empty -> filled (and, for a tombstone reuse, tombstone ->
+// filled) and claims take the lowest candidate slot first, so when a claim
+// lands, the bucket's fill is the filled count of the view read plus the
+// number of CAS attempts that failed on slots the view showed free.  Bounding
+// the attempts bounds the fill at the moment of the claim:
+//   * first pass: at most ceil(3B/4) - f attempts (f = filled count read);
+//   * second pass: claim in the least-filled bucket (lowest probe index on
+//     ties) with at most (next-least fill) - f + 1 attempts, so the bucket
+//     taken is never fuller than the other choice; a bucket that runs out of
+//     attempts has its known fill raised by that many and the choice is made
+//     again.
+// Without these bounds, concurrent keys pile into buckets that looked
+// emptier than they had become, the two choices fill unevenly, and keys spill
+// to the backing level faster than in key order (at 2^22 slots with four-lane
+// tiles: 1.5x the backing keys after a 90 % fill, and under repeated 10 %
+// insert/delete churn a full backing level after seven rounds).  Key-order
+// (deterministic) kernels keep the unbounded reference loops, whose layouts
+// are bit-exact.
 template <int P>
-__device__ inline uint32_t first_pass_limit(const Tile<P> &t, uint32_t B) {
-    return t.concurrent ? (3u * B + 3u) / 4u : B;
+__device__ inline uint32_t first_pass_tries(const Tile<P> &t, int B, int f) {
+    return t.concurrent ? (uint32_t)((3 * B + 3) / 4 - f) : ~0u;
+}
+
+// Index j of the least-filled of np probe buckets (lowest index on ties), the
+// next-least fill, and how many claim attempts keep j's fill at the moment of
+// the claim within the key-order rule: up to the next-least fill when j comes
+// first in probe order among the buckets at that fill (ties go to the first
+// probe), else strictly below it.
+__device__ inline void least_filled(const int (&fill)[kMaxShortcutProbes], int np, int B, int &j, int &next,
+                                    uint32_t &tries) {
+    j = 0;
+#pragma unroll
+    for (int q = 1; q < (int)kMaxShortcutProbes; q++)
+        if (q < np && fill[q] < fill[j]) j = q;
+    next = B;
+    int first_at_next = (int)kMaxShortcutProbes;
+#pragma unroll
+    for (int q = 0; q < (int)kMaxShortcutProbes; q++) {
+        if (q < np && q != j) {
+            if (fill[q] < next) next = fill[q], first_at_next = q;
+            else if (fill[q] == next && q < first_at_next) first_at_next = q;
+        }
+    }
+    int fj = B;
+#pragma unroll
+    for (int q = 0; q < (int)kMaxShortcutProbes; q++)
+        if (q == j) fj = fill[q];
+    tries = (uint32_t)(next - fj + (j < first_at_next ? 1 : 0));
 }
 
 // power_of_n_shortcut_buckets.cuh:198-306 (insert) and :309-417
@@ -430,7 +495,6 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
     uint64_t bidx[kMaxShortcutProbes];
     int fill[kMaxShortcutProbes];
     int np = 0, min_fill = B;
-    const uint32_t lim1 = first_pass_limit(t, (uint32_t)B);
 #pragma unroll
     for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
         const uint64_t p = pr.at(j);
@@ -440,24 +504,57 @@ __device__ bool gen_shortcut_insert(const LevelDev &L, uint64_t key, uint32_t fp
         t.acc[ai]++;
         int f = gen_fill<P, F>(bucket_ptr(L, b), L.B, t);
         if (4 * f < 3 * B) {
-            if (gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t, lim1)) return true;
-            f = (int)lim1;
+            const int r = gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t, first_pass_tries(t, B, f));
+            if (r == kClaimed) return true;
+            f = (r == kNoSlot) ? B : (3 * B + 3) / 4;
         }
         fill[j] = f;
         if (f < min_fill) min_fill = f;
         np = j + 1;
     }
-    int lvl = min_fill;
-    while (lvl < B) {
-        int next = B;
+    if (!t.concurrent) {
+        int lvl = min_fill;
+        while (lvl < B) {
+            int next = B;
 #pragma unroll
-        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
-            if (j < np) {
-                if (fill[j] == lvl && gen_claim<P, F>(bucket_ptr(L, bidx[j]), L.B, word, wd, t)) return true;
-                if (fill[j] > lvl && fill[j] < next) next = fill[j];
+            for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+                if (j < np) {
+                    if (fill[j] == lvl && gen_claim<P, F>(bucket_ptr(L, bidx[j]), L.B, word, wd, t) == kClaimed)
+                        return true;
+                    if (fill[j] > lvl && fill[j] < next) next = fill[j];
+                }
             }
+            lvl = next;
         }
-        lvl = next;
+        return false;
+    }
+    // The generic claim reads the bucket again, so the fill is read again just
+    // before it: the bound then counts from that fill.  Every round raises one
+    // known fill, so the loop ends within np * B rounds.
+    if (np == 0) return false;
+    for (int round = 0; round < (int)kMaxShortcutProbes * (B + 1); round++) {
+        int j, next;
+        uint32_t tries;
+        least_filled(fill, np, B, j, next, tries);
+        int fj = B;
+        uint64_t bj = 0;
+#pragma unroll
+        for (int q = 0; q < (int)kMaxShortcutProbes; q++)
+            if (q == j) fj = fill[q], bj = bidx[q];
+        if (fj >= B) return false;
+        void *base = bucket_ptr(L, bj);
+        const int now = gen_fill<P, F>(base, L.B, t);
+        int nf;
+        if (now > fj) {
+            nf = now;  // it filled since: choose again
+        } else {
+            const int r = gen_claim<P, F>(base, L.B, word, wd, t, tries);
+            if (r == kClaimed) return true;
+            nf = (r == kNoSlot) ? B : fj + (int)tries;
+        }
+#pragma unroll
+        for (int q = 0; q < (int)kMaxShortcutProbes; q++)
+            if (q == j) fill[q] = nf < B ? nf : B;
     }
     return false;
 }
@@ -474,7 +571,7 @@ __device__ bool gen_linear_insert(const LevelDev &L, uint64_t key, uint32_t fp, 
         if (p == kEnd) return false;
         const uint64_t b = fastmod(p, L.nb);
         t.acc[ai]++;
-        if (gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t)) return true;
+        if (gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t) == kClaimed) return true;
     }
 }
 
@@ -527,76 +624,73 @@ __device__ bool gen_level_remove(const LevelDev &L, uint64_t key, uint32_t fp, c
 
 
 // ---------------------------------------------------------------------------
-// Level-0 fast path: POWER_OF_N_SHORTCUT with compile-time B.  The caller has
-// already hashed the key (pr) and loaded its first probe bucket (bk0, b0), so
-// a tile can issue the first-choice loads of several keys at once; the first
-// two probe buckets stay in registers for both passes.
+// Level-0 fast path: POWER_OF_N_SHORTCUT with compile-time B and at most two
+// probes (every two-choice prober, and double hashing with Max_Probes 2; the
+// launchers send levels of more probes to the generic routines).  The caller
+// has already hashed the key (pr) and loaded its first probe bucket (bk0,
+// b0), so a tile can issue the first-choice loads of several keys at once;
+// both probe buckets stay in registers for both passes.
 // power_of_n_shortcut_buckets.cuh:198-306 (insert), :309-417 (with delete).
 // ---------------------------------------------------------------------------
 template <int P, int B>
 __device__ bool fast_shortcut_insert(const LevelDev &L, const Probe &pr, uint32_t word, bool wd,
                                      const Tile<P> &t, const Bucket<P, B> &bk0, uint64_t b0) {
     uint32_t *slots = (uint32_t *)L.slots;
-    Bucket<P, B> bk1;
-    uint64_t bidx[kMaxShortcutProbes];
-    int fill[kMaxShortcutProbes];
-    int np = 0, min_fill = B;
-    const uint32_t lim1 = first_pass_limit(t, (uint32_t)B);
-#pragma unroll
-    for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
-        uint64_t b;
-        if (j == 0) {
-            if (pr.at(0) == kEnd) break;
-            b = b0;
-        } else {
-            const uint64_t p = pr.at(j);
-            if (p == kEnd) break;
-            b = fastmod(p, L.nb);
-        }
-        bidx[j] = b;
-        int f;
-        if (j == 0) {
-            f = bk0.fill(t);
-            if (4 * f < 3 * B) {
-                if (bk0.claim(slots, b, word, wd, t, lim1)) return true;
-                f = (int)lim1;
-            }
-        } else if (j == 1) {
-            bk1.load(slots, b, t);
-            f = bk1.fill(t);
-            if (4 * f < 3 * B) {
-                if (bk1.claim(slots, b, word, wd, t, lim1)) return true;
-                f = (int)lim1;
-            }
-        } else {
-            t.acc[0]++;
-            f = gen_fill<P>(slots + b * B, B, t);
-            if (4 * f < 3 * B) {
-                if (gen_claim<P>(slots + b * B, B, word, wd, t, lim1)) return true;
-                f = (int)lim1;
-            }
-        }
-        fill[j] = f;
-        if (f < min_fill) min_fill = f;
-        np = j + 1;
+    if (pr.at(0) == kEnd) return false;
+    constexpr int kLim = (3 * B + 3) / 4;
+    uint32_t taken0 = 0, taken1 = 0;  // view candidates found taken, per bucket
+    int f0 = bk0.fill(t);
+    if (4 * f0 < 3 * B) {
+        const int r = bk0.claim(slots, b0, word, wd, t, taken0, first_pass_tries(t, B, f0));
+        if (r == kClaimed) return true;
+        f0 = (r == kNoSlot) ? B : kLim;
     }
-    int lvl = min_fill;
-    while (lvl < B) {
-        int next = B;
-#pragma unroll
-        for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
-            if (j < np) {
-                if (fill[j] == lvl) {
-                    bool ok;
-                    if (j == 0) ok = bk0.claim(slots, bidx[0], word, wd, t);
-                    else if (j == 1) ok = bk1.claim(slots, bidx[1], word, wd, t);
-                    else ok = gen_claim<P>(slots + bidx[j] * B, B, word, wd, t);
-                    if (ok) return true;
-                }
-                if (fill[j] > lvl && fill[j] < next) next = fill[j];
-            }
+    const uint64_t p1 = pr.at(1);
+    const bool two = p1 != kEnd;
+    Bucket<P, B> bk1;
+    uint64_t b1 = 0;
+    int f1 = B;  // a missing second probe counts as full
+    if (two) {
+        b1 = fastmod(p1, L.nb);
+        bk1.load(slots, b1, t);
+        f1 = bk1.fill(t);
+        if (4 * f1 < 3 * B) {
+            const int r = bk1.claim(slots, b1, word, wd, t, taken1, first_pass_tries(t, B, f1));
+            if (r == kClaimed) return true;
+            f1 = (r == kNoSlot) ? B : kLim;
         }
-        lvl = next;
+    }
+    if (!t.concurrent) {
+        // key order: ascending fill levels, probe order within a level
+        int lvl = f0 < f1 ? f0 : f1;
+        while (lvl < B) {
+            int next = B;
+            if (f0 == lvl && bk0.claim(slots, b0, word, wd, t, taken0) == kClaimed) return true;
+            if (f0 > lvl && f0 < next) next = f0;
+            if (two) {
+                if (f1 == lvl && bk1.claim(slots, b1, word, wd, t, taken1) == kClaimed) return true;
+                if (f1 > lvl && f1 < next) next = f1;
+            }
+            lvl = next;
+        }
+        return false;
+    }
+    // Bounded fill-ordered pass on the register views: the first probe may
+    // land while no fuller than the second, the second only while less full
+    // than the first (ties go to the first probe, as in key order).  Every
+    // round raises one known fill, so the loop ends within 2 * B rounds.
+    for (int round = 0; round < 2 * (B + 1); round++) {
+        const bool pick1 = f1 < f0;  // the least filled, first probe on ties
+        const int fj = pick1 ? f1 : f0;
+        if (fj >= B) return false;
+        const uint32_t tries = pick1 ? (uint32_t)(f0 - f1) : (uint32_t)(f1 - f0 + 1);
+        const int r = pick1 ? bk1.claim(slots, b1, word, wd, t, taken1, tries)
+                            : bk0.claim(slots, b0, word, wd, t, taken0, tries);
+        if (r == kClaimed) return true;
+        int nf = (r == kNoSlot) ? B : fj + (int)tries;
+        nf = nf < B ? nf : B;
+        if (pick1) f1 = nf;
+        else f0 = nf;
     }
     return false;
 }
@@ -679,7 +773,7 @@ __device__ bool gen_level_upsert(const LevelDev &L, uint64_t key, uint32_t fp, u
             if (del) gen_remove<P, F>(base, L.B, fp, t, f);
             return true;
         }
-        if (linear && gen_claim<P, F>(base, L.B, word, false, t)) return true;
+        if (linear && gen_claim<P, F>(base, L.B, word, false, t) == kClaimed) return true;
     }
     if (linear) return false;
     return gen_shortcut_insert<P, F>(L, key, fp, word, del, t, ai);
@@ -1007,7 +1101,7 @@ int launch_tcf_p(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
     if (T.rep == PG_REP_FP8_V8) return launch_tcf_pb<P, 0, Fmt8>(T, op, a, s);
     if (T.rep == PG_REP_FP10_V6) return launch_tcf_pb<P, 0, Fmt10>(T, op, a, s);
     if (T.rep == PG_REP_FP12) return launch_tcf_pb<P, 0, Fmt12>(T, op, a, s);
-    if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT) {
+    if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT && L.max_probes <= 2) {  // fast_shortcut_insert's shape
         switch (L.B) {
             case 8: return launch_tcf_pb<P, 8>(T, op, a, s);
             case 16: return launch_tcf_pb<P, 16>(T, op, a, s);
