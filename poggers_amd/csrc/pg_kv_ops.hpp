@@ -52,11 +52,18 @@ struct SlotKV {
     __device__ static S none() { return make_ulonglong2(kKvTomb, 0); }
     __device__ static uint64_t key(const S &s) { return s.x; }
     __device__ static uint64_t val(const S &s) { return s.y; }
-    // key_val_pair::atomic_swap: CAS key 0 -> key, then store the value
-    // (key_val_pair.cuh:84-95)
+    // key_val_pair::atomic_swap (key_val_pair.cuh:84-95) CASes the key from
+    // empty and then stores the value, so a reader can see the key before its
+    // value.  Here the slot is claimed with the tombstone key instead, which
+    // every op already reads as occupied and never matching: inserts do not
+    // claim it (key-value slots never reuse tombstones), queries and removes
+    // walk past it.  Key and value are then written by one 16 B store, so a
+    // reader's 16 B slot load sees the slot empty, busy, or holding the key
+    // with its value, never the key alone.  The final slot bytes are the
+    // reference's.
     __device__ static bool claim(S *p, const S &, uint64_t sk, uint64_t v) {
-        if (atomicCAS((unsigned long long *)&p->x, kKvEmpty, sk) != kKvEmpty) return false;
-        p->y = v;
+        if (atomicCAS((unsigned long long *)&p->x, kKvEmpty, kKvTomb) != kKvEmpty) return false;
+        *p = make_ulonglong2(sk, v);
         return true;
     }
     // key_val_pair::atomic_reset: CAS key -> tombstone key, value kept
@@ -311,14 +318,20 @@ __device__ bool kv_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, con
         // So the least-filled bucket (first in probe order on ties) is
         // claimed only at a slot index no larger than the next-least fill,
         // which by the prefix property keeps it no fuller than the others at
-        // the claim; a failed claim re-reads the fills.  Each failure means
-        // some probe bucket gained a key since its read, so at most np*B
-        // rounds are needed; the cap makes that explicit.
+        // the claim.  A bounded claim that fails has found every slot below
+        // its bound taken, so that bucket's known fill rises past the bound;
+        // the fills read again each round only raise the known fills (a read
+        // can come from a stale cache line).  Every round raises the sum of
+        // the known fills, so np*B rounds suffice.
         for (int round = 0; round <= np * B; round++) {
             if (round) {
 #pragma unroll
-                for (int j = 0; j < (int)kMaxShortcutProbes; j++)
-                    if (j < np) fill[j] = kv_pow_fill<P, SL>(slots + bidx[j] * L.B, L.B, t);
+                for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
+                    if (j < np) {
+                        const int f = kv_pow_fill<P, SL>(slots + bidx[j] * L.B, L.B, t);
+                        if (f > fill[j]) fill[j] = f;
+                    }
+                }
             }
             int best = -1, bf = B, second = B;
 #pragma unroll
@@ -338,6 +351,9 @@ __device__ bool kv_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, con
 #pragma unroll
             for (int j = 1; j < (int)kMaxShortcutProbes; j++) bb = (j == best) ? bidx[j] : bb;
             if (kv_pow_claim<P, SL>(slots + bb * L.B, L.B, (uint32_t)second + 1u, sk, val, t)) return true;
+#pragma unroll
+            for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+                if (j == best) fill[j] = second + 1 < B ? second + 1 : B;
         }
         return false;
     }
@@ -562,7 +578,8 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
                     if (j < np) {
                         Row16<P, V> row;
                         row.load(slots + bidx[j] * L.B, L.B, t);
-                        fill[j] = row.fill(t);
+                        const int f = row.fill(t);
+                        if (f > fill[j]) fill[j] = f;
                     }
                 }
             }
@@ -586,6 +603,9 @@ __device__ bool kv16_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, c
             Row16<P, V> row;
             row.load(slots + bb * L.B, L.B, t);
             if (row.claim(slots + bb * L.B, (uint32_t)second + 1u, word, t)) return true;
+#pragma unroll
+            for (int j = 0; j < (int)kMaxShortcutProbes; j++)
+                if (j == best) fill[j] = second + 1 < B ? second + 1 : B;
         }
         return false;
     }

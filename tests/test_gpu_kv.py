@@ -105,7 +105,7 @@ def test_double_hashing_delete_requery_concurrent():
 
 
 def test_single_slot_scheme_and_unsupported_ops():
-    desc = pg.kv_desc(1 << 10, bucket_size=1, tile=4, prober=pg.PROBE_DOUBLE, max_probes=5,
+    desc = pg.kv_desc(1 << 10, bucket_size=4, tile=4, prober=pg.PROBE_DOUBLE, max_probes=5,
                       scheme=pg.SCHEME_SINGLE_SLOT)
     o, g = pair(desc)
     ops = Ops(g)
