@@ -49,6 +49,11 @@ struct Ring {
         live.pop_front();
         if (live.empty()) head = 0;
     }
+    // Undo the newest alloc() (head always ends the newest live block).
+    void release_back() {
+        live.pop_back();
+        head = live.empty() ? 0 : live.back().second;
+    }
 };
 
 }  // namespace pg

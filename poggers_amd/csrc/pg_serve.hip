@@ -102,7 +102,7 @@ __device__ inline void serve_one(const TableDev &T, const ReqDev &r, uint64_t j,
         }
         return;
     }
-    const bool fast16 = T.lv[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT && T.lv[0].B == 16;
+    const bool fast16 = T.lv[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT && T.lv[0].B == 16 && T.lv[0].max_probes <= 2;
     if (r.op == PG_OP_INSERT) {
         const uint32_t v = r.vals ? ((const uint16_t *)r.vals)[j] : 0u;
         const int st = fast16 ? tcf_insert_key<P, 16>(T, key, v, false, t) : tcf_insert_key<P, 0>(T, key, v, false, t);
@@ -278,6 +278,10 @@ struct pg_server_s {
     std::vector<uint8_t> scratch_access;   // per table, while a batch is formed
     std::vector<uint8_t> scratch_blocked;  // per table, while a batch is formed
     pg::Plan plan;                         // the batch being formed
+    // Set when a HIP call failed after a batch's kernel was enqueued: that
+    // batch may have been applied, so its requests can be neither retried
+    // nor reported, and every later call but destroy returns PG_ERR_HIP.
+    bool failed = false;
 };
 
 extern "C" {
@@ -355,6 +359,7 @@ int pg_server_destroy(pg_server s) {
 int pg_server_submit(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,
                      uint64_t n, void *h_vals_out, uint8_t *h_flags, uint64_t *ticket) {
     if (!s || table >= s->handles.size() || op > PG_OP_REMOVE || (n && !h_keys)) return PG_ERR_INVALID;
+    if (s->failed) return PG_ERR_HIP;
     const pg::TableDev &T = s->h_tables[table];
     if (op == PG_OP_REMOVE)
         for (uint32_t l = 0; l < T.nlevels; l++)
@@ -418,6 +423,19 @@ static int launch_batch(pg_server s, uint32_t si) {
     uint64_t at = 0;
     if (!s->ring.alloc(fixed + meta + need, at)) return PG_OK;
     hipStream_t st = s->streams[si];
+    // A failure before the kernel is enqueued applied nothing: let copies
+    // already enqueued on the stream finish, return the staging block and
+    // leave the requests queued.  Once the kernel is enqueued the batch may
+    // have been applied: the server then fails for good (see `failed`).
+    auto undo = [&](int rc) {
+        (void)hipStreamSynchronize(st);
+        s->ring.release_back();
+        return rc;
+    };
+    auto fail = [&](int rc) {
+        s->failed = true;
+        return rc;
+    };
     if (ordered) {
         // earlier batches still in flight that touch a table of this batch
         // in a conflicting way finish (on the device) before this one starts
@@ -429,7 +447,7 @@ static int launch_batch(pg_server s, uint32_t si) {
                     if (x.first == y.first && pg::conflicts(x.second, y.second)) dep = true;
                 if (dep) break;
             }
-            if (dep && hipStreamWaitEvent(st, e.done, 0) != hipSuccess) return PG_ERR_HIP;
+            if (dep && hipStreamWaitEvent(st, e.done, 0) != hipSuccess) return undo(PG_ERR_HIP);
         }
     }
     char *base = s->arena + at;
@@ -455,12 +473,12 @@ static int launch_batch(pg_server s, uint32_t si) {
             d = pg::ReqDev{};
             d.keys = (const uint64_t *)p;
             if (r.n && hipMemcpyAsync(p, r.h_keys, r.n * 8, hipMemcpyHostToDevice, st) != hipSuccess)
-                return PG_ERR_HIP;
+                return undo(PG_ERR_HIP);
             p += al(r.n * 8);
             if (r.op == PG_OP_INSERT && r.h_vals) {
                 d.vals = p;
                 if (r.n && hipMemcpyAsync(p, r.h_vals, r.n * r.vbytes, hipMemcpyHostToDevice, st) != hipSuccess)
-                    return PG_ERR_HIP;
+                    return undo(PG_ERR_HIP);
                 p += al(r.n * r.vbytes);
             }
             if (r.h_flags) {
@@ -495,7 +513,7 @@ static int launch_batch(pg_server s, uint32_t si) {
         }
         meta_copy = idx_at + 4 * nidx;
     }
-    if (hipMemcpyAsync(d_off, hm, meta_copy, hipMemcpyHostToDevice, st) != hipSuccess) return PG_ERR_HIP;
+    if (hipMemcpyAsync(d_off, hm, meta_copy, hipMemcpyHostToDevice, st) != hipSuccess) return undo(PG_ERR_HIP);
     if (total) {
         pg::ServeArgs a{};
         a.tables = s->d_tables;
@@ -505,7 +523,7 @@ static int launch_batch(pg_server s, uint32_t si) {
         a.nreq = (uint32_t)take;
         a.total = total;
         const int rc = pg::launch_serve(s->P, a, st);
-        if (rc != PG_OK) return rc;
+        if (rc != PG_OK) return fail(rc);
     }
     for (size_t i = 0; i < take; i++) {
         const ServeReq &r = s->queue[pick[i]];
@@ -513,13 +531,16 @@ static int launch_batch(pg_server s, uint32_t si) {
         const pg::ReqDev &d = hd[i];
         if (r.h_flags && r.n &&
             hipMemcpyAsync(r.h_flags, d.flags, r.n, hipMemcpyDeviceToHost, st) != hipSuccess)
-            return PG_ERR_HIP;
+            return fail(PG_ERR_HIP);
         if (d.vals_out && r.n &&
             hipMemcpyAsync(r.h_vals_out, d.vals_out, r.n * r.vbytes, hipMemcpyDeviceToHost, st) != hipSuccess)
-            return PG_ERR_HIP;
+            return fail(PG_ERR_HIP);
     }
-    if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) return PG_ERR_HIP;
-    if (hipEventRecord(b.done, st) != hipSuccess) return PG_ERR_HIP;
+    if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) return fail(PG_ERR_HIP);
+    if (hipEventRecord(b.done, st) != hipSuccess) {
+        (void)hipEventDestroy(b.done);
+        return fail(PG_ERR_HIP);
+    }
     if (!ordered) {
         s->queue.erase(s->queue.begin(), s->queue.begin() + (long)take);
     } else {  // remove the picked positions (ascending), keep the rest in order
@@ -537,7 +558,9 @@ static int launch_batch(pg_server s, uint32_t si) {
 }
 
 int pg_server_progress(pg_server s, uint64_t *done, uint32_t cap, uint32_t *ndone) {
+    if (ndone) *ndone = 0;
     if (!s) return PG_ERR_INVALID;
+    if (s->failed) return PG_ERR_HIP;
     uint32_t nd = 0;
     // retire: report a batch as soon as it finishes (any order); free its
     // staging memory in launch order
@@ -602,6 +625,7 @@ int pg_server_pending(pg_server s, uint64_t *queued, uint64_t *inflight) {
 int pg_server_run_plain(pg_server s, uint32_t table, uint32_t op, const uint64_t *h_keys, const void *h_vals,
                         uint64_t n, void *h_vals_out, uint8_t *h_flags) {
     if (!s || table >= s->handles.size() || op > PG_OP_REMOVE || (n && !h_keys)) return PG_ERR_INVALID;
+    if (s->failed) return PG_ERR_HIP;
     if (!s->inflight.empty() || !s->queue.empty()) return PG_ERR_INVALID;  // not while batches run
     const pg::TableDev &T = s->h_tables[table];
     const uint32_t vb = T.rep == PG_REP_KV64 ? 8u : 2u;

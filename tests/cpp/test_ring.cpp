@@ -1,8 +1,10 @@
 // CPU test of the serving staging ring (poggers_amd/csrc/pg_ring.hpp): random
-// allocation / in-order release sequences against a byte-occupancy model.
-// Checks: blocks never overlap and stay inside the arena, largest_free()
-// equals what alloc() can actually place, and an allocation that fits some
-// free run at the head or at the arena start never fails.
+// allocation / in-order release sequences, with some allocations undone at
+// once (release_back, a launch that failed before its kernel), against a
+// byte-occupancy model.  Checks: blocks never overlap and stay inside the
+// arena, largest_free() equals what alloc() can actually place, an undo
+// restores the exact state before its allocation, and an allocation that
+// fits some free run at the head or at the arena start never fails.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -37,6 +39,14 @@ int main() {
                 CHECK(ok == (need <= lf));
                 if (!ok) continue;
                 CHECK(at + need <= cap);
+                if (rng() % 10 == 0) {  // the launch failed: undo this block
+                    r.release_back();
+                    CHECK(r.largest_free() == lf);
+                    uint64_t at2 = 0;
+                    CHECK(r.alloc(need, at2) && at2 == at);  // the same block is found again
+                    r.release_back();
+                    continue;
+                }
                 for (uint64_t b = at; b < at + need; b++) {
                     CHECK(owner[b] == -1);
                     owner[b] = next_id;
