@@ -112,7 +112,8 @@ enum pg_rep {
 
 /* Insert scheme of a level.
  *   BUCKET        insert_schemes/bucket_insert.cuh:38-361
- *   SINGLE_SLOT   insert_schemes/single_slot_insert.cuh:38-233 (no remove)
+ *   SINGLE_SLOT   insert_schemes/single_slot_insert.cuh:38-233 (no remove;
+ *                 bucket_size must equal tile, as its static_assert at :43)
  *   POWER_OF_N_SHORTCUT  insert_schemes/power_of_n_shortcut_buckets.cuh:35-783
  *                 (the TCF primary: two choices + 75 % shortcut)
  *   LINEAR_BUCKETS insert_schemes/linear_insert_buckets.cuh:35-445
@@ -189,12 +190,13 @@ typedef struct pg_level_desc {
     uint32_t reserved;     /* 0 */
 } pg_level_desc;
 
-/* A table: level 0 plus an optional backing level (Is_Recursive /
- * Recursive_Type, tables/bucketed_table.cuh:72,166-176).  The backing level
- * is seeded with seed*seed (bucketed_table.cuh:170).                     */
+/* A table: level 0 plus up to three backing levels (Is_Recursive /
+ * Recursive_Type, tables/bucketed_table.cuh:72,166-176).  Each backing level
+ * is seeded with the square of the level above's seed
+ * (bucketed_table.cuh:170).                                               */
 typedef struct pg_desc {
     uint32_t rep;          /* enum pg_rep */
-    uint32_t num_levels;   /* 1 or 2 */
+    uint32_t num_levels;   /* 1 .. PG_MAX_LEVELS */
     pg_level_desc level[PG_MAX_LEVELS];
 } pg_desc;
 
@@ -430,10 +432,13 @@ int pg_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *d_out, void
 
 /* Random 64 B-sector microbenchmark (the measured random-access roofline):
  * n_reads independent 64 B reads at hashed sector indices of d_buf (bytes, a
- * multiple of 64).  `op` bit 0 adds a CAS on one word of each sector (the
- * insert pattern: read, then dirty the line); bits 4-7 = log2 sectors in
- * flight per tile (0 -> 4); bit 8 = nontemporal loads; bits 9-10 = tile
- * shape (0: 4 lanes x 16 B, 1: 1 lane x 64 B, 2: 16 lanes x 4 B).  Writes a
+ * multiple of 64).  `op` bits 0-1: 0 read only, 1 add a CAS on one word of
+ * each sector (the insert pattern: read, then dirty the line), 2 add a plain
+ * store of that word instead; bits 4-7 = v, 1 << (v - 1) sectors in flight
+ * per tile for v in 1..4 (0 -> 4); bit 8 = nontemporal loads; bits 9-10 =
+ * tile shape (0: 4 lanes x 16 B, 1: 1 lane x 64 B, 2: 16 lanes x 4 B); bits
+ * 11-14 = w, 0 for a grid covering the work, else w blocks of 256 threads
+ * per CU walking it grid-stride (caps the requests in flight).  Writes a
  * checksum word to d_sink so nothing is optimised away. */
 int pg_random_sector_bench(void *d_buf, uint64_t bytes, uint64_t n_reads, uint32_t op,
                            uint64_t seed, uint64_t *d_sink, void *stream);

@@ -167,6 +167,46 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def value_bytes(rep: int) -> int:
+    """Bytes per element of a representation's value buffers (vals, vals_out)."""
+    if rep == REP_KV64:
+        return 8
+    if rep in (REP_FP8_V8, REP_FP10_V6):
+        return 1
+    return 2
+
+
+def _arg(t, what: str, n: int, itemsize: int, device: bool = True):
+    """Check one tensor argument of a bulk call before its pointer reaches a
+    kernel: a torch tensor, contiguous, `itemsize` bytes per element, at least
+    `n` elements, and on the GPU when `device` (else on the GPU or in host
+    memory).  Returns its pointer (None for None)."""
+    if t is None:
+        return None
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what}: expected a torch tensor, got {type(t).__name__}")
+    if device and not t.is_cuda:
+        raise ValueError(f"{what}: expected a GPU tensor, got one on {t.device}")
+    if not device and t.device.type not in ("cuda", "cpu"):
+        raise ValueError(f"{what}: expected a GPU or host tensor, got one on {t.device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: expected a contiguous tensor")
+    if t.element_size() != itemsize:
+        raise ValueError(f"{what}: expected {itemsize}-byte elements, got {t.dtype}")
+    if t.numel() < n:
+        raise ValueError(f"{what}: holds {t.numel()} elements, the call needs {n}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _keys(keys, device: bool = True):
+    """Pointer and count of a key tensor (8-byte elements)."""
+    p = _arg(keys, "keys", 0, 8, device)
+    if p is None:
+        raise ValueError("keys: a tensor is required")
+    return p, keys.numel()
+
+
 def _stream(stream):
     if stream is None:
         import torch
@@ -303,10 +343,14 @@ class Table:
     def clear(self, stream=None):
         check(lib().pg_clear(self._h, _stream(stream)), "pg_clear")
 
+    def _vb(self):
+        return value_bytes(self.desc.rep)
+
     def bulk_insert(self, keys, vals=None, status=None, fail_count=None, stream=None, with_delete=False):
         fn = lib().pg_bulk_insert_with_delete if with_delete else lib().pg_bulk_insert
-        check(fn(self._h, _ptr(keys), _ptr(vals), keys.numel(), _ptr(status), _ptr(fail_count),
-                 _stream(stream)), "pg_bulk_insert")
+        k, n = _keys(keys)
+        check(fn(self._h, k, _arg(vals, "vals", n, self._vb()), n, _arg(status, "status", n, 1),
+                 _arg(fail_count, "fail_count", 1, 8), _stream(stream)), "pg_bulk_insert")
 
     def bulk_insert_with_delete(self, keys, vals=None, status=None, fail_count=None, stream=None):
         self.bulk_insert(keys, vals, status, fail_count, stream, with_delete=True)
@@ -314,21 +358,26 @@ class Table:
     def bulk_insert_if_not_exists(self, keys, vals=None, vals_out=None, found=None, status=None,
                                   fail_count=None, stream=None, delete=False):
         fn = lib().pg_bulk_insert_if_not_exists_delete if delete else lib().pg_bulk_insert_if_not_exists
-        check(fn(self._h, _ptr(keys), _ptr(vals), keys.numel(), _ptr(vals_out), _ptr(found), _ptr(status),
-                 _ptr(fail_count), _stream(stream)), "pg_bulk_insert_if_not_exists")
+        k, n = _keys(keys)
+        vb = self._vb()
+        check(fn(self._h, k, _arg(vals, "vals", n, vb), n, _arg(vals_out, "vals_out", n, vb),
+                 _arg(found, "found", n, 1), _arg(status, "status", n, 1), _arg(fail_count, "fail_count", 1, 8),
+                 _stream(stream)), "pg_bulk_insert_if_not_exists")
 
     def bulk_query(self, keys, vals_out=None, found=None, stream=None):
-        check(lib().pg_bulk_query(self._h, _ptr(keys), keys.numel(), _ptr(vals_out), _ptr(found),
-                                  _stream(stream)), "pg_bulk_query")
+        k, n = _keys(keys)
+        check(lib().pg_bulk_query(self._h, k, n, _arg(vals_out, "vals_out", n, self._vb()),
+                                  _arg(found, "found", n, 1), _stream(stream)), "pg_bulk_query")
 
     def bulk_remove(self, keys, found=None, stream=None):
-        check(lib().pg_bulk_remove(self._h, _ptr(keys), keys.numel(), _ptr(found), _stream(stream)),
+        k, n = _keys(keys)
+        check(lib().pg_bulk_remove(self._h, k, n, _arg(found, "found", n, 1), _stream(stream)),
               "pg_bulk_remove")
 
     def set_stats(self, stats=None):
         """Accumulate algorithmic counters into a 3-element int64 CUDA tensor
         (None disables)."""
-        check(lib().pg_set_stats(self._h, _ptr(stats)), "pg_set_stats")
+        check(lib().pg_set_stats(self._h, _arg(stats, "stats", 3, 8)), "pg_set_stats")
 
     def set_items(self, insert_items: int = 0, query_items: int = 0):
         """Keys per tile for concurrent inserts / queries (1, 2, 4; 0 = the
@@ -434,18 +483,33 @@ class Batch:
     def clear(self, stream=None):
         check(lib().pg_batch_clear(self._h, _stream(stream)), "pg_batch_clear")
 
+    def _offsets(self, offsets, what):
+        p = _arg(offsets, what, self.ntables + 1, 8)
+        if p is None:
+            raise ValueError(f"{what}: a tensor is required")
+        return p
+
     def insert(self, keys, offsets, vals=None, status=None, fail_count=None, stream=None):
-        check(lib().pg_batch_insert(self._h, _ptr(keys), _ptr(offsets), keys.numel(), _ptr(vals), _ptr(status),
-                                    _ptr(fail_count), _stream(stream)), "pg_batch_insert")
+        k, n = _keys(keys)
+        vb = value_bytes(self.proto.rep)
+        check(lib().pg_batch_insert(self._h, k, self._offsets(offsets, "offsets"), n, _arg(vals, "vals", n, vb),
+                                    _arg(status, "status", n, 1), _arg(fail_count, "fail_count", 1, 8),
+                                    _stream(stream)), "pg_batch_insert")
 
     def query(self, keys, offsets, vals_out=None, found=None, stream=None):
-        check(lib().pg_batch_query(self._h, _ptr(keys), _ptr(offsets), keys.numel(), _ptr(vals_out), _ptr(found),
+        k, n = _keys(keys)
+        vb = value_bytes(self.proto.rep)
+        check(lib().pg_batch_query(self._h, k, self._offsets(offsets, "offsets"), n,
+                                   _arg(vals_out, "vals_out", n, vb), _arg(found, "found", n, 1),
                                    _stream(stream)), "pg_batch_query")
 
     def capture(self, ins_keys, ins_offsets, q_keys, q_offsets, status=None, found=None, fail_count=None,
                 clear_first=True, stream=None):
-        check(lib().pg_batch_capture(self._h, _ptr(ins_keys), _ptr(ins_offsets), ins_keys.numel(), _ptr(q_keys),
-                                     _ptr(q_offsets), q_keys.numel(), _ptr(status), _ptr(found), _ptr(fail_count),
+        ik, ni = _keys(ins_keys)
+        qk, nq = _keys(q_keys)
+        check(lib().pg_batch_capture(self._h, ik, self._offsets(ins_offsets, "ins_offsets"), ni, qk,
+                                     self._offsets(q_offsets, "q_offsets"), nq, _arg(status, "status", ni, 1),
+                                     _arg(found, "found", nq, 1), _arg(fail_count, "fail_count", 1, 8),
                                      int(clear_first), _stream(stream)), "pg_batch_capture")
 
     def replay(self, stream=None):
@@ -496,24 +560,44 @@ class Server:
         except Exception:
             pass
 
+    @staticmethod
+    def _request(tables, table, op, keys, vals, vals_out, flags):
+        """Pointers of one request's tensors, checked (GPU or host memory,
+        contiguous, element widths of the table's representation when
+        `tables` is known, at least as many elements as keys)."""
+        k, n = _keys(keys, device=False)
+        if tables is not None:
+            if not 0 <= table < len(tables):
+                raise ValueError(f"table {table} out of range")
+            vb = value_bytes(tables[table].desc.rep)
+        else:
+            vb = None
+        def val(t, what):
+            if t is None:
+                return None
+            return _arg(t, what, n, vb if vb is not None else t.element_size(), device=False)
+        return k, n, val(vals, "vals"), val(vals_out, "vals_out"), _arg(flags, "flags", n, 1, device=False)
+
     def submit(self, table: int, op: int, keys, vals=None, vals_out=None, flags=None) -> int:
         t = ctypes.c_uint64()
-        check(lib().pg_server_submit(self._h, table, op, _ptr(keys), _ptr(vals), keys.numel(), _ptr(vals_out),
-                                     _ptr(flags), ctypes.byref(t)), "pg_server_submit")
+        k, n, v, o, f = self._request(self.tables, table, op, keys, vals, vals_out, flags)
+        check(lib().pg_server_submit(self._h, table, op, k, v, n, o, f, ctypes.byref(t)), "pg_server_submit")
         return t.value
 
     @staticmethod
-    def pack(requests):
+    def pack(requests, tables=None):
         """A pg_request array for submit_many from (table, op, keys, vals,
-        vals_out, flags) tuples of tensors (None where absent)."""
+        vals_out, flags) tuples of tensors (None where absent).  Each tensor is
+        checked; pass the server's `tables` to check value widths too."""
         arr = (Request * len(requests))()
         for i, (table, op, keys, vals, out, flags) in enumerate(requests):
+            k, n, v, o, f = Server._request(tables, table, op, keys, vals, out, flags)
             r = arr[i]
-            r.table, r.op, r.n = table, op, keys.numel()
-            r.keys = keys.data_ptr()
-            r.vals = vals.data_ptr() if vals is not None else None
-            r.vals_out = out.data_ptr() if out is not None else None
-            r.flags = flags.data_ptr() if flags is not None else None
+            r.table, r.op, r.n = table, op, n
+            r.keys = k.value
+            r.vals = v.value if v is not None else None
+            r.vals_out = o.value if o is not None else None
+            r.flags = f.value if f is not None else None
         return arr
 
     def submit_many(self, packed, start: int = 0, count: int | None = None):
@@ -552,8 +636,8 @@ class Server:
         return q.value, f.value
 
     def run_plain(self, table: int, op: int, keys, vals=None, vals_out=None, flags=None):
-        check(lib().pg_server_run_plain(self._h, table, op, _ptr(keys), _ptr(vals), keys.numel(), _ptr(vals_out),
-                                        _ptr(flags)), "pg_server_run_plain")
+        k, n, v, o, f = self._request(self.tables, table, op, keys, vals, vals_out, flags)
+        check(lib().pg_server_run_plain(self._h, table, op, k, v, n, o, f), "pg_server_run_plain")
 
 
 _hip = None
@@ -584,12 +668,13 @@ def file_info(path: str):
 
 def gen_keys(seed: int, start: int, n: int, out, stream=None):
     """Fill ``out`` (int64 CUDA tensor) with splitmix64 keys start..start+n-1."""
-    check(lib().pg_gen_keys(seed, start, n, _ptr(out), _stream(stream)), "pg_gen_keys")
+    check(lib().pg_gen_keys(seed, start, n, _arg(out, "out", n, 8), _stream(stream)), "pg_gen_keys")
 
 
 def random_sector_bench(buf, n_reads: int, op: int, seed: int, sink, stream=None):
-    check(lib().pg_random_sector_bench(_ptr(buf), buf.numel() * buf.element_size(), n_reads, op, seed,
-                                       _ptr(sink), _stream(stream)), "pg_random_sector_bench")
+    p = _arg(buf, "buf", 1, buf.element_size())
+    check(lib().pg_random_sector_bench(p, buf.numel() * buf.element_size(), n_reads, op, seed,
+                                       _arg(sink, "sink", 1, 8), _stream(stream)), "pg_random_sector_bench")
 
 
 def fastmod_host(n: int, d: int) -> int:

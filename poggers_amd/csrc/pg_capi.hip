@@ -42,6 +42,9 @@ int check_level(uint32_t rep, const pg_level_desc &l, uint32_t index) {
         const bool pow = l.scheme == PG_SCHEME_POWER_OF_N || l.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
         if (l.scheme != PG_SCHEME_BUCKET && l.scheme != PG_SCHEME_SINGLE_SLOT && !pow) return PG_ERR_UNSUPPORTED;
         if (l.scheme != PG_SCHEME_SINGLE_SLOT && (l.bucket_size == 0 || l.bucket_size > 64)) return PG_ERR_INVALID;
+        // single_slot_insert static_asserts Partition_Size == Bucket_Size
+        // (single_slot_insert.cuh:43); its slots are probed one by one
+        if (l.scheme == PG_SCHEME_SINGLE_SLOT && l.bucket_size != l.tile) return PG_ERR_INVALID;
         // the per-probe fills live in registers (power_of_n.cuh:336-337
         // keeps Max_Probes of them)
         if (pow && l.max_probes > pg::kMaxShortcutProbes) return PG_ERR_UNSUPPORTED;
@@ -528,7 +531,7 @@ int pg_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *d_out, void
 
 int pg_random_sector_bench(void *d_buf, uint64_t bytes, uint64_t n_reads, uint32_t op, uint64_t seed,
                            uint64_t *d_sink, void *stream) {
-    if (!d_buf || !d_sink || bytes < 64 || (bytes % 64) || op >= (1u << 11)) return PG_ERR_INVALID;
+    if (!d_buf || !d_sink || bytes < 64 || (bytes % 64) || op >= (1u << 15)) return PG_ERR_INVALID;
     return pg::launch_random_sector(d_buf, bytes, n_reads, op, seed, d_sink, (hipStream_t)stream);
 }
 
