@@ -11,7 +11,9 @@ Prints one JSON line (the bench contract).  `value` is whole-step throughput
 in Mops/s (inserts + queries per second).  `roofline` is for the dominant
 kernel (insert), with algorithmic bytes counted by the kernels' own
 accounting counters in an untimed pass; `cpu_baseline` times the CPU oracle
-on a bounded sample of the same workload.
+on a bounded sample of the same workload.  `serving` is the served request
+loop at one operating point: 64 resident tables, requests of 1K-64K keys,
+Poisson arrivals, against per-request plain launches.
 """
 from __future__ import annotations
 
@@ -38,6 +40,8 @@ def parse():
                     help="query launch shape: keys per tile (1, 2, 4; 0 = library default)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-roofline", action="store_true", help="skip the random-sector microbenchmark")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the serving leg of the default line (64 resident tables, 1K-64K-key requests)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB streaming read+write kernel and 2^24 random 64 B reads "
                          "of a 1 GiB buffer (known byte counts that scale the PMC byte counters)")
@@ -225,86 +229,91 @@ def run_flat(args):
     print(json.dumps(out_line), flush=True)
 
 
-def run_serve(args):
-    """64 resident tables (32 TCFs of 2^26-2^28 slots, 32 double-hashing KV
-    tables of 2^24-2^27 slots, tile 4), each 40 % full.  Requests: op mix 60 %
-    query / 30 % insert / 10 % delete, table Zipf(1.1), keys in pinned host
-    memory.  For request sizes log-uniform in [1K, 4M] and in [1K, 64K]:
-    saturated throughput of the served loop (all requests queued at once)
-    against per-request plain launches (copies, one kernel, wait), and
-    p50/p99 latency under Poisson arrivals at half the served throughput;
-    then both again with per-table request ordering."""
-    import numpy as np
-    import torch
-    import poggers_amd as pg
-    from poggers_amd.workloads import keys_at
+class ServeWorkload:
+    """The serving workload: 64 resident tables (32 TCFs of 2^26-2^28 slots,
+    32 double-hashing KV tables of 2^24-2^27 slots, tile 4), each 40 % full;
+    requests with an op mix of 60 % query / 30 % insert / 10 % delete, the
+    table of each chosen Zipf(1.1), keys in pinned host memory (the server
+    uses them in place)."""
 
-    torch.cuda.set_device(0)
-    dev = torch.device("cuda", 0)
-    rng = np.random.default_rng(2024)
-    log = lambda m: print(m, file=sys.stderr, flush=True)
-    tcf_sizes = np.exp(rng.uniform(np.log(2.0 ** 26), np.log(2.0 ** 28), 32)).astype(np.int64)
-    kv_sizes = np.exp(rng.uniform(np.log(2.0 ** 24), np.log(2.0 ** 27), 32)).astype(np.int64)
-    specs = [("tcf", int(s)) for s in tcf_sizes] + [("kv", int(s)) for s in kv_sizes]
-    # Zipf ranks go to the tables with the most free room (largest first)
-    specs.sort(key=lambda x: -(x[1] * (4 if x[0] == "tcf" else 1)))
-    tables, npop = [], []
-    tmp = torch.empty(1 << 27, dtype=torch.int64, device=dev)
-    for i, (kind, s) in enumerate(specs):
-        if kind == "tcf":
-            t = pg.Table(pg.tcf_desc(s, tile=4))
-        else:
-            t = pg.Table(pg.kv_desc(s, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20))
-        n = int(0.4 * s)
-        for a in range(0, n, 1 << 27):
-            m = min(1 << 27, n - a)
-            pg.gen_keys(1000 + i, a, m, tmp[:m])
-            t.bulk_insert(tmp[:m], (tmp[:m] ^ 0x5A5A) if kind == "kv" else None)
-        tables.append(t)
-        npop.append(n)
-    del tmp
-    torch.cuda.synchronize()
-    total_bytes = sum(t.host_bytes_in_use() for t in tables)
-    log(f"serve: {len(tables)} tables, {total_bytes / 2**30:.1f} GiB resident")
+    def __init__(self, log):
+        import numpy as np
+        import torch
+        import poggers_amd as pg
+        self.np, self.torch, self.pg = np, torch, pg
+        dev = torch.device("cuda", 0)
+        rng = np.random.default_rng(2024)
+        tcf_sizes = np.exp(rng.uniform(np.log(2.0 ** 26), np.log(2.0 ** 28), 32)).astype(np.int64)
+        kv_sizes = np.exp(rng.uniform(np.log(2.0 ** 24), np.log(2.0 ** 27), 32)).astype(np.int64)
+        specs = [("tcf", int(s)) for s in tcf_sizes] + [("kv", int(s)) for s in kv_sizes]
+        # Zipf ranks go to the tables with the most free room (largest first)
+        specs.sort(key=lambda x: -(x[1] * (4 if x[0] == "tcf" else 1)))
+        self.specs = specs
+        self.tables, self.npop = [], []
+        tmp = torch.empty(1 << 27, dtype=torch.int64, device=dev)
+        for i, (kind, s) in enumerate(specs):
+            if kind == "tcf":
+                t = pg.Table(pg.tcf_desc(s, tile=4))
+            else:
+                t = pg.Table(pg.kv_desc(s, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE, max_probes=20))
+            n = int(0.4 * s)
+            for a in range(0, n, 1 << 27):
+                m = min(1 << 27, n - a)
+                pg.gen_keys(1000 + i, a, m, tmp[:m])
+                t.bulk_insert(tmp[:m], (tmp[:m] ^ 0x5A5A) if kind == "kv" else None)
+            self.tables.append(t)
+            self.npop.append(n)
+        del tmp
+        torch.cuda.synchronize()
+        self.total_bytes = sum(t.host_bytes_in_use() for t in self.tables)
+        self.fresh = [0] * len(self.tables)
+        w = 1.0 / np.arange(1, len(self.tables) + 1) ** 1.1
+        self.zipf_w = w / w.sum()
+        log(f"serve: {len(self.tables)} tables, {self.total_bytes / 2**30:.1f} GiB resident")
 
-    zipf_w = 1.0 / np.arange(1, len(tables) + 1) ** 1.1
-    zipf_w /= zipf_w.sum()
+    def describe(self):
+        return (f"64 resident tables (32 TCF 2^26-2^28 slots, 32 KV 2^24-2^27 slots; "
+                f"{self.total_bytes / 2**30:.1f} GiB), 40% full; op mix 60/30/10 query/insert/delete, "
+                f"tables Zipf(1.1), keys in pinned host memory used in place; 4 streams")
 
-    def make(nreq, lo, hi, seed):
+    def make(self, nreq, lo, hi, seed):
+        """nreq requests of log-uniform [lo, hi] keys: (table, op, keys, vals, out, flags, n)."""
+        np, torch, pg = self.np, self.torch, self.pg
+        from poggers_amd.workloads import keys_at
         r = np.random.default_rng(seed)
         reqs = []
-        fresh = [0] * len(tables)
-        for q in range(nreq):
-            t = int(r.choice(len(tables), p=zipf_w))
+        for _ in range(nreq):
+            t = int(r.choice(len(self.tables), p=self.zipf_w))
             n = int(np.exp(r.uniform(np.log(lo), np.log(hi))))
             u = r.random()
             op = pg.OP_QUERY if u < 0.6 else (pg.OP_INSERT if u < 0.9 else pg.OP_REMOVE)
             if op == pg.OP_INSERT:
-                keys = keys_at(5000 + t, np.arange(fresh[t], fresh[t] + n) + (1 << 40))
-                fresh[t] += n
+                keys = keys_at(5000 + t, np.arange(self.fresh[t], self.fresh[t] + n) + (1 << 40))
+                self.fresh[t] += n
             else:
-                keys = keys_at(1000 + t, r.integers(0, npop[t], n))
+                keys = keys_at(1000 + t, r.integers(0, self.npop[t], n))
             kt = torch.empty(n, dtype=torch.int64, pin_memory=True)
             kt.numpy()[:] = keys.view(np.int64)
-            kv = specs[t][0] == "kv"
+            kv = self.specs[t][0] == "kv"
             vals = out = None
             if op == pg.OP_INSERT:
                 vals = torch.empty(n, dtype=torch.int64 if kv else torch.int16, pin_memory=True)
-                vals.numpy()[:] = (keys ^ np.uint64(0x5A5A)).view(np.int64) if kv else (keys & np.uint64(0xFFFF)).astype(np.uint16).view(np.int16)
+                vals.numpy()[:] = ((keys ^ np.uint64(0x5A5A)).view(np.int64) if kv
+                                   else (keys & np.uint64(0xFFFF)).astype(np.uint16).view(np.int16))
             if op == pg.OP_QUERY:
                 out = torch.empty(n, dtype=torch.int64 if kv else torch.int16, pin_memory=True)
             flags = torch.empty(n, dtype=torch.uint8, pin_memory=True)
             reqs.append((t, op, kt, vals, out, flags, n))
         return reqs
 
-    def packed_of(reqs):
-        return pg.Server.pack([(t, op, kt, vals, out, flags) for (t, op, kt, vals, out, flags, n) in reqs])
+    def packed_of(self, reqs):
+        return self.pg.Server.pack([(t, op, kt, vals, out, flags) for (t, op, kt, vals, out, flags, n) in reqs])
 
-    def saturated(srv, reqs):
+    def saturated(self, srv, reqs):
         """All requests offered at once, handed over 64 at a time while the
-        engine runs (submission overlaps execution)."""
-        packed = packed_of(reqs)
-        torch.cuda.synchronize()
+        engine runs (submission overlaps execution).  Returns (s, keys)."""
+        packed = self.packed_of(reqs)
+        self.torch.cuda.synchronize()
         t0 = time.perf_counter()
         done = 0
         for s0 in range(0, len(reqs), 64):
@@ -312,24 +321,28 @@ def run_serve(args):
             done += len(srv.progress())
         while done < len(reqs):
             done += len(srv.progress())
-        t1 = time.perf_counter()
-        return (t1 - t0), sum(r[6] for r in reqs)
+        return time.perf_counter() - t0, sum(r[6] for r in reqs)
 
+    @staticmethod
     def plain(srv, reqs):
+        """Per-request plain launches: one kernel on the request's buffers, then a wait."""
         t0 = time.perf_counter()
         for (t, op, kt, vals, out, flags, n) in reqs:
             srv.run_plain(t, op, kt, vals, out, flags)
         return time.perf_counter() - t0, sum(r[6] for r in reqs)
 
-    def poisson(srv, reqs, rate, seed):
-        """Open-loop arrivals; each engine step first hands over every request
-        that has arrived."""
+    def poisson(self, srv, reqs, rate, seed):
+        """Open-loop Poisson arrivals at `rate` requests/s; each engine step first
+        hands over every request that has arrived.  Returns (latencies in s,
+        seconds from the first arrival to the last completion)."""
+        np = self.np
         r = np.random.default_rng(seed)
         arrive = np.cumsum(r.exponential(1.0 / rate, len(reqs)))
         lat = np.zeros(len(reqs))
-        packed = packed_of(reqs)
+        packed = self.packed_of(reqs)
         tick2i = {}
         i = done = 0
+        last = 0.0
         t0 = time.perf_counter()
         while done < len(reqs):
             now = time.perf_counter() - t0
@@ -340,25 +353,85 @@ def run_serve(args):
                 i = j
             fin = srv.progress()
             if fin:
-                now = time.perf_counter() - t0
+                last = time.perf_counter() - t0
                 for tk in fin:
-                    j = tick2i[tk]
-                    lat[j] = now - arrive[j]
+                    q = tick2i[tk]
+                    lat[q] = last - arrive[q]
                 done += len(fin)
-        return lat
+        return lat, last - arrive[0]
 
+
+def serving_point(log, nreq=4000, lo=1000, hi=64 << 10):
+    """The serving leg of the default bench line, on the serving workload with
+    requests of 1K-64K keys: per-request plain launches, the saturated served
+    rate, and open-loop Poisson arrivals at fractions of the saturated request
+    rate.  The operating point is the highest offered rate whose p99 latency
+    is at most 2 ms; its served rate (keys completed from the first arrival to
+    the last completion) is compared with the plain launches."""
+    import numpy as np
+    import poggers_amd as pg
+    w = ServeWorkload(log)
+    srv = pg.Server(w.tables, budget_bytes=4 << 30, nstreams=4)
+    reqs = w.make(nreq, lo, hi, 7 + hi)
+    sizes = np.array([r[6] for r in reqs])
+    w.saturated(srv, reqs[: nreq // 10])  # warm up
+    ts, keys = w.saturated(srv, reqs)
+    tp, _ = w.plain(srv, reqs)
+    plain_mops = keys / tp / 1e6
+    sat_rate = len(reqs) / ts
+    sweep = []
+    for frac in (0.5, 0.7, 0.85, 1.0):
+        lat, span = w.poisson(srv, reqs, frac * sat_rate, 11)
+        served = keys / span / 1e6
+        sweep.append({"offered_frac_of_saturated": frac, "rate_req_per_s": round(frac * sat_rate, 1),
+                      "served_mops": round(served, 1), "speedup_vs_plain": round(served / plain_mops, 2),
+                      "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                      "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3)})
+        log(f"serve poisson {frac}: {sweep[-1]}")
+    ok = [p for p in sweep if p["p99_ms"] <= 2.0]
+    point = max(ok, key=lambda p: p["served_mops"]) if ok else min(sweep, key=lambda p: p["p99_ms"])
+    srv.free()
+    out = {
+        "workload": w.describe() + f"; {len(reqs)} requests of log-uniform {lo}-{hi} keys "
+                                   f"(mean {sizes.mean():.0f})",
+        "plain_mops": round(plain_mops, 1),
+        "saturated_mops": round(keys / ts / 1e6, 1),
+        "served_mops": point["served_mops"],
+        "speedup_vs_plain": point["speedup_vs_plain"],
+        "p50_ms": point["p50_ms"],
+        "p99_ms": point["p99_ms"],
+        "rate_req_per_s": point["rate_req_per_s"],
+        "sweep": sweep,
+    }
+    del reqs, w
+    return out
+
+
+def run_serve(args):
+    """The serving workload (ServeWorkload).  For request sizes log-uniform in
+    [1K, 4M] and in [1K, 64K]: saturated throughput of the served loop (all
+    requests queued at once) against per-request plain launches, and p50/p99
+    latency under Poisson arrivals at half the served throughput; then both
+    again with per-table request ordering."""
+    import numpy as np
+    import torch
+    import poggers_amd as pg
+
+    torch.cuda.set_device(0)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    w = ServeWorkload(log)
     results = {}
-    srv = pg.Server(tables, budget_bytes=4 << 30, nstreams=4)
+    srv = pg.Server(w.tables, budget_bytes=4 << 30, nstreams=4)
     for name, lo, hi, nreq in (("mix_1K_4M", 1000, 4 << 20, args.requests),
                                ("small_1K_64K", 1000, 64 << 10, 4 * args.requests)):
-        reqs = make(nreq, lo, hi, 7 + hi)
-        saturated(srv, reqs[: max(1, nreq // 10)])  # warm up
-        ts, keys = saturated(srv, reqs)
-        tp, _ = plain(srv, reqs)
+        reqs = w.make(nreq, lo, hi, 7 + hi)
+        w.saturated(srv, reqs[: max(1, nreq // 10)])  # warm up
+        ts, keys = w.saturated(srv, reqs)
+        tp, _ = w.plain(srv, reqs)
         served = keys / ts / 1e6
         plain_mops = keys / tp / 1e6
         rate = 0.5 * (len(reqs) / ts)
-        lat = poisson(srv, reqs, rate, 11)
+        lat, _ = w.poisson(srv, reqs, rate, 11)
         sizes = np.array([r[6] for r in reqs])
         small = sizes <= 64 * 1024
         results[name] = {
@@ -372,9 +445,9 @@ def run_serve(args):
         }
         # the same requests with per-table ordering (writes in submission order)
         srv.set_order(pg.ORDER_TABLE)
-        to, _ = saturated(srv, reqs)
+        to, _ = w.saturated(srv, reqs)
         rate_o = 0.5 * (len(reqs) / to)
-        lat_o = poisson(srv, reqs, rate_o, 11)
+        lat_o, _ = w.poisson(srv, reqs, rate_o, 11)
         srv.set_order(pg.ORDER_NONE)
         results[name]["ordered"] = {
             "served_mops": round(keys / to / 1e6, 1), "speedup": round(keys / to / 1e6 / plain_mops, 2),
@@ -391,9 +464,7 @@ def run_serve(args):
         "value": results["mix_1K_4M"]["served_mops"], "unit": "Mops/s",
         "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32/u64",
         "data": "synthetic (splitmix64 keys in pinned host memory)",
-        "config": {"workload": f"64 resident tables (32 TCF 2^26-2^28, 32 KV 2^24-2^27; "
-                               f"{total_bytes / 2**30:.1f} GiB), 40% full; op mix 60/30/10 query/insert/delete, "
-                               f"tables Zipf(1.1); continuous batching, 4 streams, 4 GiB staging budget"},
+        "config": {"workload": w.describe() + ", 4 GiB staging budget"},
         "results": results,
     }
     print(json.dumps(out), flush=True)
@@ -407,8 +478,11 @@ def run_batched(args):
     85 % and queried with as many keys (half present, half absent), all in one
     hipGraph replay of a persistent insert kernel and a persistent query
     kernel; tile width, bucket size, work chunk and keys per tile in flight
-    picked by the autotuner on a 512-table sample.  Compared with plain
-    per-table launches."""
+    picked by the autotuner on a 512-table sample.  To separate the gain of
+    batching from that of the shape, the graph and plain per-table launches
+    (a clear, an insert and a query launch per table) are both measured at
+    the tuned shape and at the reference's TCF shape (4-lane tiles, 16-slot
+    buckets)."""
     import numpy as np
     import torch
     import poggers_amd as pg
@@ -427,68 +501,78 @@ def run_batched(args):
     best = tune["best"]
     ins, off, q, qoff, hoff, npres = batched_workload(sizes, 0.85, 77, dev)
     total = ins.numel()
-    desc = pg.tcf_desc(1 << 12, 1 << 6, bucket_size=best["bucket"], tile=best["tile"])
-    batch = pg.Batch(desc, sizes)
-    batch.set_launch(best["chunk"])
-    batch.set_items(best["items"])
+    present = ~absent_mask(hoff, npres, dev)
     found = torch.zeros(total, dtype=torch.uint8, device=dev)
     fails = torch.zeros(1, dtype=torch.int64, device=dev)
-    stream = torch.cuda.Stream()
-    batch.capture(ins, off, q, qoff, None, found, fails, True, stream)
-    for _ in range(args.warmup):
-        batch.replay(stream)
-    stream.synchronize()
-    fails.zero_()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        batch.replay(stream)
-    e1.record(stream)
-    stream.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    gpu_ms = e0.elapsed_time(e1) / args.steps
-    present = ~absent_mask(hoff, npres, dev)
-    false_neg = int((found[present] == 0).sum().item())
-    fp_rate = float(found[~present].float().mean().item())
-    insert_fails = int(fails.item()) // max(1, args.steps)
-    batch.free()
-    del batch
 
-    # plain launches: one table object and one insert + one query launch per
-    # table, at the reference's TCF shape (4-lane tiles, 16-slot buckets).  The
-    # batch's tuned shape is not used here: with 1-lane tiles and 8-slot
-    # buckets a plain launch keeps every key of a large table in flight at
-    # once, the stale bucket fills spill far more keys than the 1 % backing
-    # level holds, and absent-key queries walk the full backing level.
-    tables = [pg.Table(pg.tcf_desc(int(s), max(int(s) // 100, 8), bucket_size=PLAIN_BUCKET, tile=PLAIN_TILE))
-              for s in sizes]
-    torch.cuda.synchronize()
+    def quality():
+        return {"false_negatives": int((found[present] == 0).sum().item()),
+                "fp_rate": float(found[~present].float().mean().item()),
+                "insert_failures": int(fails.item())}
 
-    def plain():
-        for t, tb in enumerate(tables):
-            a, z = int(hoff[t]), int(hoff[t + 1])
-            tb.clear()
-            tb.bulk_insert(ins[a:z])
-            tb.bulk_query(q[a:z], None, found[a:z])
-    plain()
-    torch.cuda.synchronize()
-    p0 = time.perf_counter()
-    for _ in range(max(1, args.steps // 2)):
-        plain()
-    torch.cuda.synchronize()
-    plain_ms = (time.perf_counter() - p0) * 1e3 / max(1, args.steps // 2)
-    for tb in tables:
-        tb.free_on_device()
+    def graph(tile, bucket, chunk, items):
+        desc = pg.tcf_desc(1 << 12, 1 << 6, bucket_size=bucket, tile=tile)
+        batch = pg.Batch(desc, sizes)
+        batch.set_launch(chunk)
+        batch.set_items(items)
+        stream = torch.cuda.Stream()
+        batch.capture(ins, off, q, qoff, None, found, fails, True, stream)
+        for _ in range(args.warmup):
+            batch.replay(stream)
+        stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            fails.zero_()
+            batch.replay(stream)
+        e1.record(stream)
+        stream.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        out = {"ms": round(ms, 3), "gpu_ms_per_replay": round(e0.elapsed_time(e1) / args.steps, 3), **quality()}
+        batch.free()
+        return out
+
+    def plain(tile, bucket):
+        tables = [pg.Table(pg.tcf_desc(int(s), max(int(s) // 100, 8), bucket_size=bucket, tile=tile))
+                  for s in sizes]
+        torch.cuda.synchronize()
+
+        def once():
+            fails.zero_()
+            for t, tb in enumerate(tables):
+                a, z = int(hoff[t]), int(hoff[t + 1])
+                tb.clear()
+                tb.bulk_insert(ins[a:z], None, None, fails)
+                tb.bulk_query(q[a:z], None, found[a:z])
+        once()
+        torch.cuda.synchronize()
+        reps = max(1, args.steps // 2)
+        p0 = time.perf_counter()
+        for _ in range(reps):
+            once()
+        torch.cuda.synchronize()
+        out = {"ms": round((time.perf_counter() - p0) * 1e3 / reps, 3), **quality()}
+        for tb in tables:
+            tb.free_on_device()
+        return out
+
+    shapes = {"tuned": (best["tile"], best["bucket"]), "reference": (PLAIN_TILE, PLAIN_BUCKET)}
+    results = {}
+    for name, (tile, bucket) in shapes.items():
+        g = graph(tile, bucket, best["chunk"], best["items"] if name == "tuned" else 1)
+        p = plain(tile, bucket)
+        results[name] = {"tile": tile, "bucket": bucket, "graph": g, "plain_launches": p,
+                         "speedup_of_batching": round(p["ms"] / g["ms"], 2)}
+        log(f"batched {name}: {results[name]}")
+    head = results["tuned"]["graph"]
     ops = 2 * total
     out = {
         "metric": "batched TCF insert+query Mops/s (4096 uneven filters, one hipGraph)",
-        "value": round(ops / (ms / 1e3) / 1e6, 1),
+        "value": round(ops / (head["ms"] / 1e3) / 1e6, 1),
         "unit": "Mops/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
-        "gpu_ms_per_replay": round(gpu_ms, 3),
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms"],
+        "gpu_ms_per_replay": head["gpu_ms_per_replay"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
         "data": "synthetic (splitmix64 uint64 keys generated on the GPU)",
         "config": {"workload": f"{len(sizes)} TCFs, slots log-uniform in [2^12, 2^22] "
@@ -496,10 +580,11 @@ def run_batched(args):
                                f"(half present, half absent)",
                    "tile": best["tile"], "bucket": best["bucket"], "chunk": best["chunk"],
                    "items": best["items"]},
-        "false_negatives": false_neg, "fp_rate": fp_rate, "insert_failures": insert_fails,
-        "plain_launches_ms": round(plain_ms, 3),
-        "plain_launch_shape": {"tile": PLAIN_TILE, "bucket": PLAIN_BUCKET},
-        "speedup_vs_plain_launches": round(plain_ms / ms, 2),
+        "false_negatives": head["false_negatives"], "fp_rate": head["fp_rate"],
+        "insert_failures": head["insert_failures"],
+        "shapes": results,
+        "speedup_vs_plain_launches_same_shape": results["tuned"]["speedup_of_batching"],
+        "speedup_vs_plain_launches_reference_shape": round(results["reference"]["plain_launches"]["ms"] / head["ms"], 2),
         "autotune": tune,
     }
     print(json.dumps(out), flush=True)
@@ -507,17 +592,20 @@ def run_batched(args):
 
 CAL_COPY_BYTES = 1 << 30          # --calibrate-copy: bytes read and written by the copy
 CAL_RANDOM_READS = 1 << 24        # --calibrate-copy: random 64 B reads
+_PMC = "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-roofline --no-serve --calibrate-copy"
 PROFILE_CMDS = {
     "stats": "rocprofv3 --kernel-trace --stats -f csv -d gpurun_out/prof_stats -o tcf -- "
-             "python3 bench.py --steps 3 --warmup 1 --no-cpu --no-roofline",
-    "fetch": "rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/pmc_fetch -o tcf -- "
-             "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-roofline --calibrate-copy",
-    "write": "rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/pmc_write -o tcf -- "
-             "python3 bench.py --steps 1 --warmup 0 --no-cpu --no-roofline --calibrate-copy",
+             "python3 bench.py --steps 3 --warmup 1 --no-cpu --no-roofline --no-serve",
+    "fetch": f"rocprofv3 --pmc FETCH_SIZE -f csv -d gpurun_out/pmc_fetch -o tcf -- {_PMC}",
+    "write": f"rocprofv3 --pmc WRITE_SIZE -f csv -d gpurun_out/pmc_write -o tcf -- {_PMC}",
     "tcc": "rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum TCC_ATOMIC_sum TCC_EA0_ATOMIC_sum -f csv "
-           "-d gpurun_out/pmc_tcc -o tcf -- python3 bench.py --steps 1 --warmup 0 --no-cpu --no-roofline",
+           f"-d gpurun_out/pmc_tcc -o tcf -- {_PMC}",
+    "rdreq": "rocprofv3 --pmc TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum TCC_EA0_RDREQ_64B_sum "
+             f"TCC_EA0_RDREQ_128B_sum -f csv -d gpurun_out/pmc_rdreq -o tcf -- {_PMC}",
+    "wrreq": "rocprofv3 --pmc TCC_EA0_WRREQ_sum TCC_EA0_WRREQ_64B_sum TCC_EA0_RDREQ_DRAM_sum "
+             f"TCC_EA0_WRREQ_DRAM_sum -f csv -d gpurun_out/pmc_wrreq -o tcf -- {_PMC}",
     "summary": "python3 bench.py --summarize-profiles gpurun_out/prof_stats gpurun_out/pmc_fetch "
-               "gpurun_out/pmc_write gpurun_out/pmc_tcc",
+               "gpurun_out/pmc_write gpurun_out/pmc_tcc gpurun_out/pmc_rdreq gpurun_out/pmc_wrreq",
 }
 
 
@@ -535,119 +623,111 @@ def _short(name, n=110):
     return name if len(name) <= n else name[:n - 3] + "..."
 
 
-def summarize_profiles(stats_dir, fetch_dir, write_dir, tcc_dir=None):
+def summarize_profiles(stats_dir, fetch_dir, write_dir, tcc_dir, rdreq_dir, wrreq_dir):
     """Condense rocprofv3 csv output into the committed text summaries:
-    profiles/tcf_2p28_kernel_stats.md (per-kernel launch statistics, and the
-    per-dispatch HBM counters with their calibration) and
-    profiles/tcf_2p28_traffic.json (the insert kernel's HBM bytes per launch,
-    read by the bench line's roofline.traffic)."""
-    stats = _csv_rows(stats_dir, "_kernel_stats.csv")
-    fetch = _csv_rows(fetch_dir, "_counter_collection.csv")
-    write = _csv_rows(write_dir, "_counter_collection.csv")
-
-    def per_kernel(rows):
-        out = {}
-        for r in rows:
-            out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
-        return out
-
-    grids = {}
-    for r in fetch:
-        grids.setdefault(r["Kernel_Name"], []).append(int(r["Grid_Size"]))
-
-    fk, wk = per_kernel(fetch), per_kernel(write)
-
-    def pick(table, pred):
-        best = None
-        for name, vals in table.items():
-            if pred(name):
-                v = max(vals)
-                if best is None or v > best[1]:
-                    best = (name, v)
-        return best
-
-    is_cal_copy = lambda n: "neg" in n and "double" in n
-    copy_f = pick(fk, is_cal_copy)
-    copy_w = pick(wk, is_cal_copy)
-    rand_f = pick(fk, lambda n: "k_random_sector" in n)
-    if copy_f is None or copy_w is None:
-        raise SystemExit("the calibration kernel (torch.neg over 1 GiB of float64) is not in the PMC runs")
-    fetch_factor = CAL_COPY_BYTES / (copy_f[1] * 1024.0)
-    write_factor = CAL_COPY_BYTES / (copy_w[1] * 1024.0)
-    rand_factor = CAL_RANDOM_READS * 64 / (rand_f[1] * 1024.0) if rand_f else None
-
-    def mean_of(table, key):
-        names = [n for n in table if key in n and ", true" not in n]
-        vals = [v for n in names for v in table[n]]
-        return (names[0] if names else None), (sum(vals) / len(vals) if vals else 0.0), len(vals)
-
-    rows = []
-    for key in ("k_tcf_insert", "k_tcf_query", "k_zero16"):
-        nf, mf, cf = mean_of(fk, key)
-        _, mw, _ = mean_of(wk, key)
-        if nf is None:
-            continue
-        rows.append((key, nf, cf, mf, mw))
-    ins = next(r for r in rows if r[0] == "k_tcf_insert")
-    # The insert kernel streams its keys (8 B each; threads = keys * tile, the
-    # tile being the kernel's first template argument) and reads buckets at
-    # random in 64 B requests.  FETCH_SIZE counts streamed line fills at half
-    # (the streaming calibration) and random 64 B reads exactly (the random
-    # calibration), so each part takes its own factor.
+    profiles/tcf_2p28_kernel_stats.md (per-kernel launch statistics and
+    per-dispatch memory counters) and profiles/tcf_2p28_traffic.json (the
+    insert kernel's bytes between L2 and memory per launch, read by the bench
+    line's roofline.traffic).  The bytes come from the request counters by
+    size (TCC_EA0_RDREQ_32B/64B/128B, TCC_EA0_WRREQ/_64B); the runs include
+    kernels of known traffic (a 1 GiB streaming read + write, 2^24 random
+    64 B reads) that check them, and FETCH_SIZE / WRITE_SIZE are reported
+    beside them with the factor the copy kernel gives them."""
     import re
-    tile = int(re.search(r"k_tcf_insert<(\d+),", ins[1]).group(1))
-    keys_per_launch = sum(grids[ins[1]]) / len(grids[ins[1]]) / tile
-    key_bytes = keys_per_launch * 8
-    fetch_b, write_b = ins[3] * 1024, ins[4] * 1024
-    copy_scaled = fetch_b * fetch_factor + write_b * write_factor
-    mixed = ((fetch_b - key_bytes / fetch_factor) * rand_factor + key_bytes + write_b * write_factor
-             if rand_factor else copy_scaled)
-    # L2 (TCC) hit rate and atomics per dispatch, from the four-counter run
-    tcc = {}
-    if tcc_dir:
-        rows_t = _csv_rows(tcc_dir, "_counter_collection.csv")
-        avg_ns = {r["Name"]: float(r["AverageNs"]) for r in stats}
-        for key, label in (("k_tcf_insert", "insert"), ("k_tcf_query", "query")):
-            per = {}
-            names = set()
-            for r in rows_t:
-                n = r["Kernel_Name"]
-                if key in n and ", true" not in n:
-                    names.add(n)
-                    per.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
-            if not names:
-                continue
-            m = {c: sum(v) / len(v) for c, v in per.items()}
-            hit, miss = m.get("TCC_HIT_sum", 0.0), m.get("TCC_MISS_sum", 0.0)
-            ns = next((avg_ns[n] for n in names if n in avg_ns), None)
-            atom = m.get("TCC_ATOMIC_sum", 0.0)
-            tcc[label] = {
-                "kernel_name": sorted(names)[0],
-                "dispatches": len(per.get("TCC_HIT_sum", [])),
-                "tcc_hit_rate": round(hit / (hit + miss), 4) if hit + miss else None,
-                "tcc_requests_per_launch": int(hit + miss),
-                "atomics_per_launch": int(atom),
-                "atomics_to_memory_per_launch": int(m.get("TCC_EA0_ATOMIC_sum", 0.0)),
-                "atomics_per_s": round(atom / (ns * 1e-9), 1) if ns else None,
-            }
+    stats = _csv_rows(stats_dir, "_kernel_stats.csv")
+    runs = {k: _csv_rows(d, "_counter_collection.csv")
+            for k, d in (("fetch", fetch_dir), ("write", write_dir), ("tcc", tcc_dir), ("rd", rdreq_dir),
+                         ("wr", wrreq_dir))}
+    avg_ns = {r["Name"]: float(r["AverageNs"]) for r in stats}
+
+    def per_dispatch(rows, pred):
+        """{counter: mean value per dispatch} over dispatches of kernels matching pred."""
+        per, names, disp = {}, set(), {}
+        for r in rows:
+            n = r["Kernel_Name"]
+            if pred(n):
+                names.add(n)
+                per.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+                disp[r["Counter_Name"]] = disp.get(r["Counter_Name"], 0) + 1
+        return {c: sum(v) / len(v) for c, v in per.items()}, names, max(disp.values()) if disp else 0
+
+    kinds = {
+        "insert": lambda n: "k_tcf_insert" in n and ", true" not in n,
+        "query": lambda n: "k_tcf_query" in n and ", true" not in n,
+        "copy 1 GiB": lambda n: "neg" in n and "double" in n,
+        "random 64 B x 2^24": lambda n: "k_random_sector" in n,
+    }
+    m = {}
+    for kind, pred in kinds.items():
+        vals, names, nd = {}, set(), 0
+        for run in runs.values():
+            v, nm, d = per_dispatch(run, pred)
+            vals.update(v)
+            names |= nm
+            nd = max(nd, d)
+        if names:
+            m[kind] = {"values": vals, "names": sorted(names), "dispatches": nd}
+    if "insert" not in m or "copy 1 GiB" not in m:
+        raise SystemExit("the insert kernel and the calibration copy must be in the PMC runs")
+
+    def rd_bytes(v):
+        return 32 * v["TCC_EA0_RDREQ_32B_sum"] + 64 * v["TCC_EA0_RDREQ_64B_sum"] + 128 * v["TCC_EA0_RDREQ_128B_sum"]
+
+    def wr_bytes(v):
+        return 32 * (v["TCC_EA0_WRREQ_sum"] - v["TCC_EA0_WRREQ_64B_sum"]) + 64 * v["TCC_EA0_WRREQ_64B_sum"]
+
+    ins = m["insert"]
+    name = ins["names"][0]
+    tile = int(re.search(r"k_tcf_insert<(\d+),", name).group(1))
+    grid = [int(r["Grid_Size"]) for r in runs["fetch"] if r["Kernel_Name"] == name]
+    keys_per_launch = sum(grid) / len(grid) / tile
+    cp = m["copy 1 GiB"]["values"]
+    fetch_factor = CAL_COPY_BYTES / (cp["FETCH_SIZE"] * 1024.0)
+    write_factor = CAL_COPY_BYTES / (cp["WRITE_SIZE"] * 1024.0)
+    table = {}
+    for kind, d in m.items():
+        v = d["values"]
+        rb, wb = rd_bytes(v), wr_bytes(v)
+        hit, miss = v.get("TCC_HIT_sum", 0.0), v.get("TCC_MISS_sum", 0.0)
+        ns = next((avg_ns[n] for n in d["names"] if n in avg_ns), None)
+        table[kind] = {
+            "kernel_name": d["names"][0], "dispatches": d["dispatches"],
+            "read_requests": {sz: int(v[f"TCC_EA0_RDREQ_{sz}_sum"]) for sz in ("32B", "64B", "128B")},
+            "read_requests_total": int(v["TCC_EA0_RDREQ_sum"]),
+            "read_bytes": int(rb), "write_bytes": int(wb),
+            "read_requests_to_dram": int(v["TCC_EA0_RDREQ_DRAM_sum"]),
+            "write_requests_to_dram": int(v["TCC_EA0_WRREQ_DRAM_sum"]),
+            "fetch_size_kib": round(v["FETCH_SIZE"], 1), "write_size_kib": round(v["WRITE_SIZE"], 1),
+            "fetch_write_scaled_by_copy_bytes": int(v["FETCH_SIZE"] * 1024 * fetch_factor +
+                                                    v["WRITE_SIZE"] * 1024 * write_factor),
+            "tcc_hit_rate": round(hit / (hit + miss), 4) if hit + miss else None,
+            "atomics_per_launch": int(v.get("TCC_ATOMIC_sum", 0.0)),
+            "atomics_to_memory_per_launch": int(v.get("TCC_EA0_ATOMIC_sum", 0.0)),
+            "atomics_per_s": round(v.get("TCC_ATOMIC_sum", 0.0) / (ns * 1e-9), 1) if ns else None,
+            "average_us": round(ns / 1e3, 2) if ns else None,
+        }
+    t = table["insert"]
     traffic = {
-        "kernel": "insert",
-        "kernel_name": ins[1],
-        "dispatches": ins[2],
+        "kernel": "insert", "kernel_name": name, "dispatches": t["dispatches"],
         "keys_per_launch": int(keys_per_launch),
-        "fetch_kib_per_launch": round(ins[3], 1),
-        "write_kib_per_launch": round(ins[4], 1),
-        "streaming_fetch_factor": round(fetch_factor, 4),
-        "write_factor": round(write_factor, 4),
-        "random64_fetch_factor": round(rand_factor, 4) if rand_factor else None,
-        "hbm_bytes_per_launch": int(mixed),
-        "hbm_bytes_per_launch_all_streaming_scaled": int(copy_scaled),
-        "commands": [PROFILE_CMDS["fetch"], PROFILE_CMDS["write"], PROFILE_CMDS["tcc"], PROFILE_CMDS["summary"]],
-        "tcc": tcc or None,
+        "hbm_bytes_per_launch": t["read_bytes"] + t["write_bytes"],
+        "read_bytes_per_launch": t["read_bytes"], "write_bytes_per_launch": t["write_bytes"],
+        "bytes_per_key": round((t["read_bytes"] + t["write_bytes"]) / keys_per_launch, 2),
+        "copy_check": {"read_bytes": table["copy 1 GiB"]["read_bytes"],
+                       "write_bytes": table["copy 1 GiB"]["write_bytes"], "expected_each": CAL_COPY_BYTES},
+        "fetch_size_factor_from_copy": round(fetch_factor, 4), "write_size_factor_from_copy": round(write_factor, 4),
+        "hbm_bytes_per_launch_fetch_write_scaled": t["fetch_write_scaled_by_copy_bytes"],
+        "commands": [PROFILE_CMDS[k] for k in ("fetch", "write", "tcc", "rdreq", "wrreq", "summary")],
+        "tcc": {"insert": {k: table["insert"][k] for k in ("tcc_hit_rate", "atomics_per_launch",
+                                                           "atomics_to_memory_per_launch", "atomics_per_s")},
+                "query": ({k: table["query"][k] for k in ("tcc_hit_rate", "atomics_per_launch",
+                                                         "atomics_to_memory_per_launch", "atomics_per_s")}
+                          if "query" in table else None)},
+        "kernels": table,
     }
     lines = ["# TCF 2^28 slots, 90 % load: rocprofv3 summaries", "",
              "Commands (run from the repository root on one MI355X):", ""]
-    lines += [f"    {PROFILE_CMDS[k]}" for k in ("stats", "fetch", "write", "tcc", "summary")]
+    lines += [f"    {PROFILE_CMDS[k]}" for k in ("stats", "fetch", "write", "tcc", "rdreq", "wrreq", "summary")]
     lines += ["", "## Kernel statistics (`--kernel-trace --stats`)", "",
               "Kernels whose fourth template argument is `true` are the accounting variants,",
               "run once, untimed, to count bucket reads; the timed kernels are the `false` ones.", "",
@@ -655,33 +735,26 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, tcc_dir=None):
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:12]:
         lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                      f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.2f} |")
-    lines += ["", "## HBM counters per dispatch (`--pmc FETCH_SIZE`, `--pmc WRITE_SIZE`, KiB)", "",
-              "| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB |", "|---|---|---|---|"]
-    for key, name, cnt, mf, mw in rows:
-        lines.append(f"| `{_short(name)}` | {cnt} | {mf:.1f} | {mw:.1f} |")
-    lines += ["", "Calibration with kernels of known traffic:", "",
-              f"* 1 GiB streaming read + write (`torch.neg` on float64): FETCH_SIZE {copy_f[1]:.0f} KiB "
-              f"(factor {fetch_factor:.3f}), WRITE_SIZE {copy_w[1]:.0f} KiB (factor {write_factor:.3f}).",
-              (f"* 2^24 random 64 B reads (`k_random_sector`): FETCH_SIZE {rand_f[1]:.0f} KiB "
-               f"(factor {rand_factor:.3f})." if rand_f else "* random 64 B calibration: not in this run."),
-              "", f"Insert kernel, {int(keys_per_launch):,} keys per launch: HBM bytes per launch = "
-              f"(FETCH - keys*8/{fetch_factor:.3f}) * {rand_factor or 0:.3f} + keys*8 + WRITE * {write_factor:.3f} "
-              f"= {traffic['hbm_bytes_per_launch']:,} ({traffic['hbm_bytes_per_launch'] / keys_per_launch:.1f} B "
-              f"per key); with every read scaled as streaming: "
-              f"{traffic['hbm_bytes_per_launch_all_streaming_scaled']:,}.", ""]
-    if tcc:
-        lines += ["## L2 (TCC) hit rate and atomics (`--pmc TCC_HIT_sum TCC_MISS_sum TCC_ATOMIC_sum "
-                  "TCC_EA0_ATOMIC_sum`)", "",
-                  "Per dispatch of the timed kernels; atomics/s divides by the kernel's average duration "
-                  "from the statistics run.", "",
-                  "| kernel | dispatches | TCC requests | hit rate | atomics | atomics to memory | atomics/s |",
-                  "|---|---|---|---|---|---|---|"]
-        for label, d in tcc.items():
-            aps = f"{d['atomics_per_s'] / 1e9:.2f} G" if d["atomics_per_s"] else "-"
-            lines.append(f"| {label} `{_short(d['kernel_name'], 60)}` | {d['dispatches']} | "
-                         f"{d['tcc_requests_per_launch']:,} | {d['tcc_hit_rate']} | {d['atomics_per_launch']:,} | "
-                         f"{d['atomics_to_memory_per_launch']:,} | {aps} |")
-        lines.append("")
+    lines += ["", "## Traffic between L2 (TCC) and memory, per dispatch", "",
+              "Bytes from the request counters by size: read = 32 x RDREQ_32B + 64 x RDREQ_64B + "
+              "128 x RDREQ_128B, write = 32 x (WRREQ - WRREQ_64B) + 64 x WRREQ_64B.  Requests 'to DRAM' go to "
+              "the local memory controllers (no counter separates the memory-side Infinity Cache's hits "
+              "from DRAM).  FETCH_SIZE / WRITE_SIZE (KiB) are shown with the factor that makes the 1 GiB "
+              "copy read 1 GiB.", "",
+              "| kernel | dispatches | read req 32/64/128 B | read bytes | write bytes | rd / wr req to DRAM | "
+              "FETCH_SIZE x factor + WRITE_SIZE x factor | L2 hit rate | atomics (to memory) |",
+              "|---|---|---|---|---|---|---|---|---|"]
+    for kind, d in table.items():
+        rq = d["read_requests"]
+        lines.append(f"| {kind} `{_short(d['kernel_name'], 50)}` | {d['dispatches']} | "
+                     f"{rq['32B']:,} / {rq['64B']:,} / {rq['128B']:,} | {d['read_bytes']:,} | {d['write_bytes']:,} | "
+                     f"{d['read_requests_to_dram']:,} / {d['write_requests_to_dram']:,} | "
+                     f"{d['fetch_write_scaled_by_copy_bytes']:,} | {d['tcc_hit_rate']} | "
+                     f"{d['atomics_per_launch']:,} ({d['atomics_to_memory_per_launch']:,}) |")
+    ab = traffic["hbm_bytes_per_launch"]
+    lines += ["", f"Factors from the copy: FETCH_SIZE x {fetch_factor:.3f}, WRITE_SIZE x {write_factor:.3f}.",
+              "", f"Insert kernel, {int(keys_per_launch):,} keys per launch: {ab:,} bytes between L2 and memory "
+              f"per launch ({traffic['bytes_per_key']} B per key); the bench line's `roofline.traffic`.", ""]
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     with open(os.path.join(ROOT, "profiles", "tcf_2p28_kernel_stats.md"), "w") as f:
         f.write("\n".join(lines))
@@ -759,9 +832,63 @@ def load_traffic(path):
         return None
 
 
-def cpu_baseline(log2_slots=26):
-    """CPU oracle (single thread) on a bounded sample: a 2^26-slot TCF with the
-    same shape, filled to 90 % and queried with 2^24 present + 2^24 absent keys."""
+RR_OPS = (("read", 0, 64), ("read_write", 2, 128), ("read_cas", 1, 128))
+RR_SHAPES = (0, 1, 2)          # 4 lanes x 16 B, 1 lane x 64 B, 16 lanes x 4 B
+RR_SECTORS = (1, 2, 4, 8)      # sectors in flight per tile
+RR_WAVES = (1, 2, 3, 4, 6, 0)  # waves per SIMD of a grid-striding grid; 0 = a grid covering the work
+
+
+def random_roofline(pg, torch, dev, stream, footprint, n_reads=1 << 26, reps=3):
+    """Random 64 B-sector bandwidth over a buffer of `footprint` bytes: the best
+    variant per op kind, and for the best tile shape the whole
+    sectors-in-flight x waves-per-SIMD grid (GB/s), so the position of the
+    best variant in the sweep is on record."""
+    buf = torch.zeros(footprint // 8, dtype=torch.int64, device=dev)
+    sink = torch.zeros(1, dtype=torch.int64, device=dev)
+    ev = lambda: torch.cuda.Event(enable_timing=True)
+    out = {"best_variant": {}, "grid": {}}
+    for name, wop, per_op in RR_OPS:
+        best, grids = 0.0, {}
+        for shape in RR_SHAPES:
+            grid = {}
+            for sec in RR_SECTORS:
+                for w in RR_WAVES:
+                    v = sec.bit_length()  # 1 << (v - 1) sectors
+                    op = wop | (v << 4) | (shape << 9) | (w << 11)
+                    pg.random_sector_bench(buf, n_reads, op, 7, sink, stream)
+                    e0, e1 = ev(), ev()
+                    e0.record(stream)
+                    for r in range(reps):
+                        pg.random_sector_bench(buf, n_reads, op, 8 + r, sink, stream)
+                    e1.record(stream)
+                    torch.cuda.synchronize()
+                    gbs = n_reads * per_op / (e0.elapsed_time(e1) / reps / 1e3) / 1e9
+                    grid[f"s{sec}_w{w}"] = round(gbs, 1)
+                    if gbs > best:
+                        best = gbs
+                        out["best_variant"][name] = {"tile_shape": shape, "sectors_in_flight": sec,
+                                                     "waves_per_simd": w or "grid"}
+            grids[shape] = grid
+        out[name] = round(best, 1)
+        out["grid"][name] = grids[out["best_variant"][name]["tile_shape"]]
+    del buf
+    return out
+
+
+def cpu_baseline(log2_slots=28, runs=3, sample_shift=4, query_log2=22):
+    """The CPU oracle on the headline workload, single-threaded and pinned to
+    one core, on a stratified sample that keeps the run near 30 s.
+
+    Workload: the 2^28-slot TCF + 1 % backing filled to 90 % in batches of
+    2^24, then 2^26 present + 2^26 absent queries.  In every insert batch the
+    multi-threaded oracle (every core this process may use) inserts the first
+    15/16 of the keys untimed, then the single-threaded oracle, pinned to one
+    core with sched_setaffinity, inserts the last 1/16 timed, so the timed
+    inserts cover every load level of the fill; at 90 % load it then times
+    2^22 present + 2^22 absent queries.  The per-op times scale to the full
+    op counts, and the rate is the whole workload's ops over that time; the
+    median of `runs` runs.  `multi_thread` is the untimed part's rate."""
+    import numpy as np
     import oracle
     oracle.build()
     nslots = 1 << log2_slots
@@ -770,44 +897,65 @@ def cpu_baseline(log2_slots=26):
                            max_probes=4000)]
     t = oracle.Table(oracle.REP_FP16_V16, levels, seed=42)
     n = int(0.9 * nslots)
+    batch = 1 << 24
+    nq_full = 1 << (log2_slots - 2)
+    nq = 1 << query_log2
     keys = oracle.gen_keys(1234, 0, n)
-    nq = 1 << (log2_slots - 2)
     absent = oracle.gen_keys(1234, 1 << 40, nq)
-    t0 = time.perf_counter()
-    t.bulk_insert(keys)
-    t1 = time.perf_counter()
-    t.bulk_query(keys[::max(1, n // nq)][:nq])  # present keys sampled across the insert order
-    t.bulk_query(absent)
-    t2 = time.perf_counter()
-    ops = n + 2 * nq
-    # the same algorithm on every core this process may use (atomic slot
-    # claims), capped by OMP_NUM_THREADS when the host sets a CPU share
-    threads = len(os.sched_getaffinity(0))
+    present = keys[::max(1, n // nq)][:nq].copy()   # present keys sampled across the insert order
+    allowed = sorted(os.sched_getaffinity(0))
+    threads = len(allowed)
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         threads = max(1, min(threads, int(os.environ["OMP_NUM_THREADS"])))
-    tm = oracle.Table(oracle.REP_FP16_V16, levels, seed=42)
-    m0 = time.perf_counter()
-    st, used = tm.bulk_insert_mt(keys, threads=threads)
-    m1 = time.perf_counter()
-    tm.bulk_query_mt(keys[::max(1, n // nq)][:nq], threads=threads)
-    tm.bulk_query_mt(absent, threads=threads)
-    m2 = time.perf_counter()
+    core = allowed[-1]
+    rates, ins_rates, q_rates, mt = [], [], [], []
+    used = 1
+    for _ in range(runs):
+        t.clear()
+        t_ins = t_mt = 0.0
+        n_ins = n_mt = 0
+        for s0 in range(0, n, batch):
+            s1 = min(n, s0 + batch)
+            cut = s1 - max(1, (s1 - s0) >> sample_shift)
+            m0 = time.perf_counter()
+            _, used = t.bulk_insert_mt(keys[s0:cut], threads=threads)
+            t_mt += time.perf_counter() - m0
+            n_mt += cut - s0
+            os.sched_setaffinity(0, {core})
+            c0 = time.perf_counter()
+            t.bulk_insert(keys[cut:s1])
+            t_ins += time.perf_counter() - c0
+            os.sched_setaffinity(0, allowed)
+            n_ins += s1 - cut
+        os.sched_setaffinity(0, {core})
+        c0 = time.perf_counter()
+        t.bulk_query(present)
+        t.bulk_query(absent)
+        t_q = time.perf_counter() - c0
+        os.sched_setaffinity(0, allowed)
+        per_ins, per_q = t_ins / n_ins, t_q / (2 * nq)
+        total = per_ins * n + per_q * 2 * nq_full
+        rates.append((n + 2 * nq_full) / total / 1e6)
+        ins_rates.append(1 / per_ins / 1e6)
+        q_rates.append(1 / per_q / 1e6)
+        mt.append(n_mt / t_mt / 1e6)
+    med = lambda v: float(np.median(v))
     return {
-        "value": round(ops / (t2 - t0) / 1e6, 3),
+        "value": round(med(rates), 3),
         "unit": "Mops/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"TCF 2^{log2_slots} slots + 1% backing, {n} inserts to 90% + {nq} present + {nq} "
-                  f"absent queries, single-threaded oracle",
-        "insert_mops": round(n / (t1 - t0) / 1e6, 3),
-        "query_mops": round(2 * nq / (t2 - t1) / 1e6, 3),
-        "seconds": round(t2 - t0, 2),
-        "multi_thread": {
-            "value": round(ops / (m2 - m0) / 1e6, 3), "unit": "Mops/s", "cores": used,
-            "insert_mops": round(n / (m1 - m0) / 1e6, 3), "query_mops": round(2 * nq / (m2 - m1) / 1e6, 3),
-            "failed_inserts": int((st == 0).sum()), "seconds": round(m2 - m0, 2),
-            "host_threads_online": oracle.hw_threads(),
-        },
+        "sample": f"TCF 2^{log2_slots} slots + 1% backing filled to 90% in batches of 2^24 (the headline "
+                  f"workload); single-threaded oracle pinned to one core timed on the last 1/{1 << sample_shift} "
+                  f"of every insert batch ({n_ins} inserts, the rest inserted untimed by the multi-threaded "
+                  f"oracle) and on 2^{query_log2} present + 2^{query_log2} absent queries at 90% load, "
+                  f"per-op times scaled to {n} inserts + {2 * nq_full} queries; median of {runs} runs",
+        "insert_mops": round(med(ins_rates), 3),
+        "query_mops": round(med(q_rates), 3),
+        "runs": [round(r, 3) for r in rates],
+        "pinned_core": core,
+        "multi_thread": {"insert_mops": round(med(mt), 3), "cores": used,
+                         "host_threads_online": oracle.hw_threads()},
         "cpu_model": _cpu_model(),
     }
 
@@ -827,8 +975,8 @@ def _cpu_model() -> str:
 def main():
     args = parse()
     if args.summarize_profiles:
-        if len(args.summarize_profiles) not in (3, 4):
-            raise SystemExit("--summarize-profiles STATS_DIR FETCH_DIR WRITE_DIR [TCC_DIR]")
+        if len(args.summarize_profiles) != 6:
+            raise SystemExit("--summarize-profiles STATS FETCH WRITE TCC RDREQ WRREQ (rocprofv3 csv directories)")
         summarize_profiles(*args.summarize_profiles)
         return
     if args.summarize_kernel_stats:
@@ -973,35 +1121,13 @@ def main():
 
     # measured random 64 B-sector roofline over a buffer of the table's
     # footprint: best over the microbenchmark's tile shapes, sectors in flight
-    # and load kinds.  read = 64 B per op; read_cas = 64 B read + the 64 B
-    # sector the CAS dirties (the same accounting as the insert kernel).
+    # per tile (1-8) and waves per SIMD (1-6 grid-striding, or a grid covering
+    # the work as the table kernels launch).  read = 64 B per op; read_write
+    # and read_cas = 64 B read + the 64 B sector the store or CAS dirties (the
+    # same accounting as the insert kernel).
     rand = None
     if not args.no_roofline:
-        foot = nslots * 4
-        buf = torch.zeros(foot // 8, dtype=torch.int64, device=dev)
-        sink = torch.zeros(1, dtype=torch.int64, device=dev)
-        rand = {"read": 0.0, "read_cas": 0.0, "best_variant": {}}
-        n_reads = 1 << 26
-        for cas, name, per_op in ((0, "read", 64), (1, "read_cas", 128)):
-            for shape in (0, 1, 2):
-                for li in (1, 2, 3):
-                    for nt in (0, 1):
-                        op = cas | (li << 4) | (nt << 8) | (shape << 9)
-                        pg.random_sector_bench(buf, n_reads, op, 7, sink, stream)
-                        e0, e1 = ev(), ev()
-                        reps = 3
-                        e0.record(stream)
-                        for r in range(reps):
-                            pg.random_sector_bench(buf, n_reads, op, 8 + r, sink, stream)
-                        e1.record(stream)
-                        torch.cuda.synchronize()
-                        ms = e0.elapsed_time(e1) / reps
-                        gbs = n_reads * per_op / (ms / 1e3) / 1e9
-                        if gbs > rand[name]:
-                            rand[name] = round(gbs, 1)
-                            rand["best_variant"][name] = {"tile_shape": shape, "sectors_in_flight": 1 << li,
-                                                          "nontemporal": nt}
-        del buf
+        rand = random_roofline(pg, torch, dev, stream, nslots * 4)
         kernels["insert"]["frac_random_roofline"] = round(kernels["insert"]["achieved_gbs"] / rand["read_cas"], 4)
         for q in ("query_present", "query_absent"):
             kernels[q]["frac_random_roofline"] = round(kernels[q]["achieved_gbs"] / rand["read"], 4)
@@ -1075,6 +1201,11 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None,
     }
+    if not args.no_serve:
+        del keys, absent, present, table
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        out["serving"] = serving_point(lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -142,12 +142,18 @@ int launch_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *out, hi
 // Random 64 B-sector reads, shaped like a TCF bucket probe.  Sector indices
 // come from fmix64 of a counter (no index array is read).  Variants (packed in
 // `op`, see pg_random_sector_bench):
-//   bit 0     add the CAS an insert performs on one word of the sector (it
-//             rewrites the value it read, so the line is dirtied)
-//   bits 4-7  log2 of sectors per tile kept in flight (default 2 -> 4)
+//   bits 0-1  0 = read only; 1 = add the CAS an insert performs on one word of
+//             the sector (it rewrites the value it read, so the line is
+//             dirtied); 2 = add a plain store of that word instead
+//   bits 4-7  v: sectors per tile kept in flight, 1 << (v - 1) for v in 1..4
+//             (1, 2, 4, 8); 0 = 4
 //   bit 8     nontemporal loads
 //   bits 9-10 tile shape: 0 = 4 lanes x 16 B, 1 = 1 lane x 4 x 16 B,
 //             2 = 16 lanes x 4 B
+//   bits 11-14 w: 0 = one tile per group of sectors (the grid covers the
+//             work, as the table kernels do); 1..15 = w blocks of 256 threads
+//             per CU (w waves per SIMD) walking the work grid-stride, which
+//             caps the requests in flight
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 template <bool NT>
@@ -162,60 +168,74 @@ __device__ inline uint32_t ld4(const uint32_t *p) {
 }
 
 template <int LANES, int I, bool NT>
-__global__ __launch_bounds__(kBlock) void k_random_sector(const uint32_t *buf, FastMod nsec, uint64_t n,
-                                                          uint32_t cas, uint64_t seed, uint64_t *sink) {
+__global__ __launch_bounds__(kBlock) void k_random_sector(uint32_t *buf, FastMod nsec, uint64_t n,
+                                                          uint32_t wop, uint64_t seed, uint64_t *sink) {
     const uint32_t lane = __lane_id();
     const uint32_t rank = lane & (LANES - 1);
-    const uint64_t tile = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / LANES;
-    uint64_t sec[I];
+    const uint64_t tiles_in_grid = (uint64_t(gridDim.x) * kBlock) / LANES;
+    const uint64_t ntiles = (n + I - 1) / I;
     uint32_t x = 0;
-    u32x4 v[I];
+    for (uint64_t tile = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / LANES; tile < ntiles;
+         tile += tiles_in_grid) {
+        uint64_t sec[I];
+        u32x4 v[I];
 #pragma unroll
-    for (int i = 0; i < I; i++) {
-        const uint64_t r = tile * I + i;
-        sec[i] = fastmod(fmix64(r ^ seed), nsec);
-        const uint32_t *base = buf + sec[i] * 16;
-        if (r >= n) {
-            v[i] = u32x4{0, 0, 0, 0};
-        } else if constexpr (LANES == 4) {
-            v[i] = ld16<NT>((const u32x4 *)(base + rank * 4));
-        } else if constexpr (LANES == 1) {
-            const u32x4 a = ld16<NT>((const u32x4 *)base), b = ld16<NT>((const u32x4 *)(base + 4));
-            const u32x4 c = ld16<NT>((const u32x4 *)(base + 8)), d = ld16<NT>((const u32x4 *)(base + 12));
-            v[i] = a ^ b ^ c ^ d;
-        } else {
-            const uint32_t w = ld4<NT>(base + rank);
-            v[i] = u32x4{w, 0, 0, 0};
+        for (int i = 0; i < I; i++) {
+            const uint64_t r = tile * I + i;
+            sec[i] = fastmod(fmix64(r ^ seed), nsec);
+            const uint32_t *base = buf + sec[i] * 16;
+            if (r >= n) {
+                v[i] = u32x4{0, 0, 0, 0};
+            } else if constexpr (LANES == 4) {
+                v[i] = ld16<NT>((const u32x4 *)(base + rank * 4));
+            } else if constexpr (LANES == 1) {
+                const u32x4 a = ld16<NT>((const u32x4 *)base), b = ld16<NT>((const u32x4 *)(base + 4));
+                const u32x4 c = ld16<NT>((const u32x4 *)(base + 8)), d = ld16<NT>((const u32x4 *)(base + 12));
+                v[i] = a ^ b ^ c ^ d;
+            } else {
+                const uint32_t w = ld4<NT>(base + rank);
+                v[i] = u32x4{w, 0, 0, 0};
+            }
         }
-    }
 #pragma unroll
-    for (int i = 0; i < I; i++) {
-        x ^= v[i].x ^ v[i].y ^ v[i].z ^ v[i].w;
-        if (cas && rank == 0 && tile * I + i < n) {
-            unsigned int *w = (unsigned int *)(buf + sec[i] * 16);
-            x ^= atomicCAS(w, v[i].x, v[i].x);
+        for (int i = 0; i < I; i++) {
+            x ^= v[i].x ^ v[i].y ^ v[i].z ^ v[i].w;
+            if (wop && rank == 0 && tile * I + i < n) {
+                uint32_t *w = buf + sec[i] * 16;
+                if (wop == 1) x ^= atomicCAS(w, v[i].x, v[i].x);
+                else *w = v[i].x ^ v[i].y ^ v[i].z ^ v[i].w;  // rewrites the sector's first word
+            }
         }
     }
     if (x == 0x9E3779B9u) sink[0] = x;
 }
 
 template <int LANES, int I, bool NT>
-static void launch_rs(const uint32_t *buf, FastMod f, uint64_t n, uint32_t cas, uint64_t seed, uint64_t *sink,
-                      hipStream_t s) {
+static void launch_rs(uint32_t *buf, FastMod f, uint64_t n, uint32_t wop, uint32_t per_cu, uint64_t seed,
+                      uint64_t *sink, hipStream_t s) {
     const uint64_t tiles = (n + I - 1) / I;
-    const unsigned grid = (unsigned)((tiles * LANES + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL((k_random_sector<LANES, I, NT>), dim3(grid), dim3(kBlock), 0, s, buf, f, n, cas, seed,
+    unsigned grid = (unsigned)((tiles * LANES + kBlock - 1) / kBlock);
+    if (per_cu) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        const unsigned cap = (unsigned)cus * per_cu;
+        if (cap < grid) grid = cap;
+    }
+    hipLaunchKernelGGL((k_random_sector<LANES, I, NT>), dim3(grid), dim3(kBlock), 0, s, buf, f, n, wop, seed,
                        sink);
 }
 
 template <int LANES, bool NT>
-static int launch_rs_i(uint32_t li, const uint32_t *buf, FastMod f, uint64_t n, uint32_t cas, uint64_t seed,
-                       uint64_t *sink, hipStream_t s) {
-    switch (li) {
-        case 0: launch_rs<LANES, 1, NT>(buf, f, n, cas, seed, sink, s); break;
-        case 1: launch_rs<LANES, 2, NT>(buf, f, n, cas, seed, sink, s); break;
-        case 2: launch_rs<LANES, 4, NT>(buf, f, n, cas, seed, sink, s); break;
-        case 3: launch_rs<LANES, 8, NT>(buf, f, n, cas, seed, sink, s); break;
+static int launch_rs_i(uint32_t iv, uint32_t *buf, FastMod f, uint64_t n, uint32_t wop, uint32_t per_cu,
+                       uint64_t seed, uint64_t *sink, hipStream_t s) {
+    switch (iv) {
+        case 1: launch_rs<LANES, 1, NT>(buf, f, n, wop, per_cu, seed, sink, s); break;
+        case 2: launch_rs<LANES, 2, NT>(buf, f, n, wop, per_cu, seed, sink, s); break;
+        case 0:
+        case 3: launch_rs<LANES, 4, NT>(buf, f, n, wop, per_cu, seed, sink, s); break;
+        case 4: launch_rs<LANES, 8, NT>(buf, f, n, wop, per_cu, seed, sink, s); break;
         default: return PG_ERR_INVALID;
     }
     return PG_OK;
@@ -226,19 +246,21 @@ int launch_random_sector(void *buf, uint64_t bytes, uint64_t n, uint32_t op, uin
     const uint64_t nsec = bytes / 64;
     if (nsec == 0) return PG_ERR_INVALID;
     const FastMod f = make_fastmod(nsec);
-    const uint32_t cas = op & 1u;
-    const uint32_t li = ((op >> 4) & 0xFu) ? ((op >> 4) & 0xFu) : 2u;
+    const uint32_t wop = op & 3u;
+    const uint32_t iv = (op >> 4) & 0xFu;
     const bool nt = (op >> 8) & 1u;
     const uint32_t shape = (op >> 9) & 3u;
+    const uint32_t per_cu = (op >> 11) & 0xFu;
+    if (wop == 3) return PG_ERR_INVALID;
     if (n == 0) return PG_OK;
-    const uint32_t *b = (const uint32_t *)buf;
+    uint32_t *b = (uint32_t *)buf;
     int rc;
-    if (shape == 0) rc = nt ? launch_rs_i<4, true>(li, b, f, n, cas, seed, sink, s)
-                            : launch_rs_i<4, false>(li, b, f, n, cas, seed, sink, s);
-    else if (shape == 1) rc = nt ? launch_rs_i<1, true>(li, b, f, n, cas, seed, sink, s)
-                                 : launch_rs_i<1, false>(li, b, f, n, cas, seed, sink, s);
-    else if (shape == 2) rc = nt ? launch_rs_i<16, true>(li, b, f, n, cas, seed, sink, s)
-                                 : launch_rs_i<16, false>(li, b, f, n, cas, seed, sink, s);
+    if (shape == 0) rc = nt ? launch_rs_i<4, true>(iv, b, f, n, wop, per_cu, seed, sink, s)
+                            : launch_rs_i<4, false>(iv, b, f, n, wop, per_cu, seed, sink, s);
+    else if (shape == 1) rc = nt ? launch_rs_i<1, true>(iv, b, f, n, wop, per_cu, seed, sink, s)
+                                 : launch_rs_i<1, false>(iv, b, f, n, wop, per_cu, seed, sink, s);
+    else if (shape == 2) rc = nt ? launch_rs_i<16, true>(iv, b, f, n, wop, per_cu, seed, sink, s)
+                                 : launch_rs_i<16, false>(iv, b, f, n, wop, per_cu, seed, sink, s);
     else return PG_ERR_INVALID;
     if (rc != PG_OK) return rc;
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;

@@ -237,3 +237,77 @@ def test_autotune_picks_a_lossless_shape():
     best = tune["best"]
     assert best in qualified and best["ms"] == min(r["ms"] for r in qualified)
     assert (best["tile"], best["bucket"]) in {(1, 8), (1, 16), (4, 8), (4, 16)} and best["chunk"] in (256, 2048)
+
+
+@pytest.mark.parametrize("tile,B", [(2, 8), (4, 16)], ids=["tuned-shape", "reference-shape"])
+def test_full_size_batched_workload_through_the_graph(tile, B):
+    """The batched workload at full size: 4096 filters with slot counts
+    log-uniform in [2^12, 2^22] (about 2.5e9 slots), each filled to 85 % and
+    queried with as many keys (half stored, half never inserted), all through
+    the captured hipGraph (clear + insert + query), replayed twice.  At the
+    tuned shape (2-lane tiles, 8-slot buckets) and the reference's (4 lanes,
+    16 slots): no stored key is missed, every table's fill equals its
+    successful inserts, and the false-positive rate pooled over 128 tables
+    spread over the size range (every 32nd by size) lies between those of
+    the oracle building the same tables in array order and in a shuffled
+    order, within 5 %."""
+    from concurrent.futures import ThreadPoolExecutor
+    from poggers_amd.workloads import absent_mask, batched_workload, log_uniform_sizes
+    sizes = log_uniform_sizes(4096, 12, 22, seed=11)
+    ins, off, q, qoff, hoff, npres = batched_workload(sizes, 0.85, 77)
+    total = ins.numel()
+    print(f"{len(sizes)} tables, {int(sizes.sum())} slots, {total} inserts + {total} queries", flush=True)
+    desc = proto(tile, B)
+    b = pg.Batch(desc, sizes)
+    b.set_launch(2048)
+    st = torch.zeros(total, dtype=torch.uint8, device="cuda")
+    found = torch.zeros(total, dtype=torch.uint8, device="cuda")
+    fails = torch.zeros(1, dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    b.capture(ins, off, q, qoff, st, found, fails, True, s)
+    for _ in range(2):
+        fails.zero_()
+        found.fill_(7)
+        torch.cuda.synchronize()
+        b.replay(s)
+        s.synchronize()
+    absent = absent_mask(hoff, npres)
+    assert int((found[~absent] != 1).sum().item()) == 0          # no stored key missed
+    assert int((found > 1).sum().item()) == 0
+    nfail = int(fails.item())
+    assert nfail == int((st == 0).sum().item())
+    failed_tables = set()
+    if nfail:
+        idx = torch.nonzero(st == 0).flatten().cpu().numpy()
+        failed_tables = set(np.searchsorted(hoff, idx, side="right") - 1)
+    for t in range(len(sizes)):
+        want = int(hoff[t + 1] - hoff[t])
+        if t in failed_tables:
+            want = int((st[int(hoff[t]):int(hoff[t + 1])] > 0).sum().item())
+        assert b.fill(t) == want, t
+    print(f"failed inserts {nfail} in {len(failed_tables)} tables", flush=True)
+    # pooled false-positive rate of a stratified sample against two key-order builds
+    order = np.argsort(sizes, kind="stable")
+    sample = order[16::32]
+    seg = {}
+    for t in sample:
+        a, m, z = int(hoff[t]), int(hoff[t]) + int(npres[t]), int(hoff[t + 1])
+        seg[t] = (ins[a:z].cpu().numpy().view(np.uint64), q[m:z].cpu().numpy().view(np.uint64),
+                  int(found[m:z].sum().item()))
+
+    def build(t, shuffled):
+        o = oracle_table(desc, int(sizes[t]))
+        k, absent_keys, _ = seg[t]
+        o.bulk_insert(k[np.random.default_rng(5 + int(t)).permutation(len(k))] if shuffled else k)
+        return int(o.bulk_query(absent_keys)[0].sum())
+
+    with ThreadPoolExecutor(8) as ex:
+        arr = sum(ex.map(lambda t: build(t, False), sample))
+        shf = sum(ex.map(lambda t: build(t, True), sample))
+    gpu = sum(seg[t][2] for t in sample)
+    nabs = sum(int(hoff[t + 1] - hoff[t] - npres[t]) for t in sample)
+    print(f"false positives over {nabs} absent queries of {len(sample)} tables: gpu {gpu}, oracle array "
+          f"order {arr}, shuffled order {shf}", flush=True)
+    lo, hi = min(arr, shf), max(arr, shf)
+    assert 0.95 * lo - 4 * np.sqrt(lo + 1) <= gpu <= 1.05 * hi + 4 * np.sqrt(hi + 1)
+    b.free()
