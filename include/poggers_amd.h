@@ -54,7 +54,8 @@ enum pg_status {
 /* Slot representation.
  *   KV64      key_val_pair<uint64_t,uint64_t>: 16 B {key, val}
  *             (representations/key_val_pair.cuh:63-171).  Schemes BUCKET,
- *             SINGLE_SLOT, POWER_OF_N and POWER_OF_N_SHORTCUT_SLOTS.
+ *             SINGLE_SLOT, POWER_OF_N, POWER_OF_N_SHORTCUT_SLOTS and
+ *             POWER_OF_N_SHORTCUT_REPLACE.
  *   FP16_V16  dynamic_container<bit_grouped_container<16,16>, uint16_t>:
  *             one uint32 word = val16 << 16 | fp16
  *             (representations/dynamic_container.cuh:23-141,
@@ -93,8 +94,8 @@ enum pg_status {
  *             A remove turns the key into the tombstone and keeps the value
  *             bits (key_val_pair::atomic_reset).  The slot of the
  *             reference's flat TCF (data_structs/tcf.cuh:29-30).  Schemes
- *             BUCKET, SINGLE_SLOT, POWER_OF_N and POWER_OF_N_SHORTCUT_SLOTS,
- *             as KV64 and KEY64.
+ *             BUCKET, SINGLE_SLOT, POWER_OF_N, POWER_OF_N_SHORTCUT_SLOTS and
+ *             POWER_OF_N_SHORTCUT_REPLACE, as KV64 and KEY64.
  *   Value buffers hold the reference's Val type: uint64 (KV64), uint16
  *   (FP16_V16, FP12, KEY64, KV16), uint8 (FP8_V8 and FP10_V6, which keep the low
  *   8 / 6 bits).  The 8/8, 10/6 and 12-bit formats and grouped levels run the
@@ -133,7 +134,22 @@ enum pg_rep {
  *   POWER_OF_N_SHORTCUT_SLOTS  insert_schemes/power_of_n_shortcut.cuh:
  *                 39-505: POWER_OF_N with the 75 % shortcut (a probe bucket
  *                 whose fill is below 3B/4 takes the key at once), over
- *                 key-value slots (the reference's flat TCF primary)       */
+ *                 key-value slots (the reference's flat TCF primary)
+ *   POWER_OF_N_SHORTCUT_REPLACE  power_of_n_insert_shortcut_replace_scheme
+ *                 (insert_schemes/power_of_n_shortcut_key_replace.cuh:39-523):
+ *                 POWER_OF_N_SHORTCUT_SLOTS whose insert also takes a slot
+ *                 that already holds the key and replaces its value: a probe
+ *                 bucket holding the key is tried at once whatever its fill
+ *                 (check_fill_bucket's Bucket_Size + 1, :233-270), and a claim
+ *                 takes the lowest slot that is empty or holds the key
+ *                 (insert_into_bucket, :134-190).  Probe buckets are visited
+ *                 in order, so a key held in a later probe bucket is stored
+ *                 again in an earlier one with room, as in the reference;
+ *                 queries find the earlier copy.  The reference header does
+ *                 not compile (a brace of insert() is missing after :366,
+ *                 found_ballot starts true at :235, atomic_swap only swaps
+ *                 from the empty key); this is the scheme its code describes
+ *                 with those three slips fixed.  Key-value slots only.   */
 enum pg_scheme {
     PG_SCHEME_BUCKET = 0,
     PG_SCHEME_SINGLE_SLOT = 1,
@@ -141,7 +157,8 @@ enum pg_scheme {
     PG_SCHEME_LINEAR_BUCKETS = 3,
     PG_SCHEME_GROUPED_SHORTCUT = 4,
     PG_SCHEME_POWER_OF_N = 5,
-    PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS = 6
+    PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS = 6,
+    PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE = 7
 };
 
 /* Probing scheme (the probing_schemes directory).  Probe i of a key, before the

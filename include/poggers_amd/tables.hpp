@@ -79,6 +79,12 @@ struct power_of_n_insert_scheme { static constexpr uint32_t scheme = PG_SCHEME_P
 struct power_of_n_insert_shortcut_scheme {
     static constexpr uint32_t scheme = PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
 };
+// power_of_n_insert_shortcut_replace_scheme
+// (power_of_n_shortcut_key_replace.cuh:39-523): the shortcut scheme whose
+// insert also replaces the value of a slot holding the key.
+struct power_of_n_insert_shortcut_replace_scheme {
+    static constexpr uint32_t scheme = PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE;
+};
 // blocked_bucket_insert<K> (insert_schemes/grouped_power_buckets.cuh:388-393):
 // the shortcut scheme over K buckets per 128-byte line.
 template <std::size_t Buckets_per_cache_line>
@@ -190,7 +196,8 @@ class bucketed_table {
     static_assert(rep != PG_REP_KV16 || std::is_same<Val, uint16_t>::value,
                   "16/16 key_val_pair values are uint16_t");
     static constexpr bool flat_pow = Insert_Scheme::scheme == PG_SCHEME_POWER_OF_N ||
-                                     Insert_Scheme::scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+                                     Insert_Scheme::scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS ||
+                                     Insert_Scheme::scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE;
     static_assert(!flat_pow || Max_Probes <= 4, "power-of-n schemes take at most 4 probes");
     static_assert((rep == PG_REP_KV64 || rep == PG_REP_KEY64 || rep == PG_REP_KV16)
                       ? (Insert_Scheme::scheme == PG_SCHEME_BUCKET || Insert_Scheme::scheme == PG_SCHEME_SINGLE_SLOT ||

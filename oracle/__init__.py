@@ -21,7 +21,7 @@ _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
 REP_KV64, REP_FP16_V16, REP_FP8_V8, REP_FP10_V6, REP_FP12, REP_KEY64, REP_KV16 = 0, 1, 2, 3, 4, 5, 6
 SCHEME_BUCKET, SCHEME_SINGLE_SLOT, SCHEME_SHORTCUT, SCHEME_LINEAR_BUCKETS, SCHEME_GROUPED = 0, 1, 2, 3, 4
-SCHEME_POWER_OF_N, SCHEME_POWER_OF_N_SHORTCUT_SLOTS = 5, 6
+SCHEME_POWER_OF_N, SCHEME_POWER_OF_N_SHORTCUT_SLOTS, SCHEME_POWER_OF_N_SHORTCUT_REPLACE = 5, 6, 7
 PROBE_LINEAR, PROBE_DOUBLE, PROBE_P2, PROBE_XOR_P2, PROBE_SPLIT64 = 0, 1, 2, 3, 4
 HASH_FMIX64, HASH_MURMUR64A = 0, 1
 

@@ -33,6 +33,11 @@
  *                    hand-derived slot words, kept tombstone values, fills
  *                    that count tombstones, least-filled choice and tie
  *                    order (tests/test_oracle_flat.py).
+ *   replace scheme -- hand-derived scenarios (value replaced in place, in
+ *                    a full bucket, a copy stored in an earlier probe
+ *                    bucket with room and what a remove then uncovers, the
+ *                    16/16 word and the key-only slot) and a dictionary
+ *                    model of insert-or-assign (tests/test_oracle_replace.py).
  *   multi-thread  -- with one thread, byte-identical layouts to the
  *                    sequential functions; with several, zero false
  *                    negatives, fill = stored keys, queries identical to
@@ -53,7 +58,8 @@ enum { OR_REP_KV64 = 0, OR_REP_FP16_V16 = 1, OR_REP_FP8_V8 = 2, OR_REP_FP10_V6 =
        OR_REP_KEY64 = 5, OR_REP_KV16 = 6 };
 enum { OR_SCHEME_BUCKET = 0, OR_SCHEME_SINGLE_SLOT = 1,
        OR_SCHEME_POWER_OF_N_SHORTCUT = 2, OR_SCHEME_LINEAR_BUCKETS = 3,
-       OR_SCHEME_GROUPED = 4, OR_SCHEME_POWER_OF_N = 5, OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS = 6 };
+       OR_SCHEME_GROUPED = 4, OR_SCHEME_POWER_OF_N = 5, OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS = 6,
+       OR_SCHEME_POWER_OF_N_SHORTCUT_REPLACE = 7 };
 enum { OR_PROBE_LINEAR = 0, OR_PROBE_DOUBLE = 1, OR_PROBE_P2 = 2,
        OR_PROBE_XOR_P2 = 3, OR_PROBE_SPLIT64 = 4 };
 enum { OR_HASH_FMIX64 = 0, OR_HASH_MURMUR64A = 1 };
@@ -858,9 +864,41 @@ static int kv_pow_claim(or_table *t, or_level *L, uint64_t b, uint64_t key, uint
     return 0;
 }
 
+/* power_of_n_insert_shortcut_replace_scheme (power_of_n_shortcut_key_replace.cuh:
+ * 39-523), read with three slips fixed: insert()'s `if` at :361 closes after
+ * `current_fill = Bucket_Size;` as in power_of_n_shortcut.cuh; found_ballot
+ * (:235) starts false, so a bucket reads Bucket_Size + 1 only when a slot holds
+ * the key; the swap on a slot holding the key replaces its value.
+ *   fill  = as above, or B + 1 when a slot holds the key (:233-270);
+ *   claim = the lowest slot that is empty or holds the key (:134-190): an
+ *           empty slot takes key and value, a held key gets the new value.
+ * Query and remove are the shortcut scheme's (:446-523). */
+static int kv_pow_find(or_table *t, or_level *L, uint64_t b, uint64_t key)
+{
+    for (uint32_t i = 0; i < L->B; i++)
+        if (kv_key_at(t, L, b * L->B + i) == kv_skey(t, key)) return 1;
+    return 0;
+}
+
+static int kv_pow_claim_replace(or_table *t, or_level *L, uint64_t b, uint64_t key, uint64_t val)
+{
+    for (uint32_t i = 0; i < L->B; i++) {
+        const uint64_t k = kv_key_at(t, L, b * L->B + i);
+        if (k == KV_EMPTY || k == kv_skey(t, key)) {
+            kv_put(t, L, b * L->B + i, kv_skey(t, key), val);
+            t->sectors_written += 1;
+            return 1;
+        }
+    }
+    return 0;
+}
+
 static int kv_pow_insert(or_table *t, or_level *L, int lvl, uint64_t key, uint64_t val)
 {
-    const int shortcut = L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+    const int replace = L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_REPLACE;
+    const int shortcut = L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS || replace;
+    int (*claim)(or_table *, or_level *, uint64_t, uint64_t, uint64_t) =
+        replace ? kv_pow_claim_replace : kv_pow_claim;
     const int B = (int)L->B;
     uint64_t buckets[MAXP];
     int fill[MAXP];
@@ -872,8 +910,9 @@ static int kv_pow_insert(or_table *t, or_level *L, int lvl, uint64_t key, uint64
         buckets[np] = b;
         account_bucket(t, lvl, b);
         int f = kv_pow_fill(t, L, b);
-        if (shortcut && 4 * f < 3 * B) {
-            if (kv_pow_claim(t, L, b, key, val)) return 1;
+        if (replace && kv_pow_find(t, L, b, key)) f = B + 1;
+        if (shortcut && (4 * f < 3 * B || f == B + 1)) {
+            if (claim(t, L, b, key, val)) return 1;
             f = B;
         }
         fill[np] = f;
@@ -883,7 +922,7 @@ static int kv_pow_insert(or_table *t, or_level *L, int lvl, uint64_t key, uint64
     for (int lvl_fill = min_fill; lvl_fill < B;) {
         int next = B;
         for (int j = 0; j < np; j++) {
-            if (fill[j] == lvl_fill && kv_pow_claim(t, L, buckets[j], key, val)) return 1;
+            if (fill[j] == lvl_fill && claim(t, L, buckets[j], key, val)) return 1;
             if (fill[j] > lvl_fill && fill[j] < next) next = fill[j];
         }
         lvl_fill = next;
@@ -926,7 +965,8 @@ static int kv_pow_remove(or_table *t, or_level *L, int lvl, uint64_t key)
 
 static inline int kv_pow_scheme(const or_level *L)
 {
-    return L->d.scheme == OR_SCHEME_POWER_OF_N || L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+    return L->d.scheme == OR_SCHEME_POWER_OF_N || L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS ||
+           L->d.scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_REPLACE;
 }
 
 /* ------------------------------------------------------------------ */
@@ -950,11 +990,12 @@ or_table *or_create(uint32_t rep, uint32_t nlevels, const or_level_desc *lv, uin
         if (grouped && (is_kv_rep(rep) || lv[l].bucket_size == 0 ||
                         lv[l].group * bucket_bytes_of(rep, lv[l].bucket_size) > 128))
             return NULL;
-        const int kv_scheme = lv[l].scheme <= OR_SCHEME_SINGLE_SLOT || lv[l].scheme == OR_SCHEME_POWER_OF_N ||
-                              lv[l].scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+        const int flat = lv[l].scheme == OR_SCHEME_POWER_OF_N || lv[l].scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS ||
+                         lv[l].scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_REPLACE;
+        const int kv_scheme = lv[l].scheme <= OR_SCHEME_SINGLE_SLOT || flat;
         if (is_kv_rep(rep) != kv_scheme) return NULL;
         /* the flat power-of-n schemes keep one fill per probe (MAXP) */
-        if ((lv[l].scheme == OR_SCHEME_POWER_OF_N || lv[l].scheme == OR_SCHEME_POWER_OF_N_SHORTCUT_SLOTS) &&
+        if (flat &&
             (lv[l].max_probes == 0 || lv[l].max_probes > MAXP))
             return NULL;
     }

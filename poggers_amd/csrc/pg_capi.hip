@@ -39,7 +39,8 @@ int check_level(uint32_t rep, const pg_level_desc &l, uint32_t index) {
         l.max_probes != 2)
         return PG_ERR_INVALID;
     if (pg::is_kv_rep(rep)) {
-        const bool pow = l.scheme == PG_SCHEME_POWER_OF_N || l.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+        const bool pow = l.scheme == PG_SCHEME_POWER_OF_N || l.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS ||
+                         l.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE;
         if (l.scheme != PG_SCHEME_BUCKET && l.scheme != PG_SCHEME_SINGLE_SLOT && !pow) return PG_ERR_UNSUPPORTED;
         if (l.scheme != PG_SCHEME_SINGLE_SLOT && (l.bucket_size == 0 || l.bucket_size > 64)) return PG_ERR_INVALID;
         // single_slot_insert static_asserts Partition_Size == Bucket_Size

@@ -6,7 +6,7 @@ namespace pg {
 
 constexpr int kBlock = 256;
 
-template <int P, bool ACC, class SL>
+template <int P, bool ACC, class SL, bool RP = false>
 __global__ __launch_bounds__(kBlock) void k_kv_insert(TableDev T, OpArgs a) {
     Tile<P> t;
     t.concurrent = true;
@@ -15,7 +15,7 @@ __global__ __launch_bounds__(kBlock) void k_kv_insert(TableDev T, OpArgs a) {
     int st = 1;
     if (act) {
         const uint64_t val = a.vals ? (uint64_t)((const typename SL::V *)a.vals)[tid] : 0ull;
-        st = kv_insert_key<P, SL>(T, a.keys[tid], val, t);
+        st = kv_insert_key<P, SL, RP>(T, a.keys[tid], val, t);
         if (a.flags && t.leader()) a.flags[tid] = (uint8_t)st;
     }
     wave_count(a.fail_count, act && t.leader() && st == 0);
@@ -50,13 +50,13 @@ __global__ __launch_bounds__(kBlock) void k_kv_remove(TableDev T, OpArgs a) {
     if constexpr (ACC) flush_acc(a.stats, t, tid < a.n);
 }
 
-template <int P, class SL>
+template <int P, class SL, bool RP = false>
 __global__ __launch_bounds__(64) void k_kv_insert_seq(TableDev T, OpArgs a) {
     Tile<P> t;
     uint64_t fails = 0;
     for (uint64_t i = 0; i < a.n; i++) {
         const uint64_t val = a.vals ? (uint64_t)((const typename SL::V *)a.vals)[i] : 0ull;
-        const int st = kv_insert_key<P, SL>(T, a.keys[i], val, t);
+        const int st = kv_insert_key<P, SL, RP>(T, a.keys[i], val, t);
         if (a.flags && t.leader()) a.flags[i] = (uint8_t)st;
         fails += (st == 0);
         fence_agent();
@@ -77,23 +77,34 @@ __global__ __launch_bounds__(64) void k_kv_remove_seq(TableDev T, OpArgs a) {
 }
 
 template <int P, bool ACC, class SL>
-static void launch_kv_conc(Op op, unsigned grid, const TableDev &T, const OpArgs &a, hipStream_t s) {
+static void launch_kv_conc(Op op, unsigned grid, const TableDev &T, const OpArgs &a, bool rp, hipStream_t s) {
     if (op == Op::Query) hipLaunchKernelGGL((k_kv_query<P, ACC, SL>), dim3(grid), dim3(kBlock), 0, s, T, a);
+    else if (op == Op::Insert && rp)
+        hipLaunchKernelGGL((k_kv_insert<P, ACC, SL, true>), dim3(grid), dim3(kBlock), 0, s, T, a);
     else if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert<P, ACC, SL>), dim3(grid), dim3(kBlock), 0, s, T, a);
     else hipLaunchKernelGGL((k_kv_remove<P, ACC, SL>), dim3(grid), dim3(kBlock), 0, s, T, a);
+}
+
+// Whether a level of the table runs the replace scheme.
+static bool has_replace_level(const TableDev &T) {
+    for (uint32_t l = 0; l < T.nlevels; l++)
+        if (T.lv[l].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE) return true;
+    return false;
 }
 
 template <int P, class SL>
 static int launch_kv_ps(const TableDev &T, Op op, const OpArgs &a, hipStream_t s) {
     if (a.n == 0) return PG_OK;
     const unsigned grid = (unsigned)((a.n * (uint64_t)P + kBlock - 1) / kBlock);
+    const bool rp = has_replace_level(T);
     if (op != Op::Query && a.deterministic) {
-        if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert_seq<P, SL>), dim3(1), dim3(P), 0, s, T, a);
+        if (op == Op::Insert && rp) hipLaunchKernelGGL((k_kv_insert_seq<P, SL, true>), dim3(1), dim3(P), 0, s, T, a);
+        else if (op == Op::Insert) hipLaunchKernelGGL((k_kv_insert_seq<P, SL>), dim3(1), dim3(P), 0, s, T, a);
         else hipLaunchKernelGGL((k_kv_remove_seq<P, SL>), dim3(1), dim3(P), 0, s, T, a);
     } else if (a.stats) {
-        launch_kv_conc<P, true, SL>(op, grid, T, a, s);
+        launch_kv_conc<P, true, SL>(op, grid, T, a, rp, s);
     } else {
-        launch_kv_conc<P, false, SL>(op, grid, T, a, s);
+        launch_kv_conc<P, false, SL>(op, grid, T, a, rp, s);
     }
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
 }

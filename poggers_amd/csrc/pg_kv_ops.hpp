@@ -71,6 +71,12 @@ struct SlotKV {
     __device__ static bool bury(S *p, const S &, uint64_t sk) {
         return atomicCAS((unsigned long long *)&p->x, sk, kKvTomb) == sk;
     }
+    // The replace scheme's swap on a slot seen holding sk: the value word is
+    // stored alone, so a reader's 16 B load sees the old or the new value.
+    __device__ static bool replace(S *p, const S &, uint64_t, uint64_t v) {
+        __hip_atomic_store(&p->y, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
 };
 struct SlotKey {
     using S = uint64_t;
@@ -91,6 +97,8 @@ struct SlotKey {
     __device__ static bool bury(S *p, const S &, uint64_t sk) {
         return atomicCAS((unsigned long long *)p, sk, kKvTomb) == sk;
     }
+    // key only: a slot holding the key already is the result
+    __device__ static bool replace(S *, const S &, uint64_t, uint64_t) { return true; }
 };
 // dynamic_container<key_val_pair<uint64_t, uint16_t>, uint16_t>: one 32-bit
 // word val16 << 16 | key16 (the flat TCF's slot, data_structs/tcf.cuh:29-30).
@@ -112,6 +120,18 @@ struct SlotKV16 {
     }
     __device__ static bool bury(S *p, const S &cur, uint64_t) {
         return atomicCAS(p, cur, (cur & 0xFFFF0000u) | 0xFFFFu) == cur;
+    }
+    // New value bits under the same key16; repeats only while another thread
+    // changes the word and the key stays, so it ends once the swap lands or
+    // the key is gone (then the slot is no longer a candidate).
+    __device__ static bool replace(S *p, const S &cur, uint64_t sk, uint64_t v) {
+        S w = cur;
+        while ((w & 0xFFFFu) == (uint32_t)sk) {
+            const S prev = atomicCAS(p, w, ((uint32_t)v << 16) | (uint32_t)sk);
+            if (prev == w) return true;
+            w = prev;
+        }
+        return false;
     }
 };
 
@@ -250,8 +270,27 @@ __device__ int kv_pow_fill(const typename SL::S *bk, uint32_t B, const Tile<P> &
     return f;
 }
 
-// The lowest empty slot among [0, lim).
+// The fill and whether a slot holds sk (the replace scheme's
+// check_fill_bucket, power_of_n_shortcut_key_replace.cuh:233-270).
 template <int P, class SL>
+__device__ int kv_pow_fill_find(const typename SL::S *bk, uint32_t B, uint64_t sk, const Tile<P> &t, bool &found) {
+    int f = 0;
+    bool hit = false;
+    for (uint32_t s0 = 0; s0 < B; s0 += P) {
+        const uint32_t i = s0 + t.rank;
+        const uint64_t k = i < B ? SL::key(SL::load(bk + i)) : kKvEmpty;
+        f += t.popc(k != kKvEmpty);
+        hit |= t.any(i < B && k == sk);
+    }
+    t.acc[0] += B;
+    found = hit;
+    return f;
+}
+
+// The lowest empty slot among [0, lim); with RP (the replace scheme), the
+// lowest slot that is empty or holds sk, whose value is then replaced
+// (insert_into_bucket, power_of_n_shortcut_key_replace.cuh:134-190).
+template <int P, class SL, bool RP = false>
 __device__ bool kv_pow_claim(typename SL::S *bk, uint32_t B, uint32_t lim, uint64_t sk, uint64_t val,
                              const Tile<P> &t) {
     using S = typename SL::S;
@@ -261,11 +300,17 @@ __device__ bool kv_pow_claim(typename SL::S *bk, uint32_t B, uint32_t lim, uint6
         const bool ok = i < end;
         const S s = ok ? SL::load(bk + i) : SL::none();
         t.acc[0] += (end - s0 < P) ? end - s0 : P;
-        uint64_t cm = t.bits(ok && SL::key(s) == kKvEmpty);
+        uint64_t cm = t.bits(ok && (SL::key(s) == kKvEmpty || (RP && SL::key(s) == sk)));
         while (cm) {
             const uint32_t l = (uint32_t)ctz64(cm);
             bool success = false;
-            if (t.rank == l) success = SL::claim(bk + i, s, sk, val);
+            if (t.rank == l) {
+                if constexpr (RP) {
+                    success = SL::key(s) == kKvEmpty ? SL::claim(bk + i, s, sk, val) : SL::replace(bk + i, s, sk, val);
+                } else {
+                    success = SL::claim(bk + i, s, sk, val);
+                }
+            }
             if (t.any(success)) {
                 t.acc[2]++;
                 return true;
@@ -284,12 +329,20 @@ __device__ bool kv_pow_claim(typename SL::S *bk, uint32_t B, uint32_t lim, uint6
 // which applies the rule at the moment of the claim, and a bucket whose
 // bounded claim fails enters the fill-ordered pass at fill ceil(3B/4).  In
 // key order the bound is B, and a failed claim means the bucket is full.
-template <int P, class SL>
+//
+// The replace scheme (power_of_n_shortcut_key_replace.cuh:336-444) is the
+// shortcut scheme whose first pass also enters a probe bucket that holds the
+// key, whatever its fill, and whose claims take the lowest slot that is empty
+// or holds the key.  By the prefix property no empty slot lies below a held
+// key, so such a claim replaces the value; the bound is then B, as the key's
+// slot may lie past 3B/4.  RP selects it at compile time, so the other
+// schemes' kernels carry none of its code.
+template <int P, class SL, bool RP = false>
 __device__ bool kv_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
     using S = typename SL::S;
     Probe pr;
     pr.begin(L, key, SL::tag(key));
-    const bool shortcut = L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+    const bool shortcut = RP || L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
     const uint64_t sk = SL::stored(key);
     S *slots = (S *)L.slots;
     const int B = (int)L.B;
@@ -303,9 +356,12 @@ __device__ bool kv_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, con
         if (p == kEnd) break;
         const uint64_t b = fastmod(p, L.nb);
         bidx[j] = b;
-        int f = kv_pow_fill<P, SL>(slots + b * L.B, L.B, t);
-        if (shortcut && 4 * f < 3 * B) {
-            if (kv_pow_claim<P, SL>(slots + b * L.B, L.B, lim1, sk, val, t)) return true;
+        bool found = false;
+        int f;
+        if constexpr (RP) f = kv_pow_fill_find<P, SL>(slots + b * L.B, L.B, sk, t, found);
+        else f = kv_pow_fill<P, SL>(slots + b * L.B, L.B, t);
+        if (shortcut && (4 * f < 3 * B || found)) {
+            if (kv_pow_claim<P, SL, RP>(slots + b * L.B, L.B, found ? L.B : lim1, sk, val, t)) return true;
             f = (int)lim1;
         }
         fill[j] = f;
@@ -350,7 +406,7 @@ __device__ bool kv_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, con
             uint64_t bb = bidx[0];
 #pragma unroll
             for (int j = 1; j < (int)kMaxShortcutProbes; j++) bb = (j == best) ? bidx[j] : bb;
-            if (kv_pow_claim<P, SL>(slots + bb * L.B, L.B, (uint32_t)second + 1u, sk, val, t)) return true;
+            if (kv_pow_claim<P, SL, RP>(slots + bb * L.B, L.B, (uint32_t)second + 1u, sk, val, t)) return true;
 #pragma unroll
             for (int j = 0; j < (int)kMaxShortcutProbes; j++)
                 if (j == best) fill[j] = second + 1 < B ? second + 1 : B;
@@ -363,7 +419,8 @@ __device__ bool kv_pow_insert(const LevelDev &L, uint64_t key, uint64_t val, con
 #pragma unroll
         for (int j = 0; j < (int)kMaxShortcutProbes; j++) {
             if (j < np) {
-                if (fill[j] == lvl && kv_pow_claim<P, SL>(slots + bidx[j] * L.B, L.B, L.B, sk, val, t)) return true;
+                if (fill[j] == lvl && kv_pow_claim<P, SL, RP>(slots + bidx[j] * L.B, L.B, L.B, sk, val, t))
+                    return true;
                 if (fill[j] > lvl && fill[j] < next) next = fill[j];
             }
         }
@@ -671,17 +728,20 @@ __device__ inline int row16_words(const LevelDev &L) {
 }
 
 __device__ inline bool pow_scheme(const LevelDev &L) {
-    return L.scheme == PG_SCHEME_POWER_OF_N || L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS;
+    return L.scheme == PG_SCHEME_POWER_OF_N || L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_SLOTS ||
+           L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE;
 }
 
 // bucket_insert::insert (bucket_insert.cuh:127-203) / single_slot_insert::
 // insert (single_slot_insert.cuh:125-166): lowest empty slot in probe order;
 // key_val_pair::atomic_swap = CAS key 0 -> key, then store the value
 // (key_val_pair.cuh:84-95).
-template <int P, class SL = SlotKV>
+template <int P, class SL = SlotKV, bool RP = false>
 __device__ bool kv_level_insert(const LevelDev &L, uint64_t key, uint64_t val, const Tile<P> &t) {
     using S = typename SL::S;
     if (pow_scheme(L)) {
+        if constexpr (RP)
+            if (L.scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE) return kv_pow_insert<P, SL, true>(L, key, val, t);
         if constexpr (std::is_same<SL, SlotKV16>::value) {
             const int rw = row16_words<P>(L);
             if (rw == 4) return kv16_pow_insert<P, 4>(L, key, val, t);
@@ -903,10 +963,12 @@ __device__ bool kv_level_remove(const LevelDev &L, uint64_t key, const Tile<P> &
 
 // Whole-table ops: level 0, then the backing level on false
 // (tables/base_table.cuh:167-308).
-template <int P, class SL = SlotKV>
+// RP: the table has a level of the replace scheme (a separate kernel, so the
+// other tables' kernels carry none of that scheme's code).
+template <int P, class SL = SlotKV, bool RP = false>
 __device__ int kv_insert_key(const TableDev &T, uint64_t key, uint64_t val, const Tile<P> &t) {
     for (uint32_t l = 0; l < T.nlevels; l++)
-        if (kv_level_insert<P, SL>(T.lv[l], key, val, t)) return 1 + (int)l;
+        if (kv_level_insert<P, SL, RP>(T.lv[l], key, val, t)) return 1 + (int)l;
     return 0;
 }
 template <int P, class SL = SlotKV>

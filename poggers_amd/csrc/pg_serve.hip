@@ -305,6 +305,13 @@ int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_
             delete s;
             return PG_ERR_UNSUPPORTED;
         }
+        // the served kernel carries no replace-scheme insert
+        for (uint32_t l = 0; l < T->nlevels; l++) {
+            if (T->lv[l].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE) {
+                delete s;
+                return PG_ERR_UNSUPPORTED;
+            }
+        }
         s->P = tile;
         s->h_tables[i] = *T;
     }

@@ -66,6 +66,13 @@ static int cpu_checks() {
     size_in_num_slots<1> one_b(4096);
     tcqf_twelve::describe(d, 0, &one_b);
     CHECK(d.level[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT && d.level[0].group == 0);
+    // power_of_n_insert_shortcut_replace_scheme over key_val_pair slots
+    using replace_table = static_table<uint64_t, uint64_t, key_val_pair, 4, 16,
+                                       power_of_n_insert_shortcut_replace_scheme, 2, doubleHasher, murmurHasher>;
+    size_in_num_slots<1> one_r(4096);
+    replace_table::describe(d, 0, &one_r);
+    CHECK(d.level[0].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE && d.level[0].bucket_size == 16 &&
+          d.level[0].tile == 4 && d.level[0].max_probes == 2);
     size_in_num_slots<2> two_b(1000, 10);  // a sizing is consumed as levels take their sizes
     tcf_10_6::describe(d, 0, &two_b);
     CHECK(d.level[0].tile == 1 && d.level[0].bucket_size == 8);

@@ -69,6 +69,12 @@ def test_create_rejects_bad_descriptors_without_gpu_work():
                                             prober=pg.PROBE_DOUBLE, max_probes=8)]),
         pg.TableDesc(pg.REP_KV16, [pg.Level(1024, pg.SCHEME_SHORTCUT, 16, prober=pg.PROBE_DOUBLE)]),
         pg.TableDesc(pg.REP_KV16, [pg.Level(1024, pg.SCHEME_POWER_OF_N, 0, prober=pg.PROBE_DOUBLE)]),
+        # the replace scheme is over key-value slots only, with at most 4 probes
+        pg.TableDesc(pg.REP_FP16_V16, [pg.Level(1024, pg.SCHEME_POWER_OF_N_SHORTCUT_REPLACE, 16,
+                                                prober=pg.PROBE_DOUBLE, max_probes=2)]),
+        pg.TableDesc(pg.REP_KV64, [pg.Level(1024, pg.SCHEME_POWER_OF_N_SHORTCUT_REPLACE, 16,
+                                            prober=pg.PROBE_DOUBLE, max_probes=5)]),
+        pg.TableDesc(pg.REP_KV64, [pg.Level(1024, 8, 16, prober=pg.PROBE_DOUBLE, max_probes=2)]),
         # single_slot_insert needs Partition_Size == Bucket_Size (single_slot_insert.cuh:43)
         pg.TableDesc(pg.REP_KV64, [pg.Level(1024, pg.SCHEME_SINGLE_SLOT, 1, tile=4, prober=pg.PROBE_DOUBLE,
                                             max_probes=5)]),
