@@ -131,6 +131,43 @@ def test_graph_capture_replays_match_eager():
         torch.zeros(1 << 20, device="cuda").sum()  # eager work between replays
 
 
+def test_graph_capture_deterministic_replays_bit_exact():
+    """In deterministic mode the captured clear + insert + query is a fixed
+    computation: every replay, with eager work between replays, gives the
+    oracle's statuses, found flags and per-table slot bytes exactly."""
+    desc = proto(4, 16)
+    sizes, ins, ins_off, q, q_off = make_workload(40, 8, 12, 0.95, 9)
+    b = pg.Batch(desc, sizes)
+    b.set_mode(pg.DETERMINISTIC)
+    ki, oi, kq, oq = dev_u64(ins), dev_u64(ins_off), dev_u64(q), dev_u64(q_off)
+    st = torch.zeros(len(ins), dtype=torch.uint8, device="cuda")
+    found = torch.zeros(len(q), dtype=torch.uint8, device="cuda")
+    s = torch.cuda.Stream()
+    b.capture(ki, oi, kq, oq, st, found, None, True, s)
+    want_st, want_found, want_img = [], [], []
+    for t, ns in enumerate(sizes):
+        o = oracle_table(desc, ns)
+        a, z = int(ins_off[t]), int(ins_off[t + 1])
+        want_st.append(o.bulk_insert(ins[a:z]))
+        qa, qz = int(q_off[t]), int(q_off[t + 1])
+        want_found.append(o.bulk_query(q[qa:qz])[0])
+        want_img.append([o.level_array(lvl) for lvl in range(2)])
+    want_st, want_found = np.concatenate(want_st), np.concatenate(want_found)
+    for r in range(3):
+        st.fill_(9)
+        found.fill_(7)
+        torch.cuda.synchronize()
+        b.replay(s)
+        s.synchronize()
+        assert (st.cpu().numpy() == want_st).all(), r
+        assert (found.cpu().numpy() == want_found).all(), r
+        for t in range(len(sizes)):
+            for lvl in range(2):
+                img = np.frombuffer(b.level_bytes(t, lvl), dtype=np.uint32)
+                assert (img == want_img[t][lvl]).all(), (r, t, lvl)
+        torch.zeros(1 << 20, device="cuda").sum()  # eager work between replays
+
+
 @pytest.mark.parametrize("tile,B", [(1, 8), (2, 8), (1, 16), (2, 32)])
 def test_persistent_narrow_tiles(tile, B):
     """1- and 2-lane tiles (one lane reads a whole bucket): every stored key
