@@ -43,6 +43,9 @@ CASES = {
     "kv16_flat_tile4": flat_replace(1 << 12),
     "kv16_flat_tile2": flat_replace(1 << 12, tile=2),
     "key64_tile4": key64(1 << 12),
+    # a replace level backing a replace level: spilled keys are replaced there
+    "kv64_two_replace_levels": pg.TableDesc(pg.REP_KV64, [pg.Level(1 << 12, R, 16, 4, D, M, 2),
+                                                          pg.Level(1 << 9, R, 8, 4, D, M, 2)]),
 }
 
 
