@@ -652,8 +652,10 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, tcc_dir, rdreq_dir, wrre
         return {c: sum(v) / len(v) for c, v in per.items()}, names, max(disp.values()) if disp else 0
 
     kinds = {
-        "insert": lambda n: "k_tcf_insert" in n and ", true" not in n,
-        "query": lambda n: "k_tcf_query" in n and ", true" not in n,
+        # the timed kernels: fourth template argument false (true marks the
+        # accounting variant)
+        "insert": lambda n: re.search(r"k_tcf_insert<\d+, \d+, \d+, false", n) is not None,
+        "query": lambda n: re.search(r"k_tcf_query<\d+, \d+, \d+, false", n) is not None,
         "copy 1 GiB": lambda n: "neg" in n and "double" in n,
         "random 64 B x 2^24": lambda n: "k_random_sector" in n,
     }
@@ -730,7 +732,8 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir, tcc_dir, rdreq_dir, wrre
     lines += [f"    {PROFILE_CMDS[k]}" for k in ("stats", "fetch", "write", "tcc", "rdreq", "wrreq", "summary")]
     lines += ["", "## Kernel statistics (`--kernel-trace --stats`)", "",
               "Kernels whose fourth template argument is `true` are the accounting variants,",
-              "run once, untimed, to count bucket reads; the timed kernels are the `false` ones.", "",
+              "run once, untimed, to count bucket reads; the timed kernels are the `false` ones.",
+              "A sixth argument `true` marks the plain insert built for 8 waves per SIMD.", "",
               "| kernel | calls | average us | total ms | % |", "|---|---|---|---|---|"]
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:12]:
         lines.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "

@@ -845,8 +845,14 @@ __device__ bool tcf_remove_key(const TableDev &T, uint64_t key, const Tile<P> &t
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
 
-template <int P, int B, int I, bool ACC, class F = Fmt16>
-__global__ __launch_bounds__(kBlock) void k_tcf_insert(TableDev T, OpArgs a) {
+// W8: plain inserts of the one-key register path ask the compiler for 8
+// waves per SIMD (it then trims the scalar registers that otherwise hold the
+// kernel at 7; the same 55 VGPRs, no scratch).  insert_with_delete keeps 7:
+// with more keys in flight the fills they read are staler, and under
+// sustained churn with tombstone reuse that fills the backing level faster.
+template <int P, int B, int I, bool ACC, class F = Fmt16, bool W8 = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W8 ? 8 : 1, 8)))
+void k_tcf_insert(TableDev T, OpArgs a) {
     Tile<P> t;
     t.concurrent = true;
     const uint64_t tile_id = (uint64_t(blockIdx.x) * kBlock + threadIdx.x) / P;
@@ -1061,8 +1067,11 @@ inline unsigned tcf_grid(uint64_t tiles, int P) {
 template <int P, int B, int I, bool ACC, class F = Fmt16>
 void launch_tcf_kernel(Op op, const TableDev &T, const OpArgs &a, hipStream_t s) {
     const uint64_t tiles = (a.n + I - 1) / I;
+    constexpr bool w8 = !ACC && I == 1 && B != 0 && B <= 16 && P <= 16;
     if (op == Op::Query)
         hipLaunchKernelGGL((k_tcf_query<P, B, I, ACC, F>), dim3(tcf_grid(tiles, P)), dim3(kBlock), 0, s, T, a);
+    else if (w8 && !a.with_delete)
+        hipLaunchKernelGGL((k_tcf_insert<P, B, I, ACC, F, w8>), dim3(tcf_grid(tiles, P)), dim3(kBlock), 0, s, T, a);
     else
         hipLaunchKernelGGL((k_tcf_insert<P, B, I, ACC, F>), dim3(tcf_grid(tiles, P)), dim3(kBlock), 0, s, T, a);
 }

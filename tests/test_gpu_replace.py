@@ -106,7 +106,15 @@ def test_concurrent_reinsert_replaces_values(name):
     st, fails = ops.insert(keys, vals_for(desc, keys, 1).astype(vdt))
     ok = st > 0
     fill = g.get_fill()
-    assert fill == int(ok.sum())
+    if desc.rep == pg.REP_KV16:
+        # a 16/16 slot stores the key's low 16 bits: a key that meets an
+        # earlier key with the same low bits in a probe bucket replaces that
+        # key's value instead of taking a slot (the oracle does the same when
+        # the earlier key lands first).  About 82 pairs of these 3 276 keys
+        # share their low bits, and about 2 % of those share a probe bucket.
+        assert 0 <= int(ok.sum()) - fill <= 8
+    else:
+        assert fill == int(ok.sum())
     kept = keys[ok]
     new = vals_for(desc, kept, 7).astype(vdt)
     st2, fails2 = ops.insert(kept, new)
