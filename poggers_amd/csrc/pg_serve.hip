@@ -61,13 +61,16 @@ __device__ inline uint32_t find_req(const uint64_t *off, uint32_t n, uint64_t i)
     return lo;
 }
 
-template <int P>
+// RP: the server holds a table with a replace-scheme level; its kernel then
+// carries that insert (a separate instantiation, so other servers' kernels
+// carry none of its code).
+template <int P, bool RP>
 __device__ inline void serve_one(const TableDev &T, const ReqDev &r, uint64_t j, const Tile<P> &t) {
     const uint64_t key = r.keys[j];
     if (T.rep == PG_REP_KV64) {
         if (r.op == PG_OP_INSERT) {
             const uint64_t v = r.vals ? ((const uint64_t *)r.vals)[j] : 0ull;
-            const int st = kv_insert_key<P>(T, key, v, t);
+            const int st = kv_insert_key<P, SlotKV, RP>(T, key, v, t);
             if (r.flags && t.leader()) r.flags[j] = (uint8_t)st;
         } else if (r.op == PG_OP_QUERY) {
             uint64_t v = 0;
@@ -86,7 +89,7 @@ __device__ inline void serve_one(const TableDev &T, const ReqDev &r, uint64_t j,
     if (T.rep == PG_REP_KV16) {  // the flat TCF's 16/16 key-value slots
         if (r.op == PG_OP_INSERT) {
             const uint64_t v = r.vals ? ((const uint16_t *)r.vals)[j] : 0u;
-            const int st = kv_insert_key<P, SlotKV16>(T, key, v, t);
+            const int st = kv_insert_key<P, SlotKV16, RP>(T, key, v, t);
             if (r.flags && t.leader()) r.flags[j] = (uint8_t)st;
         } else if (r.op == PG_OP_QUERY) {
             uint64_t v = 0;
@@ -143,7 +146,7 @@ __device__ inline void serve_one(const TableDev &T, const ReqDev &r, uint64_t j,
 // in front of every key.  The grid is capped at 2^31 work-items (HIP's limit
 // for one dimension is below 2^32), and a tile takes key i + k * tiles until
 // the batch is done.
-template <int P>
+template <int P, bool RP>
 __global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
     Tile<P> t;
     t.concurrent = true;
@@ -157,22 +160,27 @@ __global__ __launch_bounds__(256) void k_serve(ServeArgs a) {
             r = find_req(a.off, a.nreq, i);
         }
         const ReqDev &rq = a.reqs[r];
-        serve_one<P>(a.tables[rq.table], rq, i - a.off[r], t);
+        serve_one<P, RP>(a.tables[rq.table], rq, i - a.off[r], t);
     }
 }
 
-static int launch_serve(uint32_t P, const ServeArgs &a, hipStream_t s) {
+template <bool RP>
+static int launch_serve_rp(uint32_t P, const ServeArgs &a, hipStream_t s) {
     const uint64_t want = (a.total * P + 255) / 256;
     const unsigned blocks = (unsigned)(want < (1ull << 23) ? want : (1ull << 23));
     switch (P) {
-        case 1: hipLaunchKernelGGL((k_serve<1>), dim3(blocks), dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_serve<2>), dim3(blocks), dim3(256), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_serve<4>), dim3(blocks), dim3(256), 0, s, a); break;
-        case 8: hipLaunchKernelGGL((k_serve<8>), dim3(blocks), dim3(256), 0, s, a); break;
-        case 16: hipLaunchKernelGGL((k_serve<16>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 1: hipLaunchKernelGGL((k_serve<1, RP>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_serve<2, RP>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_serve<4, RP>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_serve<8, RP>), dim3(blocks), dim3(256), 0, s, a); break;
+        case 16: hipLaunchKernelGGL((k_serve<16, RP>), dim3(blocks), dim3(256), 0, s, a); break;
         default: return PG_ERR_INVALID;
     }
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
+}
+
+static int launch_serve(uint32_t P, bool rp, const ServeArgs &a, hipStream_t s) {
+    return rp ? launch_serve_rp<true>(P, a, s) : launch_serve_rp<false>(P, a, s);
 }
 
 }  // namespace pg
@@ -264,6 +272,7 @@ struct pg_server_s {
     std::vector<pg::TableDev> h_tables;
     pg::TableDev *d_tables = nullptr;
     uint32_t P = 0;
+    bool rp = false;  // a table has a replace-scheme level: serve with that kernel
     char *arena = nullptr;
     pg::Ring ring;
     std::vector<hipStream_t> streams;
@@ -305,13 +314,8 @@ int pg_server_create(const pg_handle *tables, uint32_t ntables, uint64_t budget_
             delete s;
             return PG_ERR_UNSUPPORTED;
         }
-        // the served kernel carries no replace-scheme insert
-        for (uint32_t l = 0; l < T->nlevels; l++) {
-            if (T->lv[l].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE) {
-                delete s;
-                return PG_ERR_UNSUPPORTED;
-            }
-        }
+        for (uint32_t l = 0; l < T->nlevels; l++)
+            if (T->lv[l].scheme == PG_SCHEME_POWER_OF_N_SHORTCUT_REPLACE) s->rp = true;
         s->P = tile;
         s->h_tables[i] = *T;
     }
@@ -529,7 +533,7 @@ static int launch_batch(pg_server s, uint32_t si) {
         a.idx = use_idx ? (const uint32_t *)((char *)d_off + idx_at) : nullptr;
         a.nreq = (uint32_t)take;
         a.total = total;
-        const int rc = pg::launch_serve(s->P, a, st);
+        const int rc = pg::launch_serve(s->P, s->rp, a, st);
         if (rc != PG_OK) return fail(rc);
     }
     for (size_t i = 0; i < take; i++) {

@@ -8,8 +8,8 @@ with the oracle through inserts, re-inserts with new values, removes and
 re-inserts after removes, for key_val_pair<u64,u64>, the 16/16 key-value
 slot (with the flat TCF's bucket backing level) and key-only slots.
 Concurrently: re-inserting every stored key with a new value takes no new
-slot and every key then reads its new value; a server refuses such tables
-(the served kernel has no replace insert)."""
+slot and every key then reads its new value, also through the serving
+loop."""
 import numpy as np
 import pytest
 
@@ -130,7 +130,41 @@ def test_concurrent_reinsert_replaces_values(name):
         assert (v == new).mean() > 0.99
 
 
-def test_server_refuses_replace_tables():
-    t = pg.Table(kv64(1 << 10))
-    with pytest.raises(pg.PgError):
-        pg.Server([t])
+def _drain(srv, tickets):
+    done = []
+    for _ in range(1_000_000):
+        done += srv.progress()
+        if set(tickets) <= set(done):
+            return
+    raise AssertionError("requests did not finish")
+
+
+def test_server_serves_replace_tables():
+    """A server holding a replace-scheme table (next to a plain one) serves
+    its inserts with the replace kernel: a second insert of the same keys
+    with new values takes no slot, and served queries read the new values."""
+    import torch
+    # the bucket_insert table stores every key twice: twice the slots
+    rt, pt = pg.Table(kv64(1 << 14)), pg.Table(pg.kv_desc(1 << 15, bucket_size=16, tile=4, prober=pg.PROBE_DOUBLE,
+                                                               max_probes=20))
+    srv = pg.Server([rt, pt], budget_bytes=1 << 24, nstreams=2)
+    n = 8000
+    keys = oracle.gen_keys(141, 0, n)
+    pin = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a).view(dt)).pin_memory()
+    kk = pin(keys, np.int64)
+    flags = [torch.zeros(n, dtype=torch.uint8).pin_memory() for _ in range(4)]
+    v1, v2 = pin(keys ^ np.uint64(0x1111), np.int64), pin(keys ^ np.uint64(0x2222), np.int64)
+    for tab in (0, 1):
+        _drain(srv, [srv.submit(tab, pg.OP_INSERT, kk, v1, None, flags[0])])
+        assert flags[0].numpy().all(), tab
+        _drain(srv, [srv.submit(tab, pg.OP_INSERT, kk, v2, None, flags[1])])
+        assert flags[1].numpy().all(), tab
+        out = torch.zeros(n, dtype=torch.int64).pin_memory()
+        _drain(srv, [srv.submit(tab, pg.OP_QUERY, kk, None, out, flags[2])])
+        assert flags[2].numpy().all()
+        got = out.numpy().view(np.uint64)
+        if tab == 0:     # replace: one slot per key, the new values
+            assert rt.get_fill() == n and (got == (keys ^ np.uint64(0x2222))).all()
+        else:            # bucket_insert: a second copy, the query finds the first
+            assert pt.get_fill() == 2 * n and (got == (keys ^ np.uint64(0x1111))).all()
+    srv.free()
