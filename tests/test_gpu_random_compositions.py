@@ -1,0 +1,131 @@
+"""Randomized differential test: random table compositions, each run in
+deterministic mode through inserts, queries, removes, tombstone-reusing
+re-inserts and upserts, checked against the oracle after every step
+(per-key statuses, every level's slot bytes, fill, query answers and values).
+
+A composition draws its slot representation, tile width, bucket size,
+insert scheme per level (primary plus up to two backing levels), prober,
+hasher and probe count among the combinations the C ABI accepts (a draw
+that pg_create refused would fail its test).  Seeds are fixed, so a failure
+names its composition."""
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+import poggers_amd as pg
+from tests.gpu_util import Ops, pair
+
+pytestmark = pytest.mark.gpu
+
+FILTERS = (pg.REP_FP16_V16, pg.REP_FP8_V8, pg.REP_FP10_V6)
+KVS = (pg.REP_KV64, pg.REP_KEY64, pg.REP_KV16)
+KV_SCHEMES = (pg.SCHEME_BUCKET, pg.SCHEME_SINGLE_SLOT, pg.SCHEME_POWER_OF_N,
+              pg.SCHEME_POWER_OF_N_SHORTCUT_SLOTS, pg.SCHEME_POWER_OF_N_SHORTCUT_REPLACE)
+
+
+def two_choice(rng, rep):
+    probers = [pg.PROBE_DOUBLE, pg.PROBE_P2, pg.PROBE_XOR_P2, pg.PROBE_SPLIT64]
+    if rep == pg.REP_FP12:
+        probers.remove(pg.PROBE_XOR_P2)   # twelve_bucket has no tag
+    p = rng.choice(probers)
+    return p, (rng.choice([2, 3, 4]) if p == pg.PROBE_DOUBLE else 2)
+
+
+def random_desc(rng):
+    rep = rng.choice(FILTERS + KVS + (pg.REP_FP12,))
+    tile = rng.choice([1, 2, 4, 8, 16, 32, 64])
+    hasher = rng.choice([pg.HASH_FMIX64, pg.HASH_MURMUR64A])
+    n0 = rng.choice([1 << 10, 3000, 1 << 12, 5000])
+    levels = []
+    for lvl in range(rng.choice([1, 2, 2, 3])):
+        nslots = n0 if lvl == 0 else max(64, n0 >> rng.choice([3, 4, 5]))
+        if rep in FILTERS or rep == pg.REP_FP12:
+            if lvl > 0 and rep != pg.REP_FP12 and rng.random() < 0.7:
+                prober = rng.choice([pg.PROBE_LINEAR, pg.PROBE_DOUBLE])
+                levels.append(pg.Level(nslots, pg.SCHEME_LINEAR_BUCKETS, rng.choice([4, 8, 16]), tile, prober,
+                                       hasher, 4000 if prober == pg.PROBE_LINEAR else 20))
+            else:
+                prober, probes = two_choice(rng, rep)
+                levels.append(pg.Level(nslots, pg.SCHEME_SHORTCUT, rng.choice([8, 12, 16, 32]), tile, prober,
+                                       hasher, probes))
+        else:
+            scheme = rng.choice(KV_SCHEMES)
+            if scheme == pg.SCHEME_SINGLE_SLOT:
+                B = tile                      # single_slot_insert: Partition_Size == Bucket_Size
+            else:
+                B = rng.choice([1, 2, 4, 8, 16]) if scheme == pg.SCHEME_BUCKET else rng.choice([4, 8, 16])
+            if scheme in (pg.SCHEME_BUCKET, pg.SCHEME_SINGLE_SLOT):
+                prober = rng.choice([pg.PROBE_LINEAR, pg.PROBE_DOUBLE])
+                probes = 20 if prober == pg.PROBE_DOUBLE else 64
+            else:
+                prober, probes = two_choice(rng, rep)
+            levels.append(pg.Level(nslots, scheme, B, tile, prober, hasher, probes))
+    return pg.TableDesc(rep, levels)
+
+
+def compositions(count, seed=2024):
+    rng = random.Random(seed)
+    return [random_desc(rng) for _ in range(count)]
+
+
+def vals_for(desc, keys, salt):
+    if desc.rep == pg.REP_KV64:
+        return keys * np.uint64(3) + np.uint64(salt)
+    v = ((keys >> np.uint64(20)) + np.uint64(salt)) & np.uint64(0xFFFF)
+    if desc.rep in (pg.REP_FP8_V8, pg.REP_FP10_V6):
+        return (v & np.uint64(0xFF)).astype(np.uint8)
+    return v.astype(np.uint16)
+
+
+def check_state(o, g, ops, desc, q, what):
+    for lvl in range(len(desc.levels)):
+        assert g.level_bytes(lvl) == o.level_bytes(lvl), (what, lvl)
+    assert g.get_fill() == o.fill(), what
+    f_o, v_o = o.bulk_query(q)
+    f_g, v_g = ops.query(q)
+    assert (f_g == f_o).all(), what
+    assert (v_g == v_o.astype(v_g.dtype)).all(), what
+
+
+CASES = compositions(120)
+
+
+@pytest.mark.parametrize("i", range(len(CASES)))
+def test_random_composition_bit_exact(i):
+    desc = CASES[i]
+    rng = random.Random(i)
+    o, g = pair(desc, seed=42 + i)
+    g.set_mode(pg.DETERMINISTIC)
+    ops = Ops(g)
+    n0 = desc.levels[0].nslots
+    keys = oracle.gen_keys(500 + i, 0, int(rng.uniform(0.5, 1.05) * n0))
+    q = np.concatenate([keys, oracle.gen_keys(500 + i, 1 << 40, 1024)])
+    vs = vals_for(desc, keys, 1)
+    st_o = o.bulk_insert(keys, vs)
+    st_g, fails = ops.insert(keys, vs)
+    assert (st_g == st_o).all() and fails == int((st_o == 0).sum()), (i, desc)
+    check_state(o, g, ops, desc, q, (i, "insert"))
+    removable = desc.rep != pg.REP_FP12 and all(lv.scheme != pg.SCHEME_SINGLE_SLOT for lv in desc.levels)
+    if not removable:
+        return
+    rm = keys[::3]
+    assert (ops.remove(rm) == o.bulk_remove(rm)).all(), (i, "remove")
+    check_state(o, g, ops, desc, q, (i, "remove"))
+    more = np.concatenate([rm[::2], oracle.gen_keys(600 + i, 0, len(keys) // 6)])
+    wd = desc.rep in FILTERS
+    vm = vals_for(desc, more, 2)
+    st_o = o.bulk_insert(more, vm, with_delete=wd)
+    st_g, _ = ops.insert(more, vm, with_delete=wd)
+    assert (st_g == st_o).all(), (i, "re-insert")
+    check_state(o, g, ops, desc, q, (i, "re-insert"))
+    if desc.rep in FILTERS:
+        up = np.concatenate([keys[1::4], oracle.gen_keys(700 + i, 0, len(keys) // 8)])
+        vu = vals_for(desc, up, 3)
+        for delete in (False, True):
+            st_o, f_o, v_o = o.bulk_insert_if_not_exists(up, vu.astype(np.uint16), delete=delete)
+            st_g, f_g, v_g = ops.upsert(up, vu, delete=delete)
+            assert (st_g == st_o).all() and (f_g == f_o).all(), (i, "upsert", delete)
+            assert (v_g[f_g > 0] == v_o[f_o > 0].astype(v_g.dtype)).all(), (i, "upsert values", delete)
+            check_state(o, g, ops, desc, q, (i, "upsert", delete))
