@@ -129,3 +129,52 @@ def test_random_composition_bit_exact(i):
             assert (st_g == st_o).all() and (f_g == f_o).all(), (i, "upsert", delete)
             assert (v_g[f_g > 0] == v_o[f_o > 0].astype(v_g.dtype)).all(), (i, "upsert values", delete)
             check_state(o, g, ops, desc, q, (i, "upsert", delete))
+
+
+@pytest.mark.parametrize("i", range(len(CASES)))
+def test_random_composition_concurrent_properties(i):
+    """The same compositions with every key of a batch in flight at once:
+    placements then depend on timing, so the checks are the properties that
+    hold for any interleaving.  Every stored key is found; the fill equals
+    the stored keys (a 16-bit key of a replace level may instead take over
+    an earlier key's slot that has the same low bits); full-width keys
+    (key_val_pair<u64,u64>) read back their own values; each successful
+    remove lowers the fill by one."""
+    desc = CASES[i]
+    rng = random.Random(1000 + i)
+    g = pg.Table(desc, seed=42 + i)
+    ops = Ops(g)
+    n0 = desc.levels[0].nslots
+    keys = oracle.gen_keys(800 + i, 0, int(rng.uniform(0.5, 0.95) * n0))
+    vs = vals_for(desc, keys, 1)
+    st, fails = ops.insert(keys, vs)
+    ok = st > 0
+    assert fails == int((~ok).sum()), i
+    fill = g.get_fill()
+    replace16 = desc.rep == pg.REP_KV16 and any(lv.scheme == pg.SCHEME_POWER_OF_N_SHORTCUT_REPLACE
+                                                for lv in desc.levels)
+    # 10/6 slots: a key whose remapped fp16 has low 10 bits 0 or 0x3FF is
+    # stored as an empty- or tombstone-looking word, which the fill does not
+    # count (and a later insert may claim); about 2 in 1 024 keys
+    quirk = 0
+    if desc.rep == pg.REP_FP10_V6:
+        low = keys[ok] & np.uint64(0xFFFF)
+        fp = np.where(low == 0, 1, np.where(low == 0xFFFF, 0xFFFE, low)).astype(np.uint64) & np.uint64(0x3FF)
+        quirk = int(((fp == 0) | (fp == 0x3FF)).sum())
+    if replace16:
+        assert 0 <= int(ok.sum()) - fill <= 16, (i, fill, int(ok.sum()))
+    else:
+        assert int(ok.sum()) - quirk <= fill <= int(ok.sum()), (i, fill, int(ok.sum()), quirk)
+    f, v = ops.query(keys[ok])
+    assert f.all(), (i, "false negative")
+    if desc.rep == pg.REP_KV64:
+        assert (v == vs[ok]).all(), (i, "values")
+    removable = desc.rep != pg.REP_FP12 and all(lv.scheme != pg.SCHEME_SINGLE_SLOT for lv in desc.levels)
+    if removable:
+        rm = keys[ok][::3]
+        removed = ops.remove(rm)
+        after = g.get_fill()
+        # a 10/6 remove can bury an empty-looking word, which was not counted
+        assert fill - int(removed.sum()) <= after <= fill - int(removed.sum()) + quirk, (i, "remove")
+        if desc.rep == pg.REP_KV64:
+            assert removed.all(), (i, "remove found")
