@@ -348,3 +348,61 @@ def test_full_size_batched_workload_through_the_graph(tile, B):
     lo, hi = min(arr, shf), max(arr, shf)
     assert 0.95 * lo - 4 * np.sqrt(lo + 1) <= gpu <= 1.05 * hi + 4 * np.sqrt(hi + 1)
     b.free()
+
+
+def _random_shapes(count, seed=77):
+    import random
+    rng = random.Random(seed)
+    out = []
+    for _ in range(count):
+        tile = rng.choice([1, 2, 4, 8, 16, 32, 64])
+        B = rng.choice([8, 16, 32])
+        prober = rng.choice([pg.PROBE_SPLIT64, pg.PROBE_XOR_P2, pg.PROBE_P2, pg.PROBE_DOUBLE])
+        hasher = rng.choice([pg.HASH_FMIX64, pg.HASH_MURMUR64A])
+        desc = pg.tcf_desc(1 << 12, 1 << 6, bucket_size=B, tile=tile, prober=prober, hasher=hasher)
+        out.append((desc, rng.choice([1, 2, 4]), rng.choice([256, 1024, 4096]), rng.randrange(1 << 30)))
+    return out
+
+
+RANDOM_SHAPES = _random_shapes(24)
+
+
+@pytest.mark.parametrize("k", range(len(RANDOM_SHAPES)))
+def test_random_batch_shapes(k):
+    """Random batch shapes (tile, bucket size, prober, hasher, keys per tile,
+    chunk): in key order every table's statuses, slot bytes, answers and
+    fill equal the oracle's; with the persistent kernel every stored key is
+    found and each table's fill equals its stored keys."""
+    desc, items, chunk, seed = RANDOM_SHAPES[k]
+    sizes, ins, ins_off, q, q_off = make_workload(24, 8, 12, 0.9, seed % 1000)
+    oracles = [oracle_table(desc, ns) for ns in sizes]
+    want_st = np.concatenate([o.bulk_insert(ins[int(ins_off[t]):int(ins_off[t + 1])])
+                              for t, o in enumerate(oracles)])
+    for mode in (pg.DETERMINISTIC, pg.CONCURRENT):
+        b = pg.Batch(desc, sizes)
+        b.set_mode(mode)
+        b.set_items(items)
+        b.set_launch(chunk)
+        st = torch.zeros(len(ins), dtype=torch.uint8, device="cuda")
+        b.insert(dev_u64(ins), dev_u64(ins_off), None, st)
+        found = torch.zeros(len(q), dtype=torch.uint8, device="cuda")
+        vals = torch.zeros(len(q), dtype=torch.int16, device="cuda")
+        b.query(dev_u64(q), dev_u64(q_off), vals, found)
+        torch.cuda.synchronize()
+        st, found, vals = st.cpu().numpy(), found.cpu().numpy(), host(vals, np.uint16)
+        for t, o in enumerate(oracles):
+            a, z = int(ins_off[t]), int(ins_off[t + 1])
+            qa, qz = int(q_off[t]), int(q_off[t + 1])
+            if mode == pg.DETERMINISTIC:
+                assert (st[a:z] == want_st[a:z]).all(), (k, t)
+                for lvl in range(2):
+                    img = np.frombuffer(b.level_bytes(t, lvl), dtype=np.uint32)
+                    assert (img == o.level_array(lvl)).all(), (k, t, lvl)
+                f_o, v_o = o.bulk_query(q[qa:qz])
+                assert (found[qa:qz] == f_o).all() and (vals[qa:qz] == v_o).all(), (k, t)
+                assert b.fill(t) == o.fill(), (k, t)
+            else:
+                ok = st[a:z] > 0
+                npres = (z - a) // 2
+                assert found[qa:qa + npres][ok[:npres]].all(), (k, t, "false negative")
+                assert b.fill(t) == int(ok.sum()), (k, t)
