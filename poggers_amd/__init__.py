@@ -548,11 +548,18 @@ class Server:
               "pg_server_create")
         self._h = h
         self._done = (ctypes.c_uint64 * 65536)()
+        # ticket -> the request's tensors (or its packed array), held until
+        # progress() reports the ticket: the server reads and writes them
+        # in place, so they must outlive the request
+        self._live = {}
 
     def free(self):
         if getattr(self, "_h", None):
-            check(lib().pg_server_destroy(self._h), "pg_server_destroy")
-            self._h = None
+            try:
+                check(lib().pg_server_destroy(self._h), "pg_server_destroy")  # waits for in-flight work
+            finally:
+                self._h = None
+                self._live.clear()
 
     def __del__(self):
         try:
@@ -582,6 +589,7 @@ class Server:
         t = ctypes.c_uint64()
         k, n, v, o, f = self._request(self.tables, table, op, keys, vals, vals_out, flags)
         check(lib().pg_server_submit(self._h, table, op, k, v, n, o, f, ctypes.byref(t)), "pg_server_submit")
+        self._live[t.value] = (keys, vals, vals_out, flags)
         return t.value
 
     @staticmethod
@@ -590,6 +598,7 @@ class Server:
         vals_out, flags) tuples of tensors (None where absent).  Each tensor is
         checked; pass the server's `tables` to check value widths too."""
         arr = (Request * len(requests))()
+        arr._tensors = [tuple(r[2:]) for r in requests]  # kept alive with the array
         for i, (table, op, keys, vals, out, flags) in enumerate(requests):
             k, n, v, o, f = Server._request(tables, table, op, keys, vals, out, flags)
             r = arr[i]
@@ -609,8 +618,10 @@ class Server:
         tickets = (ctypes.c_uint64 * count)()
         nq = ctypes.c_uint32()
         ptr = ctypes.cast(ctypes.byref(packed, start * ctypes.sizeof(Request)), ctypes.c_void_p)
-        check(lib().pg_server_submit_many(self._h, ptr, count, tickets, ctypes.byref(nq)),
-              "pg_server_submit_many")
+        rc = lib().pg_server_submit_many(self._h, ptr, count, tickets, ctypes.byref(nq))
+        for tk in list(tickets)[:nq.value]:   # the requests queued (all of them on success)
+            self._live[tk] = packed
+        check(rc, "pg_server_submit_many")
         return list(tickets)
 
     def progress(self, cap: int = 65536):
@@ -619,7 +630,10 @@ class Server:
         nd = ctypes.c_uint32()
         check(lib().pg_server_progress(self._h, self._done, min(cap, 65536), ctypes.byref(nd)),
               "pg_server_progress")
-        return [self._done[i] for i in range(nd.value)]
+        done = [self._done[i] for i in range(nd.value)]
+        for tk in done:
+            self._live.pop(tk, None)
+        return done
 
     def set_batch(self, max_batch_keys: int, max_batch_requests: int = 4096):
         check(lib().pg_server_set_batch(self._h, max_batch_keys, max_batch_requests), "pg_server_set_batch")

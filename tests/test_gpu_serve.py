@@ -326,3 +326,42 @@ def test_flat_tcf_served_like_plain_calls():
     assert tabs[0].get_fill() == tabs[1].get_fill() == len(keys) - n_rm
     srv.free()
     plain.free()
+
+
+def test_binding_keeps_request_tensors_until_done():
+    """The server reads and writes request buffers in place after submit
+    returns: the binding holds each request's tensors until progress()
+    reports its ticket, so temporaries passed to submit / submit_many stay
+    valid, and the results land in them."""
+    import gc
+    tabs = build_tables()[:1]
+    srv = pg.Server(tabs, budget_bytes=1 << 22, nstreams=2)
+    keys = oracle.gen_keys(91, 0, 20000)
+    kk, vv, _, ff = buffers(tabs[0], pg.OP_INSERT, keys)
+    t0 = srv.submit(0, pg.OP_INSERT, kk, vv, None, ff)
+    flags_ref = ff
+    del kk, vv, ff
+    gc.collect()
+    assert t0 in srv._live
+    done = []
+    for _ in range(1_000_000):
+        done += srv.progress()
+        if t0 in done:
+            break
+    assert t0 not in srv._live and flags_ref.numpy().all()
+    # submit_many: the packed array keeps its tensors; the tickets keep the array
+    qk, _, qo, qf = buffers(tabs[0], pg.OP_QUERY, keys)
+    packed = pg.Server.pack([(0, pg.OP_QUERY, qk, None, qo, qf)], tabs)
+    out_ref, found_ref = qo, qf
+    del qk, qo, qf
+    tickets = srv.submit_many(packed)
+    del packed
+    gc.collect()
+    assert all(tk in srv._live for tk in tickets)
+    done = []
+    for _ in range(1_000_000):
+        done += srv.progress()
+        if set(tickets) <= set(done):
+            break
+    assert not srv._live and found_ref.numpy().all()
+    srv.free()
