@@ -345,7 +345,8 @@ int pg_batch_insert(pg_batch b, const uint64_t *d_keys, const uint64_t *d_offset
 int pg_batch_query(pg_batch b, const uint64_t *d_keys, const uint64_t *d_offsets, uint64_t total,
                    uint16_t *d_vals_out, uint8_t *d_found, void *stream);
 /* Record [clear] + insert + query into a hipGraph held by the batch (kernels
- * only, no memset nodes); pg_batch_replay launches it. */
+ * only, no memset nodes); pg_batch_replay launches it.  The graph holds the
+ * buffers' addresses: they must stay allocated while it can be replayed. */
 int pg_batch_capture(pg_batch b, const uint64_t *d_ins_keys, const uint64_t *d_ins_offsets, uint64_t ins_total,
                      const uint64_t *d_q_keys, const uint64_t *d_q_offsets, uint64_t q_total, uint8_t *d_status,
                      uint8_t *d_found, uint64_t *d_fail_count, uint32_t clear_first, void *stream);

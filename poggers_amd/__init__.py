@@ -505,12 +505,18 @@ class Batch:
 
     def capture(self, ins_keys, ins_offsets, q_keys, q_offsets, status=None, found=None, fail_count=None,
                 clear_first=True, stream=None):
+        """Record clear (optional) + insert + query in a hipGraph that
+        replay() relaunches.  The graph records the tensors' addresses, so
+        the Batch keeps them until the next capture; refill them in place
+        between replays."""
         ik, ni = _keys(ins_keys)
         qk, nq = _keys(q_keys)
         check(lib().pg_batch_capture(self._h, ik, self._offsets(ins_offsets, "ins_offsets"), ni, qk,
                                      self._offsets(q_offsets, "q_offsets"), nq, _arg(status, "status", ni, 1),
                                      _arg(found, "found", nq, 1), _arg(fail_count, "fail_count", 1, 8),
                                      int(clear_first), _stream(stream)), "pg_batch_capture")
+        # the graph holds these tensors' addresses: keep them for its replays
+        self._graph_tensors = (ins_keys, ins_offsets, q_keys, q_offsets, status, found, fail_count)
 
     def replay(self, stream=None):
         check(lib().pg_batch_replay(self._h, _stream(stream)), "pg_batch_replay")
