@@ -158,8 +158,12 @@ __device__ inline void batch_many(const BatchArgs &a, uint64_t i0, uint64_t end,
     }
 }
 
+// With one key per tile the compiler is asked for 7 waves per SIMD (72
+// VGPRs, 12 B of scratch) instead of its own 6 (75 VGPRs): 5-9 % faster on
+// 4096 uneven filters.  8 waves spill 48 B per lane and are slower.
 template <int P, int B, bool QUERY, int I>
-__global__ __launch_bounds__(kBatchBlock) void k_batch_persistent(BatchArgs a) {
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(I == 1 ? 7 : 1, 8)))
+void k_batch_persistent(BatchArgs a) {
     const uint64_t *off = a.off;
     Tile<P> t;
     t.concurrent = true;
