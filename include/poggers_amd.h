@@ -312,8 +312,21 @@ int pg_num_slots(pg_handle h, uint64_t *out);
 
 /* Device pointer and byte size of one level's slot array (raw bucket dump
  * for layout checks).  No reference counterpart: the reference never
- * exposes its slot arrays outside the device structs. */
+ * exposes its slot arrays outside the device structs.  After writing slots
+ * through the pointer, call pg_level_sync. */
 int pg_level_ptr(pg_handle h, uint32_t level, void **d_ptr, uint64_t *bytes);
+
+/* Rebuild what a level derives from its slots, after they were written
+ * through pg_level_ptr (pg_load does this itself).  A linear-bucket level of
+ * 16/16 or 8/8 filter slots keeps an occupancy index: one bit per bucket,
+ * set before any slot of the bucket is claimed and cleared only by
+ * pg_clear, so a clear bit proves the bucket empty and its queries and
+ * removes stop there without reading it (the reference's linear walk stops
+ * at the first bucket that is not full, linear_insert_buckets.cuh:323-398).
+ * The rebuild sets the bit of every bucket holding a non-empty word.  PG_OK
+ * and nothing to do for other levels.  Ordered on `stream`.  No reference
+ * counterpart. */
+int pg_level_sync(pg_handle h, uint32_t level, void *stream);
 
 /* ---- batches: many independent filters, one persistent launch ----
  * No reference counterpart: the reference launches one kernel per table

@@ -285,6 +285,15 @@ class Table:
         p = lib().or_level_data(self._h, lvl)
         return ctypes.string_at(p, n)
 
+    def set_level_bytes(self, lvl: int, data: bytes):
+        """Overwrite one level's slot array with a raw image (for instance a
+        concurrent GPU layout), so later operations run the sequential
+        algorithm over exactly those slots."""
+        n = lib().or_level_bytes(self._h, lvl)
+        if len(data) != n:
+            raise ValueError(f"image is {len(data)} bytes, level {lvl} holds {n}")
+        ctypes.memmove(lib().or_level_data(self._h, lvl), data, n)
+
     def level_array(self, lvl: int) -> np.ndarray:
         dt = {REP_KV64: np.uint64, REP_FP16_V16: np.uint32, REP_FP8_V8: np.uint16,
               REP_FP10_V6: np.uint16, REP_FP12: np.uint16, REP_KEY64: np.uint64, REP_KV16: np.uint32}[self.rep]

@@ -33,7 +33,7 @@ MAX_LEVELS = 4
 EXPORTS = [
     "pg_create", "pg_destroy", "pg_clear", "pg_set_mode", "pg_bulk_insert",
     "pg_bulk_insert_with_delete", "pg_bulk_query", "pg_bulk_remove", "pg_fill",
-    "pg_bytes", "pg_num_slots", "pg_level_ptr", "pg_error_string", "pg_gen_keys",
+    "pg_bytes", "pg_num_slots", "pg_level_ptr", "pg_level_sync", "pg_error_string", "pg_gen_keys",
     "pg_random_sector_bench", "pg_fastmod_host", "pg_set_stats", "pg_set_items",
     "pg_bulk_insert_if_not_exists", "pg_bulk_insert_if_not_exists_delete",
     "pg_batch_create", "pg_batch_destroy", "pg_batch_set_mode", "pg_batch_set_launch", "pg_batch_set_items",
@@ -112,6 +112,7 @@ def lib():
     L.pg_bytes.argtypes = [vp, ctypes.POINTER(u64)]
     L.pg_num_slots.argtypes = [vp, ctypes.POINTER(u64)]
     L.pg_level_ptr.argtypes = [vp, u32, ctypes.POINTER(vp), ctypes.POINTER(u64)]
+    L.pg_level_sync.argtypes = [vp, u32, vp]
     L.pg_error_string.argtypes = [i32]
     L.pg_error_string.restype = ctypes.c_char_p
     L.pg_gen_keys.argtypes = [u64, u64, u64, vp, vp]
@@ -438,6 +439,8 @@ class Table:
         torch.cuda.synchronize()
         buf = ctypes.create_string_buffer(data, n)
         _hip_memcpy(ptr, ctypes.addressof(buf), n, 1)  # hipMemcpyHostToDevice
+        check(lib().pg_level_sync(self._h, level, None), "pg_level_sync")
+        torch.cuda.synchronize()
 
 
 class Batch:

@@ -162,6 +162,14 @@ int pg_create(const pg_desc *desc, uint64_t seed, pg_handle *out) {
             break;
         }
         rc = pg::launch_zero(L.slots, bytes, nullptr);
+        if (rc == PG_OK && pg::has_occ_index(desc->rep, l.scheme)) {
+            if (hipMalloc((void **)&L.occ, pg::occ_bytes(nb)) != hipSuccess) {
+                L.occ = nullptr;
+                rc = PG_ERR_NOMEM;
+                break;
+            }
+            rc = pg::launch_zero(L.occ, pg::occ_bytes(nb), nullptr);
+        }
     }
     if (rc == PG_OK && hipMalloc((void **)&t->d_counter, 256) != hipSuccess) rc = PG_ERR_NOMEM;
     if (rc == PG_OK && hipDeviceSynchronize() != hipSuccess) rc = PG_ERR_HIP;
@@ -181,8 +189,10 @@ int pg_destroy(pg_handle h) {
         if (e != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
     };
     note(hipDeviceSynchronize());
-    for (uint32_t i = 0; i < h->dev.nlevels; i++)
+    for (uint32_t i = 0; i < h->dev.nlevels; i++) {
         if (h->dev.lv[i].slots) note(hipFree(h->dev.lv[i].slots));
+        if (h->dev.lv[i].occ) note(hipFree(h->dev.lv[i].occ));
+    }
     if (h->d_counter) note(hipFree(h->d_counter));
     delete h;
     return rc;
@@ -191,8 +201,9 @@ int pg_destroy(pg_handle h) {
 int pg_clear(pg_handle h, void *stream) {
     if (!h) return PG_ERR_INVALID;
     for (uint32_t i = 0; i < h->dev.nlevels; i++) {
-        const int rc = pg::launch_zero(h->dev.lv[i].slots, (h->bytes[i] + 255) & ~uint64_t(255),
-                                       (hipStream_t)stream);
+        const pg::LevelDev &L = h->dev.lv[i];
+        int rc = pg::launch_zero(L.slots, (h->bytes[i] + 255) & ~uint64_t(255), (hipStream_t)stream);
+        if (rc == PG_OK && L.occ) rc = pg::launch_zero(L.occ, pg::occ_bytes(L.nb.d), (hipStream_t)stream);
         if (rc != PG_OK) return rc;
     }
     return PG_OK;
@@ -428,6 +439,12 @@ int pg_level_ptr(pg_handle h, uint32_t level, void **d_ptr, uint64_t *bytes) {
     return PG_OK;
 }
 
+int pg_level_sync(pg_handle h, uint32_t level, void *stream) {
+    if (!h || level >= h->dev.nlevels) return PG_ERR_INVALID;
+    if (!h->dev.lv[level].occ) return PG_OK;
+    return pg::launch_occ_rebuild(h->dev.lv[level], h->dev.rep, (hipStream_t)stream);
+}
+
 // ---- raw bucket-array files ----
 static const char kMagic[8] = {'P', 'G', 'A', 'M', 'D', 'T', 'B', '1'};
 static const uint32_t kFileVersion = 3;  // 2: pg_level_desc carries group + reserved; 3: four levels
@@ -517,6 +534,8 @@ int pg_load(const char *path, pg_handle *out) {
     }
     if (buf && hipHostFree(buf) != hipSuccess && rc == PG_OK) rc = PG_ERR_HIP;
     fclose(f);
+    for (uint32_t l = 0; l < fh.desc.num_levels && rc == PG_OK; l++) rc = pg_level_sync(h, l, nullptr);
+    if (rc == PG_OK && hipDeviceSynchronize() != hipSuccess) rc = PG_ERR_HIP;
     if (rc != PG_OK) {
         if (h) pg_destroy(h);
         return rc;

@@ -315,18 +315,38 @@ __device__ int gen_fill(const void *base, uint32_t B, const Tile<P> &t) {
     return f;
 }
 
+// Occupancy index of a linear level (LevelDev::occ): set bucket b's bit.
+// A relaxed device-scope OR with no return value; it is issued before the
+// claim's CAS, and a kernel boundary or a release fence publishes both.
+template <int P>
+__device__ inline void occ_mark(uint32_t *occ, uint64_t b, const Tile<P> &t) {
+    if (t.leader())
+        __hip_atomic_fetch_or(occ + (b >> 5), 1u << (uint32_t)(b & 31u), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline bool occ_clear(const uint32_t *occ, uint64_t b) {
+    return ((occ[b >> 5] >> (uint32_t)(b & 31u)) & 1u) == 0u;
+}
+
 // Claim the lowest empty slot (with `wd`, the lowest empty-or-tombstone one)
 // with at most `tries` CAS attempts.  packed_bucket.cuh:64-180,
-// 12_bit_bucket.cuh:94-144.  Returns kClaimed, kNoSlot or kBudget.
+// 12_bit_bucket.cuh:94-144.  Returns kClaimed, kNoSlot or kBudget.  With an
+// occupancy index (occ, bucket b), the bucket's bit is set once the tile
+// sees a candidate slot, before its first CAS; walks past full buckets
+// set nothing.
 template <int P, class F = Fmt16>
 __device__ int gen_claim(void *base, uint32_t B, uint32_t word, bool wd, const Tile<P> &t,
-                         uint32_t tries = ~0u) {
+                         uint32_t tries = ~0u, uint32_t *occ = nullptr, uint64_t b = 0) {
     for (uint32_t s0 = 0; s0 < B; s0 += P) {
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
         const uint32_t w = ok ? s_get<F>(base, i) : 0u;
         bool cand = ok && (s_empty<F>(w) || (wd && s_tomb<F>(w)));
         uint64_t lanes = t.bits(cand);
+        if (lanes && occ) {
+            occ_mark(occ, b, t);
+            occ = nullptr;
+        }
         while (lanes) {
             if (tries == 0) return kBudget;
             tries--;
@@ -571,7 +591,7 @@ __device__ bool gen_linear_insert(const LevelDev &L, uint64_t key, uint32_t fp, 
         if (p == kEnd) return false;
         const uint64_t b = fastmod(p, L.nb);
         t.acc[ai]++;
-        if (gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t) == kClaimed) return true;
+        if (gen_claim<P, F>(bucket_ptr(L, b), L.B, word, wd, t, ~0u, L.occ, b) == kClaimed) return true;
     }
 }
 
@@ -584,7 +604,8 @@ __device__ bool gen_level_insert(const LevelDev &L, uint64_t key, uint32_t fp, u
 
 // Shortcut query (power_of_n_shortcut_buckets.cuh:676-707): every probe
 // bucket, first match wins.  Linear query (linear_insert_buckets.cuh:362-398):
-// stop at the first probe bucket whose fill != B.
+// stop at the first probe bucket whose fill != B; with an occupancy index, a
+// bucket whose bit is clear is empty and stops the walk unread.
 template <int P, class F = Fmt16>
 __device__ bool gen_level_query(const LevelDev &L, uint64_t key, uint32_t fp, const Tile<P> &t,
                                 uint32_t &val, bool &owner, int ai) {
@@ -596,6 +617,7 @@ __device__ bool gen_level_query(const LevelDev &L, uint64_t key, uint32_t fp, co
         const uint64_t p = pr.at(j);
         if (p == kEnd) return false;
         const uint64_t b = fastmod(p, L.nb);
+        if (L.occ && occ_clear(L.occ, b)) return false;
         int f;
         t.acc[ai]++;
         if (gen_find<P, F>(bucket_ptr(L, b), L.B, fp, t, val, owner, f)) return true;
@@ -604,7 +626,8 @@ __device__ bool gen_level_query(const LevelDev &L, uint64_t key, uint32_t fp, co
 }
 
 // Shortcut remove (power_of_n_shortcut_buckets.cuh:709-740); linear remove
-// (linear_insert_buckets.cuh:323-360) stops at the first bucket with fill != B.
+// (linear_insert_buckets.cuh:323-360) stops at the first bucket with fill != B
+// (or, as the query does, at a bucket whose occupancy bit is clear).
 template <int P, class F = Fmt16>
 __device__ bool gen_level_remove(const LevelDev &L, uint64_t key, uint32_t fp, const Tile<P> &t, int ai) {
     Probe pr;
@@ -615,6 +638,7 @@ __device__ bool gen_level_remove(const LevelDev &L, uint64_t key, uint32_t fp, c
         const uint64_t p = pr.at(j);
         if (p == kEnd) return false;
         const uint64_t b = fastmod(p, L.nb);
+        if (L.occ && occ_clear(L.occ, b)) return false;
         int f;
         t.acc[ai]++;
         if (gen_remove<P, F>(bucket_ptr(L, b), L.B, fp, t, f)) return true;
@@ -765,7 +789,8 @@ __device__ bool gen_level_upsert(const LevelDev &L, uint64_t key, uint32_t fp, u
     for (uint32_t j = 0;; j++) {
         const uint64_t p = pr.at(j);
         if (p == kEnd) break;
-        void *base = bucket_ptr(L, fastmod(p, L.nb));
+        const uint64_t b = fastmod(p, L.nb);
+        void *base = bucket_ptr(L, b);
         int f;
         t.acc[ai]++;
         if (gen_find<P, F>(base, L.B, fp, t, val, owner, f)) {
@@ -773,7 +798,7 @@ __device__ bool gen_level_upsert(const LevelDev &L, uint64_t key, uint32_t fp, u
             if (del) gen_remove<P, F>(base, L.B, fp, t, f);
             return true;
         }
-        if (linear && gen_claim<P, F>(base, L.B, word, false, t) == kClaimed) return true;
+        if (linear && gen_claim<P, F>(base, L.B, word, false, t, ~0u, L.occ, b) == kClaimed) return true;
     }
     if (linear) return false;
     return gen_shortcut_insert<P, F>(L, key, fp, word, del, t, ai);

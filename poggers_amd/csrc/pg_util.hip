@@ -129,6 +129,37 @@ int launch_zero(void *p, uint64_t bytes, hipStream_t s) {
     return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
 }
 
+// Occupancy index of a linear filter level from its slots: one thread per
+// index word sets bit j when bucket 32w + j holds any non-empty word (16-bit
+// words for the 8/8 slots, 32-bit for 16/16; key bits under kmask).
+template <class W>
+__global__ __launch_bounds__(kBlock) void k_occ_rebuild(const W *slots, uint64_t nb, uint32_t B, uint32_t kmask,
+                                                        uint32_t *occ) {
+    const uint64_t words = (nb + 31) / 32;
+    for (uint64_t w = uint64_t(blockIdx.x) * kBlock + threadIdx.x; w < words; w += uint64_t(gridDim.x) * kBlock) {
+        uint32_t bits = 0;
+        for (uint32_t j = 0; j < 32 && 32 * w + j < nb; j++) {
+            const W *bk = slots + (32 * w + j) * B;
+            bool any = false;
+            for (uint32_t i = 0; i < B && !any; i++) any = (bk[i] & kmask) != 0;
+            bits |= (any ? 1u : 0u) << j;
+        }
+        occ[w] = bits;
+    }
+}
+
+int launch_occ_rebuild(const LevelDev &L, uint32_t rep, hipStream_t s) {
+    if (!L.occ || !has_occ_index(rep, L.scheme)) return PG_ERR_INVALID;
+    const uint64_t words = (L.nb.d + 31) / 32;
+    if (rep == PG_REP_FP16_V16)
+        hipLaunchKernelGGL(k_occ_rebuild<uint32_t>, dim3(stride_grid(words)), dim3(kBlock), 0, s,
+                           (const uint32_t *)L.slots, L.nb.d, L.B, Fmt16::kmask, L.occ);
+    else
+        hipLaunchKernelGGL(k_occ_rebuild<uint16_t>, dim3(stride_grid(words)), dim3(kBlock), 0, s,
+                           (const uint16_t *)L.slots, L.nb.d, L.B, Fmt8::kmask, L.occ);
+    return hipGetLastError() == hipSuccess ? PG_OK : PG_ERR_HIP;
+}
+
 __global__ __launch_bounds__(kBlock) void k_gen_keys(uint64_t seed, uint64_t start, uint64_t n, uint64_t *out) {
     for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock)
         out[i] = key_at(seed, start + i);

@@ -387,3 +387,28 @@ def test_del_tcf_false_positive_rate_matches_the_reference_run():
     p = rec["del_tcf_2p20_fp_rate_at_0.9_load"]
     sigma = np.sqrt(p * (1 - p) / n + p * (1 - p) / len(absent))
     assert abs(fp - p) <= 4 * sigma, (fp, p, sigma)
+
+
+def test_set_level_bytes_carries_a_layout(oracle_lib):
+    """A slot image copied into a fresh table answers every query, and takes
+    later inserts, exactly as the table it came from; a wrong-sized image is
+    refused."""
+    o = oracle_lib
+    levels = [o.level(256, o.SCHEME_SHORTCUT, 16, prober=o.PROBE_DOUBLE),
+              o.level(64, o.SCHEME_LINEAR_BUCKETS, 8, prober=o.PROBE_LINEAR, max_probes=4000)]
+    a = o.Table(o.REP_FP16_V16, levels)
+    b = o.Table(o.REP_FP16_V16, levels)
+    keys = o.gen_keys(9, 0, 300)
+    a.bulk_insert(keys, (keys >> np.uint64(20)).astype(np.uint16))
+    assert a.level_bytes(1) != b.level_bytes(1)
+    for lvl in range(2):
+        b.set_level_bytes(lvl, a.level_bytes(lvl))
+    q = np.concatenate([keys, o.gen_keys(9, 1 << 40, 500)])
+    fa, va = a.bulk_query(q)
+    fb, vb = b.bulk_query(q)
+    assert fa.sum() > 250 and (fa == fb).all() and (va == vb).all()
+    more = o.gen_keys(9, 1 << 41, 10)
+    assert (a.bulk_insert(more) == b.bulk_insert(more)).all()
+    assert a.level_bytes(0) == b.level_bytes(0) and a.level_bytes(1) == b.level_bytes(1)
+    with pytest.raises(ValueError):
+        b.set_level_bytes(1, a.level_bytes(0))
