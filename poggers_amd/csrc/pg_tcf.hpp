@@ -251,6 +251,19 @@ __device__ inline bool s_tomb(uint32_t c) {
 }
 template <class F>
 __device__ inline bool s_filled(uint32_t c) { return !(s_empty<F>(c) || s_tomb<F>(c)); }
+// A slot a claim can take: the empty word or, with wd, the tombstone word
+// itself (the CAS expects exactly that word).  A 10/6 key whose stored bits
+// are 0 or 0x3FF leaves a word that only looks empty or buried, its value
+// bits being set; the reference's CAS fails on it and moves on
+// (grouped_storage_sub_bits.cuh:127-157), so it is no candidate.  Counting
+// it as one would also make a bounded concurrent claim read its certain
+// failure as a slot taken by another key.  For 16/16 and 8/8 slots these
+// are exactly the empty and tombstone slots.
+template <class F>
+__device__ inline bool s_claimable(uint32_t c, bool wd) {
+    if constexpr (is_twelve<F>) return c == 0u;
+    else return c == 0u || (wd && c == F::kmask);
+}
 // contains(): the key bits equal fp's; for the 12-bit bucket the composite
 // equals slot i's projection of the item (sub_byte_match).
 template <class F>
@@ -341,7 +354,7 @@ __device__ int gen_claim(void *base, uint32_t B, uint32_t word, bool wd, const T
         const uint32_t i = s0 + t.rank;
         const bool ok = i < B;
         const uint32_t w = ok ? s_get<F>(base, i) : 0u;
-        bool cand = ok && (s_empty<F>(w) || (wd && s_tomb<F>(w)));
+        bool cand = ok && s_claimable<F>(w, wd);
         uint64_t lanes = t.bits(cand);
         if (lanes && occ) {
             occ_mark(occ, b, t);

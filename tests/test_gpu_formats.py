@@ -168,3 +168,36 @@ def test_concurrent_no_false_negatives(case):
     rf = ops.remove(keys[found & ok & normal])
     assert rf.mean() > 0.99
     assert g.get_fill() == before - int(rf.sum())
+
+
+@pytest.mark.parametrize("with_delete", [False, True])
+def test_concurrent_claim_passes_over_empty_looking_words(with_delete):
+    """Every bucket of a one-level 10/6 filter holds, in slot 0, a word that
+    only looks empty (key bits 0, value bits set), or with delete reuse one
+    that only looks buried (key bits 0x3FF), then 11 filled slots.  The CAS
+    from the empty or tombstone word cannot take slot 0, so the reference
+    (and the oracle) claims slot 12, the lowest slot it can take.  A single
+    concurrent insert must do the same: the bounded first-pass claim may not
+    spend its one allowed attempt on slot 0 and give up on both buckets."""
+    B = 16
+    desc = pg.TableDesc(pg.REP_FP10_V6, [pg.Level(1 << 10, pg.SCHEME_SHORTCUT, B, 4, pg.PROBE_DOUBLE,
+                                                  pg.HASH_MURMUR64A, 2)])
+    o, g = pair(desc)
+    nb = (1 << 10) // B
+    img = np.zeros((nb, B), dtype=np.uint16)
+    img[:, 0] = (0x21 << 10) | (0x3FF if with_delete else 0)
+    img[:, 1:12] = (0x05 << 10) | 0x155
+    for src in (o, g):
+        if src is o:
+            o.set_level_bytes(0, img.tobytes())
+        else:
+            g.level_upload(0, img.tobytes())
+    key = oracle.gen_keys(77, 0, 1)
+    val = np.array([9], dtype=np.uint8)
+    st_o = o.bulk_insert(key, val, with_delete=with_delete)
+    st_g, fails = Ops(g).insert(key, val, with_delete=with_delete)
+    assert st_o[0] == 1 and st_g[0] == 1 and fails == 0
+    assert g.level_bytes(0) == o.level_bytes(0)
+    lvl = np.frombuffer(g.level_bytes(0), dtype=np.uint16).reshape(nb, B)
+    changed = np.argwhere(lvl != img)
+    assert len(changed) == 1 and changed[0][1] == 12
