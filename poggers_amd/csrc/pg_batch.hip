@@ -354,6 +354,8 @@ int pg_batch_create(const pg_desc *proto, uint32_t ntables, const uint64_t *nslo
     // lay out every table's levels in one arena
     uint64_t total = 0;
     std::vector<uint64_t> offs((size_t)ntables * PG_MAX_LEVELS, 0);
+    // occupancy index of each linear backing level, after its slots (~0: none)
+    std::vector<uint64_t> occ_offs((size_t)ntables * PG_MAX_LEVELS, ~uint64_t(0));
     for (uint32_t t = 0; t < ntables; t++) {
         if (nslots[t] == 0) {
             delete b;
@@ -368,6 +370,10 @@ int pg_batch_create(const pg_desc *proto, uint32_t ntables, const uint64_t *nslo
             offs[(size_t)t * PG_MAX_LEVELS + l] = total;
             b->level_bytes[(size_t)t * PG_MAX_LEVELS + l] = bytes;
             total += align256(bytes);
+            if (pg::has_occ_index(proto->rep, proto->level[l].scheme)) {
+                occ_offs[(size_t)t * PG_MAX_LEVELS + l] = total;
+                total += pg::occ_bytes(nb);
+            }
             pg::TableDev &T = b->h_tables[t];
             pg::LevelDev &L = T.lv[l];
             L.nb = pg::make_fastmod(nb);
@@ -391,8 +397,11 @@ int pg_batch_create(const pg_desc *proto, uint32_t ntables, const uint64_t *nslo
     if (hipMalloc(&b->arena, total) != hipSuccess) rc = PG_ERR_NOMEM;
     if (rc == PG_OK) {
         for (uint32_t t = 0; t < ntables; t++)
-            for (uint32_t l = 0; l < proto->num_levels; l++)
-                b->h_tables[t].lv[l].slots = (char *)b->arena + offs[(size_t)t * PG_MAX_LEVELS + l];
+            for (uint32_t l = 0; l < proto->num_levels; l++) {
+                const size_t i = (size_t)t * PG_MAX_LEVELS + l;
+                b->h_tables[t].lv[l].slots = (char *)b->arena + offs[i];
+                if (occ_offs[i] != ~uint64_t(0)) b->h_tables[t].lv[l].occ = (uint32_t *)((char *)b->arena + occ_offs[i]);
+            }
         if (hipMalloc((void **)&b->d_tables, sizeof(pg::TableDev) * ntables) != hipSuccess ||
             hipMalloc((void **)&b->d_counter, 256) != hipSuccess ||
             hipMalloc((void **)&b->d_fill, 256) != hipSuccess)
