@@ -1,5 +1,7 @@
 """GPU parity of the batched many-filter path (pg_batch_*): each table of a
 batch behaves exactly like a single table built from the same shape."""
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -364,7 +366,9 @@ def _random_shapes(count, seed=77):
     return out
 
 
-RANDOM_SHAPES = _random_shapes(24)
+# PG_RANDOM_BATCH_SHAPES and PG_RANDOM_SEED widen a run (default 24 and 77)
+RANDOM_SHAPES = _random_shapes(int(os.environ.get("PG_RANDOM_BATCH_SHAPES", "24")),
+                               seed=int(os.environ.get("PG_RANDOM_SEED", "77")))
 
 
 @pytest.mark.parametrize("k", range(len(RANDOM_SHAPES)))
