@@ -370,7 +370,7 @@ int pg_batch_create(const pg_desc *proto, uint32_t ntables, const uint64_t *nslo
             offs[(size_t)t * PG_MAX_LEVELS + l] = total;
             b->level_bytes[(size_t)t * PG_MAX_LEVELS + l] = bytes;
             total += align256(bytes);
-            if (pg::has_occ_index(proto->rep, proto->level[l].scheme)) {
+            if (pg::has_occ_index(proto->rep, proto->level[l].scheme, l)) {
                 occ_offs[(size_t)t * PG_MAX_LEVELS + l] = total;
                 total += pg::occ_bytes(nb);
             }

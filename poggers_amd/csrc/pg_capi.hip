@@ -162,7 +162,7 @@ int pg_create(const pg_desc *desc, uint64_t seed, pg_handle *out) {
             break;
         }
         rc = pg::launch_zero(L.slots, bytes, nullptr);
-        if (rc == PG_OK && pg::has_occ_index(desc->rep, l.scheme)) {
+        if (rc == PG_OK && pg::has_occ_index(desc->rep, l.scheme, i)) {
             if (hipMalloc((void **)&L.occ, pg::occ_bytes(nb)) != hipSuccess) {
                 L.occ = nullptr;
                 rc = PG_ERR_NOMEM;

@@ -115,20 +115,22 @@ struct LevelDev {
     uint32_t group;      // GROUPED_SHORTCUT: buckets per 128-byte line, else 0
     uint32_t bucket_bytes;
     uint32_t pad;
-    // Occupancy index of a linear filter level (has_occ_index), else null:
-    // bit b of occ[b / 32] is set before any claim in bucket b and cleared
-    // only with the slots, so a clear bit proves the bucket empty.
+    // Occupancy index of a backing level (has_occ_index), else null: bit b
+    // of occ[b / 32] is set before any claim in bucket b and cleared only
+    // with the slots, so a clear bit proves the bucket empty.
     uint32_t *occ = nullptr;
 };
 
-// A linear-bucket level of 16/16 or 8/8 filter slots keeps an occupancy
-// index.  Its query and remove stop at the first probe bucket whose fill is
-// not B (linear_insert_buckets.cuh:323-398); an empty bucket has fill 0 and,
-// since these fingerprints are never 0, no match, so a clear bit answers for
-// the bucket without reading it.  A 10/6 fingerprint can match an empty word,
-// so 10/6 levels read every bucket.
-__host__ __device__ inline bool has_occ_index(uint32_t rep, uint32_t scheme) {
-    return scheme == PG_SCHEME_LINEAR_BUCKETS && (rep == PG_REP_FP16_V16 || rep == PG_REP_FP8_V8);
+// A linear-bucket backing level (level >= 1, which only keys that spill
+// reach) of 16/16 or 8/8 filter slots keeps an occupancy index.  Its query
+// and remove stop at the first probe bucket whose fill is not B
+// (linear_insert_buckets.cuh:323-398); an empty bucket has fill 0 and, since
+// these fingerprints are never 0, no match, so a clear bit answers for the
+// bucket without reading it.  A 10/6 fingerprint can match an empty word, so
+// 10/6 levels read every bucket.  A primary level keeps none: every insert
+// there would pay the extra atomic.
+__host__ __device__ inline bool has_occ_index(uint32_t rep, uint32_t scheme, uint32_t level) {
+    return level > 0 && scheme == PG_SCHEME_LINEAR_BUCKETS && (rep == PG_REP_FP16_V16 || rep == PG_REP_FP8_V8);
 }
 __host__ __device__ inline uint64_t occ_bytes(uint64_t nb) { return ((nb + 31) / 32 * 4 + 255) & ~uint64_t(255); }
 

@@ -149,7 +149,7 @@ __global__ __launch_bounds__(kBlock) void k_occ_rebuild(const W *slots, uint64_t
 }
 
 int launch_occ_rebuild(const LevelDev &L, uint32_t rep, hipStream_t s) {
-    if (!L.occ || !has_occ_index(rep, L.scheme)) return PG_ERR_INVALID;
+    if (!L.occ || (rep != PG_REP_FP16_V16 && rep != PG_REP_FP8_V8)) return PG_ERR_INVALID;
     const uint64_t words = (L.nb.d + 31) / 32;
     if (rep == PG_REP_FP16_V16)
         hipLaunchKernelGGL(k_occ_rebuild<uint32_t>, dim3(stride_grid(words)), dim3(kBlock), 0, s,

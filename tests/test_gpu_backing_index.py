@@ -1,6 +1,6 @@
-"""The occupancy index of linear-bucket filter levels (16/16 and 8/8 slots):
-one bit per bucket, set before any slot of the bucket is claimed and cleared
-only with the slots.  Queries and removes stop at a bucket whose bit is
+"""The occupancy index of linear-bucket backing levels (16/16 and 8/8 filter
+slots): one bit per bucket, set before any slot of the bucket is claimed and
+cleared only with the slots.  Queries and removes stop at a bucket whose bit is
 clear without reading it; the reference's linear walk stops at the first
 bucket that is not full (linear_insert_buckets.cuh:323-398), and an empty
 bucket is not full and holds no match.  The index must never change an
@@ -152,3 +152,4 @@ def test_absent_queries_skip_an_empty_backing_level(rep):
     assert reached > 0.9 * len(absent)
     backing_read = int(stats[1].item())
     assert backing_read == (reached if rep == F10 else 0), (backing_read, reached)
+

@@ -16,6 +16,7 @@ Concurrently: failures stay near the key-order count, every stored key is
 found with its value and fill equals the stored count."""
 import numpy as np
 import pytest
+import torch
 
 import oracle
 import poggers_amd as pg
@@ -229,3 +230,61 @@ def test_concurrent_large_failures_near_key_order(name, nslots, load):
     st, fails = Ops(g).insert(keys, vals)
     assert fails == int((st == 0).sum()) and g.get_fill() == int((st > 0).sum())
     assert fails <= 2 * fails_o + 0.0005 * n, (fails, fails_o)
+
+
+# The flat TCF filled past its primary level, so its bucket_insert backing
+# level takes keys (bucket_insert.cuh:127-357).
+FLAT = {
+    "flat_tile4": pg.flat_tcf_desc(1 << 14, 1 << 9),
+    "flat_tile2": pg.flat_tcf_desc(1 << 14, 1 << 9, tile=2),
+    "flat_tile1": pg.flat_tcf_desc(1 << 14, 1 << 9, tile=1),
+}
+
+
+def kv16_vals(keys, salt):
+    return (((keys >> np.uint64(24)) + np.uint64(salt)) & np.uint64(0xFFFF)).astype(np.uint16)
+
+
+@pytest.mark.parametrize("name", list(FLAT))
+def test_flat_tcf_answers_equal_reference_over_same_slots(name, tmp_path):
+    """Concurrent inserts past the primary level's slots, then concurrent
+    removes: found flags and values equal the oracle's query over the same
+    slot bytes.  From that state, removes and inserts in key order are
+    bit-exact with the oracle.  A saved table answers the same after loading,
+    and a cleared one finds nothing."""
+    d = FLAT[name]
+    g = pg.Table(d, seed=42)
+    ops = Ops(g)
+    keys = oracle.gen_keys(331, 0, int(0.97 * (d.levels[0].nslots + d.levels[1].nslots)))
+    st, _ = ops.insert(keys, kv16_vals(keys, 1))
+    assert (st == 2).sum() > 0, "no key reached the backing level: the case tests nothing"
+    ops.remove(keys[::3])
+    q = np.concatenate([keys, oracle.gen_keys(331, 1 << 41, 1 << 14)])
+    o = oracle.Table(d.rep, [lv.as_dict() for lv in d.levels], seed=42)
+    for lvl in range(2):
+        o.set_level_bytes(lvl, g.level_bytes(lvl))
+    f_o, v_o = o.bulk_query(q)
+    f_g, v_g = ops.query(q)
+    assert (f_g == f_o).all()
+    assert (v_g[f_g > 0] == v_o[f_o > 0].astype(v_g.dtype)).all()
+    g.set_mode(pg.DETERMINISTIC)
+    rm = np.concatenate([q[::7], q[-1000:]])
+    assert (ops.remove(rm) == o.bulk_remove(rm)).all()
+    again = oracle.gen_keys(332, 0, 3000)
+    va = kv16_vals(again, 2)
+    st_g, _ = ops.insert(again, va)
+    assert (st_g == o.bulk_insert(again, va.astype(o.val_dtype))).all()
+    for lvl in range(2):
+        assert g.level_bytes(lvl) == o.level_bytes(lvl), lvl
+    f_o, v_o = o.bulk_query(q)
+    f_g, v_g = ops.query(q)
+    assert (f_g == f_o).all() and (v_g == v_o.astype(v_g.dtype)).all()
+    path = str(tmp_path / "flat.pgt")
+    g.save(path)
+    t2 = pg.Table.load(path)
+    f2, v2 = Ops(t2).query(q)
+    assert (f2 == f_g).all() and (v2 == v_g).all()
+    t2.clear()
+    torch.cuda.synchronize()
+    f3, _ = Ops(t2).query(q)
+    assert not f3.any()
