@@ -317,12 +317,13 @@ int pg_num_slots(pg_handle h, uint64_t *out);
 int pg_level_ptr(pg_handle h, uint32_t level, void **d_ptr, uint64_t *bytes);
 
 /* Rebuild what a level derives from its slots, after they were written
- * through pg_level_ptr (pg_load does this itself).  A linear-bucket level of
- * 16/16 or 8/8 filter slots keeps an occupancy index: one bit per bucket,
- * set before any slot of the bucket is claimed and cleared only by
- * pg_clear, so a clear bit proves the bucket empty and its queries and
- * removes stop there without reading it (the reference's linear walk stops
- * at the first bucket that is not full, linear_insert_buckets.cuh:323-398).
+ * through pg_level_ptr (pg_load does this itself).  A linear-bucket backing
+ * level (level >= 1) of 16/16 or 8/8 filter slots keeps an occupancy index:
+ * one bit per bucket, set before any slot of the bucket is claimed and
+ * cleared only by pg_clear, so a clear bit proves the bucket empty and its
+ * queries and removes stop there without reading it (the reference's linear
+ * walk stops at the first bucket that is not full,
+ * linear_insert_buckets.cuh:323-398).
  * The rebuild sets the bit of every bucket holding a non-empty word.  PG_OK
  * and nothing to do for other levels.  Ordered on `stream`.  No reference
  * counterpart. */
