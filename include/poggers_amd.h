@@ -367,7 +367,12 @@ int pg_batch_capture(pg_batch b, const uint64_t *d_ins_keys, const uint64_t *d_i
 int pg_batch_replay(pg_batch b, void *stream);
 /* Filled slots of one table (synchronises; *out is a host pointer). */
 int pg_batch_fill(pg_batch b, uint32_t table, uint64_t *out, void *stream);
+/* Device pointer and byte size of one table's level, for reading layouts.
+ * The backing levels keep the occupancy index described at pg_level_sync;
+ * a batch has no sync call, so slots written through this pointer would
+ * leave the index stale.  Write a batch's tables only through its ops. */
 int pg_batch_level_ptr(pg_batch b, uint32_t table, uint32_t level, void **d_ptr, uint64_t *bytes);
+/* Bytes of slot memory over every table and level. */
 int pg_batch_bytes(pg_batch b, uint64_t *out);
 
 /* ---- serving: continuous batching over resident tables ----
