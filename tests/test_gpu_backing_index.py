@@ -153,3 +153,21 @@ def test_absent_queries_skip_an_empty_backing_level(rep):
     backing_read = int(stats[1].item())
     assert backing_read == (reached if rep == F10 else 0), (backing_read, reached)
 
+
+
+def test_a_linear_primary_level_keeps_no_index():
+    """Only backing levels keep the index: on a primary level every insert
+    would pay its atomic OR.  An empty one-level linear 16/16 filter
+    therefore reads the first probe bucket of every absent key (one bucket
+    each: it is empty, so the walk stops there), where an indexed level
+    would read none."""
+    d = pg.TableDesc(F16, [pg.Level(1 << 12, pg.SCHEME_LINEAR_BUCKETS, 8, 4, pg.PROBE_LINEAR, pg.HASH_FMIX64,
+                                    4000)])
+    g = pg.Table(d, seed=42)
+    absent = oracle.gen_keys(341, 0, 1 << 12)
+    stats = torch.zeros(3, dtype=torch.int64, device="cuda")
+    g.set_stats(stats)
+    f, _ = Ops(g).query(absent)
+    g.set_stats(None)
+    assert not f.any()
+    assert int(stats[0].item()) == len(absent)
