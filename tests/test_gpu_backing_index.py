@@ -129,7 +129,12 @@ def test_load_clear_and_upload_keep_the_index(name, tmp_path):
     keys = q[: len(q) // 2]
     st, _ = Ops(t2).insert(keys, vals_for(d, keys, 4))
     f5, _ = Ops(t2).query(keys[st > 0])
-    assert f5.all()
+    # equal to the reference's query over the same slot bytes; 10/6 slots may
+    # hide a stored key (a key packing to the empty word, or an empty-looking
+    # word in a full linear bucket, where the walk stops)
+    f_o, _ = oracle_on(d, t2).bulk_query(keys[st > 0])
+    assert (f5 == f_o).all()
+    assert f5.all() or d.rep == F10
 
 
 @pytest.mark.parametrize("rep", [F16, F8, F10])
