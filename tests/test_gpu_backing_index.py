@@ -139,14 +139,19 @@ def test_load_clear_and_upload_keep_the_index(name, tmp_path):
 
 @pytest.mark.parametrize("rep", [F16, F8, F10])
 def test_absent_queries_skip_an_empty_backing_level(rep):
-    """Half full: no key leaves the primary level, so the backing level is
-    empty.  Absent keys that reach it examine no backing bucket where the
-    index exists, one bucket each where it does not (10/6)."""
+    """Half full, filled in key order: no key leaves the primary level (as
+    in the oracle), so the backing level is empty.  (With every key of this
+    small table in one concurrent launch, a bounded claim may still pass a
+    key on.)  Absent keys that reach the backing level examine no backing
+    bucket where the index exists, one bucket each where it does not (10/6)."""
     d = desc(rep, 4, 8, nslots=1 << 16, backing=1 << 10)
     o, g = pair(d)
+    g.set_mode(pg.DETERMINISTIC)
     ops = Ops(g)
     keys = oracle.gen_keys(321, 0, 1 << 15)
-    st, _ = ops.insert(keys, vals_for(d, keys, 1))
+    vs = vals_for(d, keys, 1)
+    st, _ = ops.insert(keys, vs)
+    assert (st == o.bulk_insert(keys, vs)).all()
     assert (st == 1).all(), "a key left the primary level"
     absent = oracle.gen_keys(321, 1 << 42, 1 << 14)
     stats = torch.zeros(3, dtype=torch.int64, device="cuda")
@@ -157,7 +162,6 @@ def test_absent_queries_skip_an_empty_backing_level(rep):
     assert reached > 0.9 * len(absent)
     backing_read = int(stats[1].item())
     assert backing_read == (reached if rep == F10 else 0), (backing_read, reached)
-
 
 
 def test_a_linear_primary_level_keeps_no_index():
